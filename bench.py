@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Agent-updates/s of the flockers agent step on one MI355X.
+
+Workload (the plain GPU drop-in): 1,048,576 boids on a 20,480 x 20,480
+toroidal Field2D, discretization 10/1.5, neighbour radius 10, all weights 1,
+jump 0.7, positions uniform and headings unit vectors at a uniform angle
+from the seeded Philox stream.  One step = one Schedule::step of the
+flockers model = the fused query + Bird::step kernel and the rebinning
+(lazy_update) of all agents.  State is resident in HBM before timing starts.
+
+Prints one JSON line (see the keys below).  `python bench.py --gpus N` with
+N > 1 exits: this build measures one GPU.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "agent-updates/s (1M flockers f32) & ensemble p50/p99 latency, batched vs plain"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+# algorithmic bytes per agent and launch (f32 x, y, hx, hy + u32 id)
+BYTES_FLOCK = 20 + 16    # read sorted state, write new x, y, hx, hy
+BYTES_SCATTER = 20 + 20  # read new state + id, write binned state + id
+BYTES_STEP = BYTES_FLOCK + BYTES_SCATTER
+
+
+class Hip:
+    """The few HIP runtime calls the timing needs."""
+
+    def __init__(self):
+        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.L.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.L.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                               ctypes.c_void_p]
+        self.L.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed with HIP error {rc}")
+
+    def sync(self):
+        self._ok(self.L.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    def event(self):
+        e = ctypes.c_void_p()
+        self._ok(self.L.hipEventCreate(ctypes.byref(e)), "hipEventCreate")
+        return e
+
+    def record(self, e, stream):
+        self._ok(self.L.hipEventRecord(e, ctypes.c_void_p(stream)), "hipEventRecord")
+
+    def elapsed_ms(self, a, b):
+        self._ok(self.L.hipEventSynchronize(b), "hipEventSynchronize")
+        ms = ctypes.c_float()
+        self._ok(self.L.hipEventElapsedTime(ctypes.byref(ms), a, b), "hipEventElapsedTime")
+        return ms.value
+
+
+def plain_workload():
+    return dict(n=1 << 20, side=20480.0, d=10.0 / 1.5, radius=10.0, seed=1)
+
+
+def run_plain(args, hip):
+    from krabmaga_amd import Field2D, FlockParams
+    w = plain_workload()
+    n, side = w["n"], w["side"]
+    p = FlockParams(radius=w["radius"], seed=w["seed"])
+    f = Field2D(side, side, w["d"], True, n)
+    f.init_uniform(n, w["seed"], 1)
+    f.schedule_run(p, 0, args.warmup)  # step 0 includes the first lazy_update
+    f.sync()
+    print(f"[bench] plain warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+
+    # timed region: K schedule steps replayed from a HIP graph
+    e0, e1 = hip.event(), hip.event()
+    hip.sync()
+    t0 = time.perf_counter()
+    hip.record(e0, f.stream)
+    f.schedule_run(p, args.warmup, args.steps)
+    hip.record(e1, f.stream)
+    ms = hip.elapsed_ms(e0, e1)
+    hip.sync()
+    wall = time.perf_counter() - t0
+    f.sync()
+    print(f"[bench] plain timed region: {ms:.3f} ms for {args.steps} steps", file=sys.stderr,
+          flush=True)
+
+    # per-kernel durations: the same steps launched directly, each kernel
+    # bracketed by HIP events on the field's stream
+    kprof = min(args.steps, args.profile_steps)
+    f.profile(True)
+    f.schedule_run(p, args.warmup + args.steps, kprof, use_graph=False)
+    kern = {}
+    for k in ("flock", "scan", "scatter"):
+        tot, cnt = f.profile_read(k)
+        kern[k] = {"avg_us": 1e3 * tot / max(cnt, 1), "launches": cnt}
+    f.profile(False)
+    f.close()
+    return w, ms, wall, kern
+
+
+def cpu_baseline(args):
+    """The oracle (single thread) on a bounded sample of the same workload."""
+    from oracle import oracle as O
+    w = plain_workload()
+    n, side = w["n"], w["side"]
+    x, y, hx, hy = O.init_uniform(n, side, side, w["seed"], 1)
+    secs, _ = O.time_steps(side, side, w["d"], True, x, y, hx, hy,
+                           O.params(radius=w["radius"], seed=w["seed"]), args.cpu_steps)
+    return {
+        "value": n * args.cpu_steps / secs,
+        "unit": "agent-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{args.cpu_steps} schedule steps of the same 1,048,576-boid run "
+                  f"(oracle restatement of Field2D + Bird::step, {secs:.1f} s)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    hip = Hip()
+    w, ms, wall, kern = run_plain(args, hip)
+    n, steps = w["n"], args.steps
+    value = n * steps / (ms * 1e-3)
+    dom = max(("flock", "scatter"), key=lambda k: kern[k]["avg_us"])
+    per_launch = {"flock": BYTES_FLOCK * n, "scatter": BYTES_SCATTER * n}[dom]
+    achieved = per_launch / (kern[dom]["avg_us"] * 1e-6)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "agent-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: Philox-seeded uniform positions, unit headings at uniform angle",
+        "config": {
+            "workload": "flockers plain GPU drop-in: 1,048,576 boids, 20,480 x 20,480 torus, "
+                        "discretization 10/1.5, radius 10, weights 1, jump 0.7",
+            "agents": n, "side": w["side"], "seed": w["seed"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": None,
+        },
+        "step_roofline": {
+            "bytes_per_agent_update": BYTES_STEP,
+            "achieved_GBps": BYTES_STEP * value / 1e9,
+            "frac": BYTES_STEP * value / HBM_PEAK,
+        },
+        "kernels_us": {k: round(v["avg_us"], 3) for k, v in kern.items()},
+        "wall_s_timed_region": wall,
+    }
+    if not args.no_cpu:
+        print("[bench] CPU baseline (oracle, 1 thread) ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

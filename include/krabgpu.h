@@ -1,0 +1,179 @@
+/*
+ * krabgpu.h -- C ABI of the MI355X (gfx950) agent-step engine for krABMaga's
+ * continuous 2-D field and flockers model.
+ *
+ * Every entry point returns KG_OK (0) or a negative KG_E_* status, never
+ * unwinds across the boundary, and records a message retrievable with
+ * kg_last_error() on the calling thread.  Host pointers are borrowed for the
+ * duration of a call; device memory is owned by the library objects.
+ *
+ * A kg_field is one Field2D<Bird> (src/engine/fields/field_2d.rs:267-923)
+ * living in HBM: a read buffer and a write buffer of agents, each agent an
+ * id (u32), a position (Real2D, f32) and a heading (the flockers model's
+ * last_d, f32).  One host thread uses a kg_field at a time; distinct fields
+ * may be used concurrently.  Each kg_field owns one HIP stream and all its
+ * work is ordered on it.
+ *
+ * Ordering contract.  Inside a bag the reference keeps insertion order,
+ * which for a schedule step is the scheduler's pop order.  This library keeps
+ * every bag sorted by agent id, i.e. the order a scheduler stepping agents in
+ * ascending id produces; neighbour queries return agents in bag-scan order
+ * (cell x ascending, then cell y ascending, then id ascending), as
+ * field_2d.rs:418-435 does for such a run.
+ */
+#ifndef KRABGPU_H
+#define KRABGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KG_OK 0
+#define KG_E_INVALID (-1)  /* bad argument */
+#define KG_E_CAPACITY (-2) /* caller buffer or field capacity too small; the
+                              required size is reported where the call says */
+#define KG_E_RANGE (-3)    /* a location discretizes outside the bag grid
+                              (the reference indexes out of bounds and panics) */
+#define KG_E_HIP (-4)      /* HIP runtime error */
+#define KG_E_NOMEM (-5)    /* device or host allocation failed */
+#define KG_E_STATE (-6)    /* call not valid in the field's current state */
+#define KG_E_ADMIT (-7)    /* a run does not fit the engine's memory budget */
+
+typedef struct kg_field kg_field;
+
+typedef struct kg_field_desc {
+    float width;          /* Field2D::new(w, ...) */
+    float height;         /* Field2D::new(_, h, ...) */
+    float discretization; /* Field2D::new(_, _, d, _) */
+    int32_t toroidal;     /* Field2D::new(_, _, _, t) */
+    uint32_t capacity;    /* maximum agents per buffer */
+    uint32_t bin_rows;    /* device bin height in cells; 0 = automatic */
+} kg_field_desc;
+
+/* Weights of the flockers step (tests/model/flockers/bird.rs:12-17) and the
+ * neighbour radius (bird.rs:41).  `seed` keys the Philox4x32-10 stream that
+ * replaces rand::rng(): agent `id` at schedule step `s` draws the first two
+ * words of Philox(ctr = {id, s_lo, s_hi, 0x53544550}, key = seed) mapped to
+ * [0,1) by (u >> 8) * 2^-24. */
+typedef struct kg_flock_params {
+    float cohesion;
+    float avoidance;
+    float consistency;
+    float randomness;
+    float momentum;
+    float jump;
+    float radius;
+    uint32_t reserved; /* must be 0 */
+    uint64_t seed;
+} kg_flock_params;
+
+/* Field2D::new (field_2d.rs:304-322). */
+int kg_field_create(const kg_field_desc *desc, kg_field **out);
+void kg_field_destroy(kg_field *f);
+
+/* Field2D::set_object_location (field_2d.rs:838-846), in bulk: appends n
+ * agents to the write buffer.  Returns KG_E_CAPACITY if the write buffer
+ * would exceed the capacity, KG_E_RANGE if a location falls outside the bag
+ * grid (nothing is appended then). */
+int kg_set_object_locations(kg_field *f, const uint32_t *ids, const float *x, const float *y,
+                            const float *hx, const float *hy, size_t n);
+
+/* Flocker::init (tests/model/flockers/state.rs:41-56) on the device: appends
+ * agents 0..n-1 with pos = (w, h) * U[0,1)^2 drawn from Philox(ctr = {id, k,
+ * 0, 0x494E4954}, key = seed) and heading (0,0) (heading_mode 0) or a unit
+ * vector at a uniform angle (heading_mode 1). */
+int kg_init_uniform(kg_field *f, uint32_t n, uint64_t seed, int heading_mode);
+
+/* Field2D::remove_object_location (field_2d.rs:885-898): removes from the
+ * write buffer every agent with this id in the bag of (x, y). */
+int kg_remove_object_location(kg_field *f, uint32_t id, float x, float y);
+
+/* Field::lazy_update (field_2d.rs:905-921): the write buffer becomes the read
+ * buffer (binned on the device) and the write buffer is emptied. */
+int kg_lazy_update(kg_field *f);
+
+/* Field2D::get_neighbors_within_distance (field_2d.rs:386-440) and
+ * get_neighbors_within_relax_distance (field_2d.rs:472-516) on the read
+ * buffer.  Writes up to cap ids; *n_out receives the full count and
+ * KG_E_CAPACITY is returned if it exceeds cap. */
+int kg_get_neighbors_within_distance(kg_field *f, float x, float y, float dist, uint32_t *ids,
+                                     size_t cap, size_t *n_out);
+int kg_get_neighbors_within_relax_distance(kg_field *f, float x, float y, float dist,
+                                           uint32_t *ids, size_t cap, size_t *n_out);
+
+/* Field2D::get_objects / get_objects_unbuffered (field_2d.rs:544-582): the
+ * agents in the bag of (x, y) in the read (unbuffered = 0) or write
+ * (unbuffered = 1) buffer.  Same capacity contract as above. */
+int kg_get_objects(kg_field *f, float x, float y, int unbuffered, uint32_t *ids, size_t cap,
+                   size_t *n_out);
+
+/* Field2D::num_objects_at_location (field_2d.rs:803-808). */
+int kg_num_objects_at_location(kg_field *f, float x, float y, size_t *n);
+
+/* Field2D::get_empty_bags (field_2d.rs:727-739), count only. */
+int kg_num_empty_bags(kg_field *f, size_t *n);
+
+/* Field2D::nagents (field_2d.rs:277): set_object_location calls counted
+ * until the first lazy_update. */
+int kg_nagents(kg_field *f, size_t *n);
+
+/* Number of agents in the read (buffer 0) or write (buffer 1) buffer. */
+int kg_count(kg_field *f, int buffer, size_t *n);
+
+/* get_neighbors_within_distance for every agent of the read buffer at once,
+ * as CSR over the read buffer's storage order (see kg_read_agents):
+ * row_ptr has count+1 entries, ids receives the neighbour ids of each row in
+ * bag-scan order.  *total receives the number of ids; KG_E_CAPACITY if it
+ * exceeds cap (row_ptr is still filled). */
+int kg_neighbor_sets(kg_field *f, float dist, uint32_t *row_ptr, uint32_t *ids, size_t cap,
+                     size_t *total);
+
+/* Bird::step (bird.rs:38-154) for every agent of the read buffer, i.e. the
+ * agent loop of Schedule::step (src/engine/schedule.rs:395-408) at schedule
+ * step `step`: reads the read buffer, writes the write buffer.  The write
+ * buffer must be empty. */
+int kg_flock_step(kg_field *f, const kg_flock_params *p, uint64_t step);
+
+/* The simulate! loop (src/lib.rs:1163-1173) for a flockers state:
+ * `nsteps` calls of Schedule::step starting at schedule step `first_step`,
+ * each = [lazy_update if the step is 0] + kg_flock_step + lazy_update
+ * (schedule.rs:349-351, 395-412).  With use_graph != 0 one step is captured
+ * in a HIP graph and replayed.  Asynchronous: call kg_field_sync to wait. */
+int kg_schedule_run(kg_field *f, const kg_flock_params *p, uint64_t first_step, uint32_t nsteps,
+                    int use_graph);
+
+/* Copies a buffer (0 = read, 1 = write) to the host in storage order.  Any
+ * output pointer may be NULL.  *n receives the agent count; KG_E_CAPACITY
+ * if it exceeds cap. */
+int kg_read_agents(kg_field *f, int buffer, uint32_t *ids, float *x, float *y, float *hx,
+                   float *hy, size_t cap, size_t *n);
+
+/* Waits for all work queued on the field's stream. */
+int kg_field_sync(kg_field *f);
+
+/* The field's hipStream_t, for callers that time or order work around it. */
+void *kg_field_stream(kg_field *f);
+
+/* Instrumentation (no counterpart in the reference): when enabled, every
+ * kernel launch of the field is bracketed by HIP events on the field's
+ * stream and its duration accumulated per kernel; graph replay is disabled
+ * meanwhile.  kg_field_profile(f, on) also resets the totals. */
+#define KG_KERNEL_FLOCK 0          /* fused query + Bird::step */
+#define KG_KERNEL_SCAN 1           /* bin offsets */
+#define KG_KERNEL_SCATTER 2        /* rebinning of the flock output */
+#define KG_KERNEL_COUNT_APPEND 3   /* histogram of set_object_location agents */
+#define KG_KERNEL_SCATTER_APPEND 4 /* binning of set_object_location agents */
+#define KG_KERNEL_COUNT 5
+int kg_field_profile(kg_field *f, int enable);
+int kg_field_profile_read(kg_field *f, int kernel, double *total_ms, uint64_t *launches);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *kg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,34 @@
+// kg_common.hpp -- host-side helpers shared by the C ABI implementations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/krabgpu.h"
+#include "kg_geom.hpp"
+#include "kg_launch.hpp"
+
+namespace kg {
+
+// Records the message of the failing call for kg_last_error (per thread).
+int set_error(int code, const char* fmt, ...);
+void clear_error();
+
+#define KG_HIP(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return ::kg::set_error(KG_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+hipError_t dmalloc(T** p, size_t n) {
+  return hipMalloc((void**)p, (n ? n : 1) * sizeof(T));
+}
+
+}  // namespace kg

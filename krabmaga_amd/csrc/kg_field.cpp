@@ -1,0 +1,694 @@
+// kg_field.cpp -- C ABI of one device-resident Field2D<Bird> (krabgpu.h).
+//
+// A kg_field is a replica table of one: the same kernels drive the batched
+// ensemble.  Buffers (capacity `cap` agents each, SoA):
+//   R, R2  read buffer and the rebinning target (swapped by lazy_update)
+//   W      write buffer produced by the flock kernel, aligned with R
+//   A      write buffer filled by set_object_location / init, insertion order
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "kg_common.hpp"
+
+using namespace kg;
+
+namespace {
+
+struct DevBuf {
+  Buf b{};
+  hipError_t alloc(uint32_t cap) {
+    hipError_t e;
+    if ((e = dmalloc(&b.x, cap)) != hipSuccess) return e;
+    if ((e = dmalloc(&b.y, cap)) != hipSuccess) return e;
+    if ((e = dmalloc(&b.hx, cap)) != hipSuccess) return e;
+    if ((e = dmalloc(&b.hy, cap)) != hipSuccess) return e;
+    return dmalloc(&b.id, cap);
+  }
+  void release() {
+    (void)hipFree(b.x);
+    (void)hipFree(b.y);
+    (void)hipFree(b.hx);
+    (void)hipFree(b.hy);
+    (void)hipFree(b.id);
+    b = Buf{};
+  }
+};
+
+struct GraphKey {
+  uint8_t params[sizeof(kg_flock_params)];
+  int parity;
+  bool operator<(const GraphKey& o) const {
+    if (parity != o.parity) return parity < o.parity;
+    return std::memcmp(params, o.params, sizeof(params)) < 0;
+  }
+};
+
+}  // namespace
+
+struct kg_field {
+  kg_field_desc desc{};
+  Geom g{};
+  uint32_t cap = 0;
+  int lds_cap = 1024;
+  hipStream_t stream = nullptr;
+
+  DevBuf R, R2, W, A;
+  int parity = 0;  // flips with every rebinning (R <-> R2)
+  uint32_t *count = nullptr, *offset = nullptr, *offset2 = nullptr, *cursor = nullptr;
+  ScanState ss{};
+  uint32_t* err = nullptr;
+  uint64_t* gstep = nullptr;
+  uint64_t* zero64 = nullptr;
+  uint32_t* prefix = nullptr;  // [0..1] tile prefix, [2..3] append prefix
+  RepDev* d_rep = nullptr;
+  uint32_t* scratch = nullptr;  // small query outputs
+  RepDev h_rep{};
+
+  uint32_t nR = 0, nW = 0, nA = 0;
+  bool w_aligned = false;
+  bool poisoned = false;
+
+  // Field2D bookkeeping (field_2d.rs:277, :290-293)
+  size_t nagents = 0;
+  bool dens_check = false;
+  size_t density_estimation = 0;
+
+  std::map<GraphKey, hipGraphExec_t> graphs;
+
+  // HIP-event timing of individual kernel launches (kg_field_profile)
+  bool prof_on = false;
+  bool capturing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  double prof_ms[KG_KERNEL_COUNT] = {};
+  uint64_t prof_n[KG_KERNEL_COUNT] = {};
+
+  WorkList worklist() const {
+    WorkList wl;
+    wl.reps = d_rep;
+    wl.nreps = 1;
+    wl.tile_prefix = prefix;
+    wl.ntiles = g.ntiles;
+    wl.app_prefix = prefix + 2;
+    wl.napp = nA;
+    return wl;
+  }
+};
+
+namespace {
+
+hipEvent_t take_event(kg_field* f) {
+  if (!f->ev_pool.empty()) {
+    hipEvent_t e = f->ev_pool.back();
+    f->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one kernel launch with events on the field's stream when
+// profiling is on (never inside a graph capture).
+struct ProfScope {
+  kg_field* f;
+  int kind;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(kg_field* f_, int kind_) : f(f_), kind(kind_) {
+    if (!f->prof_on || f->capturing) return;
+    a = take_event(f);
+    b = take_event(f);
+    if (a) (void)hipEventRecord(a, f->stream);
+  }
+  ~ProfScope() {
+    if (!a || !b) return;
+    (void)hipEventRecord(b, f->stream);
+    f->ev_pending.push_back({kind, {a, b}});
+  }
+};
+
+int prof_collect(kg_field* f) {
+  if (f->ev_pending.empty()) return KG_OK;
+  KG_HIP(hipStreamSynchronize(f->stream));
+  for (auto& pe : f->ev_pending) {
+    float ms = 0.f;
+    KG_HIP(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+    f->prof_ms[pe.first] += ms;
+    f->prof_n[pe.first] += 1;
+    f->ev_pool.push_back(pe.second.first);
+    f->ev_pool.push_back(pe.second.second);
+  }
+  f->ev_pending.clear();
+  return KG_OK;
+}
+
+// Keeps the flock kernel's LDS (dynamic staging + about 4 KB of static
+// tables) within the device's per-block limit.
+int clamp_lds_cap(int cap) {
+  int dev = 0, max_lds = 65536;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        v > 0)
+      max_lds = v;
+  }
+  const size_t budget = (size_t)max_lds - 8192;
+  while (cap > 64 && flock_lds_bytes(cap) > budget) cap -= 64;
+  return cap;
+}
+
+int check_poison(kg_field* f) {
+  if (f->poisoned)
+    return set_error(KG_E_RANGE,
+                     "a location fell outside the bag grid in an earlier step; the field is "
+                     "unusable");
+  return KG_OK;
+}
+
+// Reads the device error flag (synchronises the stream).
+int sync_and_check(kg_field* f) {
+  KG_HIP(hipStreamSynchronize(f->stream));
+  uint32_t e = 0;
+  KG_HIP(hipMemcpy(&e, f->err, sizeof(e), hipMemcpyDeviceToHost));
+  if (e) f->poisoned = true;
+  return check_poison(f);
+}
+
+// Stream-ordered update of the device replica descriptor (a kernel argument
+// copy, so no host buffer has to outlive the call).
+int upload_rep(kg_field* f) {
+  KG_HIP(launch_set_rep(f->d_rep, f->h_rep, f->stream));
+  return KG_OK;
+}
+
+// Field::lazy_update (field_2d.rs:905-921): bins the write buffer into a
+// fresh read buffer.  `step_gstep` advances the schedule-step counter.
+int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
+  WorkList wl = f->worklist();
+  // one replica: the append work list needs no device prefix
+  if (f->nA) {
+    ProfScope ps(f, KG_KERNEL_COUNT_APPEND);
+    KG_HIP(launch_count_append(wl, f->A.b, f->count, f->err, f->stream));
+  }
+  {
+    ProfScope ps(f, KG_KERNEL_SCAN);
+    KG_HIP(launch_scan(f->count, f->offset2, f->cursor, f->g.nbins, f->ss, step_gstep, f->stream));
+  }
+  if (f->w_aligned) {
+    ProfScope ps(f, KG_KERNEL_SCATTER);
+    KG_HIP(launch_scatter_aligned(wl, f->R.b, f->W.b, f->offset, f->cursor, f->R2.b, f->stream));
+  }
+  if (f->nA) {
+    ProfScope ps(f, KG_KERNEL_SCATTER_APPEND);
+    KG_HIP(launch_scatter_append(wl, f->A.b, f->cursor, f->R2.b, f->stream));
+  }
+  std::swap(f->R, f->R2);
+  std::swap(f->offset, f->offset2);
+  f->parity ^= 1;
+  f->nR = (f->w_aligned ? f->nW : 0) + f->nA;
+  f->nW = 0;
+  f->nA = 0;
+  f->w_aligned = false;
+  if (!f->dens_check) {
+    f->density_estimation = f->nagents / ((size_t)f->g.dw * (size_t)f->g.dh);
+    f->dens_check = true;
+  }
+  return KG_OK;
+}
+
+int flock_impl(kg_field* f, const uint64_t* gstep) {
+  ProfScope ps(f, KG_KERNEL_FLOCK);
+  KG_HIP(launch_flock(f->worklist(), f->R.b, f->offset, f->W.b, f->count, gstep, f->err,
+                      f->lds_cap, f->stream));
+  f->nW = f->nR;
+  f->w_aligned = true;
+  return KG_OK;
+}
+
+int set_params(kg_field* f, const kg_flock_params* p, uint64_t step_base) {
+  std::string why;
+  if (!p) return set_error(KG_E_INVALID, "params is NULL");
+  if (!params_valid(*p, &why)) return set_error(KG_E_INVALID, "%s", why.c_str());
+  set_rep_params(&f->h_rep, *p);
+  f->h_rep.step_base = step_base;
+  return upload_rep(f);
+}
+
+}  // namespace
+
+extern "C" {
+
+int kg_field_create(const kg_field_desc* desc, kg_field** out) {
+  clear_error();
+  if (!desc || !out) return set_error(KG_E_INVALID, "desc and out must not be NULL");
+  *out = nullptr;
+  if (desc->capacity == 0 || desc->capacity > (1u << 31))
+    return set_error(KG_E_INVALID, "capacity must be in [1, 2^31]");
+  if (desc->bin_rows & (desc->bin_rows - 1))
+    return set_error(KG_E_INVALID, "bin_rows must be 0 or a power of two");
+  kg_field* f = new kg_field();
+  f->desc = *desc;
+  f->cap = desc->capacity;
+  std::string why;
+  if (!make_geom(desc->width, desc->height, desc->discretization, desc->toroidal, desc->capacity,
+                 desc->bin_rows, &f->g, &why)) {
+    delete f;
+    return set_error(KG_E_INVALID, "%s", why.c_str());
+  }
+  f->lds_cap = clamp_lds_cap(pick_lds_cap(f->g, desc->capacity));
+  f->h_rep.g = f->g;
+  f->h_rep.bin_base = 0;
+  kg_flock_params dflt{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.7f, 10.0f, 0u, 0ull};
+  set_rep_params(&f->h_rep, dflt);
+
+  auto fail = [&](hipError_t e) {
+    kg_field_destroy(f);
+    return set_error(KG_E_NOMEM, "device allocation failed: %s", hipGetErrorString(e));
+  };
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking)) != hipSuccess) return fail(e);
+  if ((e = f->R.alloc(f->cap)) != hipSuccess) return fail(e);
+  if ((e = f->R2.alloc(f->cap)) != hipSuccess) return fail(e);
+  if ((e = f->W.alloc(f->cap)) != hipSuccess) return fail(e);
+  if ((e = f->A.alloc(f->cap)) != hipSuccess) return fail(e);
+  const uint32_t nb = f->g.nbins;
+  const uint32_t nscan = (nb + kScanTile - 1) / kScanTile;
+  if ((e = dmalloc(&f->count, nb)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->offset, nb + 1)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->offset2, nb + 1)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->cursor, nb)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->ss.status, nscan)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->ss.tile_ctr, 1)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->ss.epoch, 1)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->err, 1)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->gstep, 1)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->zero64, 1)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->prefix, 4)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->d_rep, 1)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->scratch, 4)) != hipSuccess) return fail(e);
+  // one-time clears (never inside a captured graph)
+  const std::pair<void*, size_t> clears[] = {
+      {f->count, sizeof(uint32_t) * nb},          {f->offset, sizeof(uint32_t) * (nb + 1)},
+      {f->offset2, sizeof(uint32_t) * (nb + 1)},  {f->ss.status, sizeof(uint64_t) * nscan},
+      {f->ss.tile_ctr, sizeof(uint32_t)},         {f->ss.epoch, sizeof(uint32_t)},
+      {f->err, sizeof(uint32_t)},                 {f->gstep, sizeof(uint64_t)},
+      {f->zero64, sizeof(uint64_t)}};
+  for (const auto& c : clears)
+    if ((e = hipMemset(c.first, 0, c.second)) != hipSuccess) return fail(e);
+  const uint32_t pre[4] = {0u, f->g.ntiles, 0u, 0u};
+  if ((e = hipMemcpy(f->prefix, pre, sizeof(pre), hipMemcpyHostToDevice)) != hipSuccess) return fail(e);
+  if ((e = hipMemcpy(f->d_rep, &f->h_rep, sizeof(RepDev), hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(e);
+  if ((e = hipDeviceSynchronize()) != hipSuccess) return fail(e);
+  *out = f;
+  return KG_OK;
+}
+
+void kg_field_destroy(kg_field* f) {
+  if (!f) return;
+  if (f->stream) (void)hipStreamSynchronize(f->stream);
+  for (auto& kv : f->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& pe : f->ev_pending) {
+    (void)hipEventDestroy(pe.second.first);
+    (void)hipEventDestroy(pe.second.second);
+  }
+  for (hipEvent_t e : f->ev_pool) (void)hipEventDestroy(e);
+  f->R.release();
+  f->R2.release();
+  f->W.release();
+  f->A.release();
+  (void)hipFree(f->count);
+  (void)hipFree(f->offset);
+  (void)hipFree(f->offset2);
+  (void)hipFree(f->cursor);
+  (void)hipFree(f->ss.status);
+  (void)hipFree(f->ss.tile_ctr);
+  (void)hipFree(f->ss.epoch);
+  (void)hipFree(f->err);
+  (void)hipFree(f->gstep);
+  (void)hipFree(f->zero64);
+  (void)hipFree(f->prefix);
+  (void)hipFree(f->d_rep);
+  (void)hipFree(f->scratch);
+  if (f->stream) (void)hipStreamDestroy(f->stream);
+  delete f;
+}
+
+int kg_set_object_locations(kg_field* f, const uint32_t* ids, const float* x, const float* y,
+                            const float* hx, const float* hy, size_t n) {
+  clear_error();
+  if (!f) return set_error(KG_E_INVALID, "field is NULL");
+  if (int rc = check_poison(f)) return rc;
+  if (n == 0) return KG_OK;
+  if (!ids || !x || !y) return set_error(KG_E_INVALID, "ids, x and y must not be NULL");
+  if (f->w_aligned)
+    return set_error(KG_E_STATE,
+                     "the write buffer holds the output of a flock step; call kg_lazy_update "
+                     "first");
+  if ((uint64_t)f->nA + n > f->cap)
+    return set_error(KG_E_CAPACITY, "write buffer capacity %u exceeded", f->cap);
+  // range check on the host: the reference panics on an out-of-range bag
+  for (size_t i = 0; i < n; ++i) {
+    int32_t cx, cy;
+    discretize(f->g, x[i], y[i], &cx, &cy);
+    if (!(cx >= 0 && cx < f->g.dw && cy >= 0 && cy < f->g.dh))
+      return set_error(KG_E_RANGE, "location (%g, %g) is outside the bag grid", (double)x[i],
+                       (double)y[i]);
+  }
+  const size_t bytes = n * sizeof(float);
+  const uint32_t o = f->nA;
+  KG_HIP(hipMemcpyAsync(f->A.b.x + o, x, bytes, hipMemcpyHostToDevice, f->stream));
+  KG_HIP(hipMemcpyAsync(f->A.b.y + o, y, bytes, hipMemcpyHostToDevice, f->stream));
+  KG_HIP(hipMemcpyAsync(f->A.b.id + o, ids, n * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
+  if (hx) KG_HIP(hipMemcpyAsync(f->A.b.hx + o, hx, bytes, hipMemcpyHostToDevice, f->stream));
+  else KG_HIP(hipMemsetAsync(f->A.b.hx + o, 0, bytes, f->stream));
+  if (hy) KG_HIP(hipMemcpyAsync(f->A.b.hy + o, hy, bytes, hipMemcpyHostToDevice, f->stream));
+  else KG_HIP(hipMemsetAsync(f->A.b.hy + o, 0, bytes, f->stream));
+  KG_HIP(hipStreamSynchronize(f->stream));  // host buffers are only borrowed
+  f->nA += (uint32_t)n;
+  if (!f->dens_check) f->nagents += n;
+  return KG_OK;
+}
+
+int kg_init_uniform(kg_field* f, uint32_t n, uint64_t seed, int heading_mode) {
+  clear_error();
+  if (!f) return set_error(KG_E_INVALID, "field is NULL");
+  if (int rc = check_poison(f)) return rc;
+  if (heading_mode != 0 && heading_mode != 1)
+    return set_error(KG_E_INVALID, "heading_mode must be 0 or 1");
+  if (f->w_aligned)
+    return set_error(KG_E_STATE, "the write buffer holds the output of a flock step");
+  if ((uint64_t)f->nA + n > f->cap)
+    return set_error(KG_E_CAPACITY, "write buffer capacity %u exceeded", f->cap);
+  KG_HIP(launch_init_uniform(f->A.b, f->nA, n, f->g.w, f->g.h, seed, heading_mode, f->stream));
+  f->nA += n;
+  if (!f->dens_check) f->nagents += n;
+  return KG_OK;
+}
+
+int kg_remove_object_location(kg_field* f, uint32_t id, float x, float y) {
+  clear_error();
+  if (!f) return set_error(KG_E_INVALID, "field is NULL");
+  if (int rc = check_poison(f)) return rc;
+  if (f->w_aligned) {
+    KG_HIP(launch_materialize(f->R.b, f->W.b, f->nW, f->A.b, f->stream));
+    f->nA = f->nW;
+    f->nW = 0;
+    f->w_aligned = false;
+  }
+  if (f->nA == 0) return KG_OK;
+  KG_HIP(launch_remove(f->d_rep, f->A.b, f->nA, id, x, y, f->scratch, f->stream));
+  uint32_t after = 0;
+  KG_HIP(hipMemcpyAsync(&after, f->scratch, sizeof(after), hipMemcpyDeviceToHost, f->stream));
+  KG_HIP(hipStreamSynchronize(f->stream));
+  if (!f->dens_check) f->nagents -= (f->nA - after);
+  f->nA = after;
+  return KG_OK;
+}
+
+int kg_lazy_update(kg_field* f) {
+  clear_error();
+  if (!f) return set_error(KG_E_INVALID, "field is NULL");
+  if (int rc = check_poison(f)) return rc;
+  return lazy_update_impl(f, nullptr);
+}
+
+static int query_common(kg_field* f, float x, float y, float dist, int relax, uint32_t* ids,
+                        size_t cap, size_t* n_out) {
+  if (!f || !n_out) return set_error(KG_E_INVALID, "field and n_out must not be NULL");
+  if (int rc = sync_and_check(f)) return rc;
+  uint32_t* d_out = nullptr;
+  const uint32_t dcap = (uint32_t)std::min<size_t>(cap, 0xFFFFFFFFu);
+  KG_HIP(dmalloc(&d_out, dcap));
+  hipError_t e = launch_query(f->d_rep, f->R.b, f->offset, x, y, dist, sq_threshold(dist), relax,
+                              d_out, dcap, f->scratch, f->stream);
+  uint32_t n = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&n, f->scratch, 4, hipMemcpyDeviceToHost, f->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
+  if (e == hipSuccess && ids && cap)
+    e = hipMemcpy(ids, d_out, std::min<size_t>(n, cap) * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  (void)hipFree(d_out);
+  if (e != hipSuccess) return set_error(KG_E_HIP, "query failed: %s", hipGetErrorString(e));
+  *n_out = n;
+  if (n > cap) return set_error(KG_E_CAPACITY, "%u neighbours do not fit in %zu", n, cap);
+  return KG_OK;
+}
+
+int kg_get_neighbors_within_distance(kg_field* f, float x, float y, float dist, uint32_t* ids,
+                                     size_t cap, size_t* n_out) {
+  clear_error();
+  return query_common(f, x, y, dist, 0, ids, cap, n_out);
+}
+
+int kg_get_neighbors_within_relax_distance(kg_field* f, float x, float y, float dist,
+                                           uint32_t* ids, size_t cap, size_t* n_out) {
+  clear_error();
+  return query_common(f, x, y, dist, 1, ids, cap, n_out);
+}
+
+int kg_get_objects(kg_field* f, float x, float y, int unbuffered, uint32_t* ids, size_t cap,
+                   size_t* n_out) {
+  clear_error();
+  if (!f || !n_out) return set_error(KG_E_INVALID, "field and n_out must not be NULL");
+  if (int rc = sync_and_check(f)) return rc;
+  uint32_t* d_out = nullptr;
+  const uint32_t dcap = (uint32_t)std::min<size_t>(cap, 0xFFFFFFFFu);
+  KG_HIP(dmalloc(&d_out, dcap));
+  hipError_t e;
+  if (!unbuffered) {
+    e = launch_objects_read(f->d_rep, f->R.b, f->offset, x, y, d_out, dcap, f->scratch, f->stream);
+  } else if (f->w_aligned) {
+    Buf wb = f->W.b;
+    wb.id = f->R.b.id;
+    e = launch_objects_write(f->d_rep, wb, f->nW, 1, x, y, d_out, dcap, f->scratch, f->stream);
+  } else {
+    e = launch_objects_write(f->d_rep, f->A.b, f->nA, 0, x, y, d_out, dcap, f->scratch, f->stream);
+  }
+  uint32_t n = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&n, f->scratch, 4, hipMemcpyDeviceToHost, f->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
+  if (e == hipSuccess && ids && cap)
+    e = hipMemcpy(ids, d_out, std::min<size_t>(n, cap) * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  (void)hipFree(d_out);
+  if (e != hipSuccess) return set_error(KG_E_HIP, "get_objects failed: %s", hipGetErrorString(e));
+  *n_out = n;
+  if (n > cap) return set_error(KG_E_CAPACITY, "%u objects do not fit in %zu", n, cap);
+  return KG_OK;
+}
+
+int kg_num_objects_at_location(kg_field* f, float x, float y, size_t* n) {
+  clear_error();
+  const int rc = kg_get_objects(f, x, y, 0, nullptr, 0, n);
+  if (rc == KG_E_CAPACITY) {
+    clear_error();
+    return KG_OK;
+  }
+  return rc;
+}
+
+int kg_num_empty_bags(kg_field* f, size_t* n) {
+  clear_error();
+  if (!f || !n) return set_error(KG_E_INVALID, "field and n must not be NULL");
+  if (int rc = sync_and_check(f)) return rc;
+  KG_HIP(hipMemsetAsync(f->scratch, 0, 4, f->stream));
+  KG_HIP(launch_count_cells(f->d_rep, f->R.b, f->offset, f->nR, f->scratch, f->stream));
+  uint32_t distinct = 0;
+  KG_HIP(hipMemcpyAsync(&distinct, f->scratch, 4, hipMemcpyDeviceToHost, f->stream));
+  KG_HIP(hipStreamSynchronize(f->stream));
+  *n = (size_t)f->g.dw * (size_t)f->g.dh - distinct;
+  return KG_OK;
+}
+
+int kg_nagents(kg_field* f, size_t* n) {
+  clear_error();
+  if (!f || !n) return set_error(KG_E_INVALID, "field and n must not be NULL");
+  *n = f->nagents;
+  return KG_OK;
+}
+
+int kg_count(kg_field* f, int buffer, size_t* n) {
+  clear_error();
+  if (!f || !n) return set_error(KG_E_INVALID, "field and n must not be NULL");
+  if (buffer == 0) {
+    if (int rc = sync_and_check(f)) return rc;
+    uint32_t total = 0;
+    KG_HIP(hipMemcpy(&total, f->offset + f->g.nbins, 4, hipMemcpyDeviceToHost));
+    *n = total;
+  } else {
+    *n = f->w_aligned ? f->nW : f->nA;
+  }
+  return KG_OK;
+}
+
+int kg_neighbor_sets(kg_field* f, float dist, uint32_t* row_ptr, uint32_t* ids, size_t cap,
+                     size_t* total) {
+  clear_error();
+  if (!f || !row_ptr || !total) return set_error(KG_E_INVALID, "field, row_ptr, total required");
+  if (int rc = sync_and_check(f)) return rc;
+  uint32_t nR = 0;
+  KG_HIP(hipMemcpy(&nR, f->offset + f->g.nbins, 4, hipMemcpyDeviceToHost));
+  const float T = sq_threshold(dist);
+  uint32_t* d_cnt = nullptr;
+  KG_HIP(dmalloc(&d_cnt, nR + 1));
+  std::vector<uint32_t> cnt(nR);
+  hipError_t e = launch_neighbor_count(f->d_rep, f->R.b, f->offset, nR, dist, T, d_cnt, f->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
+  if (e == hipSuccess && nR)
+    e = hipMemcpy(cnt.data(), d_cnt, nR * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    (void)hipFree(d_cnt);
+    return set_error(KG_E_HIP, "neighbour count failed: %s", hipGetErrorString(e));
+  }
+  uint64_t run = 0;
+  for (uint32_t i = 0; i < nR; ++i) {
+    row_ptr[i] = (uint32_t)run;
+    run += cnt[i];
+  }
+  row_ptr[nR] = (uint32_t)run;
+  *total = run;
+  if (run > cap || run > 0xFFFFFFFFull) {
+    (void)hipFree(d_cnt);
+    return set_error(KG_E_CAPACITY, "%llu neighbour ids do not fit in %zu",
+                     (unsigned long long)run, cap);
+  }
+  uint32_t* d_ids = nullptr;
+  e = hipMemcpy(d_cnt, row_ptr, (nR + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = dmalloc(&d_ids, run);
+  if (e == hipSuccess)
+    e = launch_neighbor_fill(f->d_rep, f->R.b, f->offset, nR, dist, T, d_cnt, d_ids, f->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
+  if (e == hipSuccess && run)
+    e = hipMemcpy(ids, d_ids, run * sizeof(uint32_t), hipMemcpyDeviceToHost);
+  (void)hipFree(d_cnt);
+  (void)hipFree(d_ids);
+  if (e != hipSuccess) return set_error(KG_E_HIP, "neighbour fill failed: %s", hipGetErrorString(e));
+  return KG_OK;
+}
+
+int kg_flock_step(kg_field* f, const kg_flock_params* p, uint64_t step) {
+  clear_error();
+  if (!f) return set_error(KG_E_INVALID, "field is NULL");
+  if (int rc = check_poison(f)) return rc;
+  if (f->nW || f->nA || f->w_aligned)
+    return set_error(KG_E_STATE, "the write buffer must be empty before a flock step");
+  if (int rc = set_params(f, p, step)) return rc;
+  return flock_impl(f, f->zero64);
+}
+
+static int run_direct_step(kg_field* f) {
+  if (int rc = flock_impl(f, f->gstep)) return rc;
+  return lazy_update_impl(f, f->gstep);
+}
+
+int kg_schedule_run(kg_field* f, const kg_flock_params* p, uint64_t first_step, uint32_t nsteps,
+                    int use_graph) {
+  clear_error();
+  if (!f) return set_error(KG_E_INVALID, "field is NULL");
+  if (int rc = check_poison(f)) return rc;
+  if (nsteps == 0) return KG_OK;
+  if (f->w_aligned) return set_error(KG_E_STATE, "a flock step output is pending");
+  // schedule.rs:349-351: step 0 first calls state.update(0)
+  if (first_step == 0)
+    if (int rc = lazy_update_impl(f, nullptr)) return rc;
+  if (f->nA) return set_error(KG_E_STATE, "the write buffer must be empty");
+  if (int rc = set_params(f, p, first_step)) return rc;
+  KG_HIP(hipMemsetAsync(f->gstep, 0, sizeof(uint64_t), f->stream));
+  uint32_t left = nsteps;
+  if (use_graph && left >= 2 && !f->prof_on) {
+    GraphKey key;
+    std::memcpy(key.params, p, sizeof(kg_flock_params));
+    key.parity = f->parity;
+    auto it = f->graphs.find(key);
+    hipGraphExec_t exec = nullptr;
+    if (it == f->graphs.end()) {
+      // capture two steps: the read buffer returns to the same array pair
+      hipGraph_t graph = nullptr;
+      KG_HIP(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
+      f->capturing = true;
+      int rc = run_direct_step(f);
+      if (rc == KG_OK) rc = run_direct_step(f);
+      f->capturing = false;
+      hipError_t e = hipStreamEndCapture(f->stream, &graph);
+      if (rc) return rc;
+      if (e != hipSuccess) return set_error(KG_E_HIP, "graph capture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return set_error(KG_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+      f->graphs[key] = exec;
+    } else {
+      exec = it->second;
+    }
+    while (left >= 2) {
+      KG_HIP(hipGraphLaunch(exec, f->stream));
+      left -= 2;
+    }
+  }
+  while (left > 0) {
+    if (int rc = run_direct_step(f)) return rc;
+    --left;
+  }
+  return KG_OK;
+}
+
+int kg_read_agents(kg_field* f, int buffer, uint32_t* ids, float* x, float* y, float* hx,
+                   float* hy, size_t cap, size_t* n) {
+  clear_error();
+  if (!f || !n) return set_error(KG_E_INVALID, "field and n must not be NULL");
+  if (int rc = sync_and_check(f)) return rc;
+  Buf src;
+  uint32_t cnt;
+  if (buffer == 0) {
+    KG_HIP(hipMemcpy(&cnt, f->offset + f->g.nbins, 4, hipMemcpyDeviceToHost));
+    src = f->R.b;
+  } else if (f->w_aligned) {
+    cnt = f->nW;
+    src = f->W.b;
+    src.id = f->R.b.id;
+  } else {
+    cnt = f->nA;
+    src = f->A.b;
+  }
+  *n = cnt;
+  if (cnt > cap) return set_error(KG_E_CAPACITY, "%u agents do not fit in %zu", cnt, cap);
+  const size_t fb = (size_t)cnt * sizeof(float);
+  if (ids) KG_HIP(hipMemcpy(ids, src.id, (size_t)cnt * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (x) KG_HIP(hipMemcpy(x, src.x, fb, hipMemcpyDeviceToHost));
+  if (y) KG_HIP(hipMemcpy(y, src.y, fb, hipMemcpyDeviceToHost));
+  if (hx) KG_HIP(hipMemcpy(hx, src.hx, fb, hipMemcpyDeviceToHost));
+  if (hy) KG_HIP(hipMemcpy(hy, src.hy, fb, hipMemcpyDeviceToHost));
+  return KG_OK;
+}
+
+int kg_field_sync(kg_field* f) {
+  clear_error();
+  if (!f) return set_error(KG_E_INVALID, "field is NULL");
+  return sync_and_check(f);
+}
+
+void* kg_field_stream(kg_field* f) { return f ? (void*)f->stream : nullptr; }
+
+int kg_field_profile(kg_field* f, int enable) {
+  clear_error();
+  if (!f) return set_error(KG_E_INVALID, "field is NULL");
+  if (int rc = prof_collect(f)) return rc;
+  f->prof_on = enable != 0;
+  for (int k = 0; k < KG_KERNEL_COUNT; ++k) {
+    f->prof_ms[k] = 0.0;
+    f->prof_n[k] = 0;
+  }
+  return KG_OK;
+}
+
+int kg_field_profile_read(kg_field* f, int kernel, double* total_ms, uint64_t* launches) {
+  clear_error();
+  if (!f || !total_ms || !launches) return set_error(KG_E_INVALID, "NULL argument");
+  if (kernel < 0 || kernel >= KG_KERNEL_COUNT) return set_error(KG_E_INVALID, "unknown kernel");
+  if (int rc = prof_collect(f)) return rc;
+  *total_ms = f->prof_ms[kernel];
+  *launches = f->prof_n[kernel];
+  return KG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+// kg_geom.cpp -- field geometry, device binning choice, replica parameters
+// and the exact squared-distance threshold (host code without HIP calls).
+#include "kg_geom.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace kg {
+
+float sq_threshold(float dist) {
+  if (dist != dist) return dist;
+  if (!(dist > 0.0f)) return -1.0f;
+  if (std::isinf(dist)) return INFINITY;
+  float t = dist * dist;
+  while (t > 0.0f && std::sqrt(t) > dist) t = std::nextafter(t, 0.0f);
+  for (;;) {
+    const float up = std::nextafter(t, INFINITY);
+    if (std::sqrt(up) <= dist) t = up;
+    else break;
+  }
+  return t;
+}
+
+static int ilog2_floor(uint64_t v) {
+  int k = 0;
+  while ((v >> (k + 1)) != 0) ++k;
+  return k;
+}
+
+bool make_geom(float w, float h, float d, int tor, uint64_t expected_agents, uint32_t bin_rows,
+               Geom* g, std::string* why) {
+  if (!(w > 0.0f) || !(h > 0.0f) || !(d > 0.0f) || std::isinf(w) || std::isinf(h) ||
+      std::isinf(d)) {
+    *why = "width, height and discretization must be finite and positive";
+    return false;
+  }
+  // field_2d.rs:317-318 and :403-404, f32 arithmetic with Rust `as i32`
+  const float cw = std::ceil(w / d), ch = std::ceil(h / d);
+  if (!(cw < 1.0e7f) || !(ch < 1.0e7f)) {
+    *why = "bag grid too large (ceil(w/d) and ceil(h/d) must stay below 1e7)";
+    return false;
+  }
+  g->w = w;
+  g->h = h;
+  g->d = d;
+  g->tor = tor ? 1 : 0;
+  g->max_x = (int32_t)cw;
+  g->max_y = (int32_t)ch;
+  g->dw = g->max_x + 1;
+  g->dh = g->max_y + 1;
+  const double cells = (double)g->dw * (double)g->dh;
+  if (cells * 2.0 > 4.0e9) {
+    *why = "bag grid too large for 32-bit bin indices";
+    return false;
+  }
+  // Binning heuristic: about 2 agents per bin, about 512 own agents per tile,
+  // 8 columns per tile; every size a power of two.
+  const double dens = expected_agents ? (double)expected_agents / cells : 0.1;
+  int hb_log2;
+  if (bin_rows) {
+    hb_log2 = ilog2_floor(bin_rows);
+  } else {
+    const double want = std::clamp(2.0 / std::max(dens, 1e-6), 1.0, 64.0);
+    hb_log2 = ilog2_floor((uint64_t)want);
+  }
+  int tx_log2 = dens > 4.0 ? 2 : 3;
+  const double tile_cells = 512.0 / std::max(dens, 1e-6);
+  double ty_want = tile_cells / (double)(1 << tx_log2);
+  ty_want = std::clamp(ty_want, (double)(1 << hb_log2), 1024.0);
+  int ty_log2 = ilog2_floor((uint64_t)ty_want);
+  if (ty_log2 < hb_log2) ty_log2 = hb_log2;
+  // do not make tiles much taller than the grid
+  while (ty_log2 > hb_log2 && (1 << (ty_log2 - 1)) >= g->dh) --ty_log2;
+  while (tx_log2 > 0 && (1 << (tx_log2 - 1)) >= g->dw) --tx_log2;
+  g->hb_log2 = hb_log2;
+  g->tx_log2 = tx_log2;
+  g->tr_log2 = ty_log2 - hb_log2;
+  g->hb = 1 << g->hb_log2;
+  g->tx = 1 << g->tx_log2;
+  g->tr = 1 << g->tr_log2;
+  g->ty = g->tr * g->hb;
+  g->ntx = (g->dw + g->tx - 1) / g->tx;
+  g->nty = (g->dh + g->ty - 1) / g->ty;
+  const uint64_t ntiles = (uint64_t)g->ntx * (uint64_t)g->nty;
+  const uint64_t nbins = ntiles * (uint64_t)g->tx * (uint64_t)g->tr;
+  if (nbins >= (1ull << 31)) {
+    *why = "too many device bins";
+    return false;
+  }
+  g->ntiles = (uint32_t)ntiles;
+  g->nbins = (uint32_t)nbins;
+  return true;
+}
+
+int pick_lds_cap(const Geom& g, uint64_t expected_agents) {
+  const double cells = (double)g.dw * (double)g.dh;
+  const double dens = expected_agents ? (double)expected_agents / cells : 0.1;
+  const double staged = dens * (double)(g.tx + 2) * (double)(g.ty + 2 * g.hb);
+  int cap = 512;
+  while (cap < 4096 && (double)cap < staged * 1.6 + 64.0) cap *= 2;
+  return cap;
+}
+
+bool params_valid(const kg_flock_params& p, std::string* why) {
+  if (p.reserved != 0) {
+    *why = "kg_flock_params.reserved must be 0";
+    return false;
+  }
+  return true;
+}
+
+void set_rep_params(RepDev* r, const kg_flock_params& p) {
+  r->p.cohesion = p.cohesion;
+  r->p.avoidance = p.avoidance;
+  r->p.consistency = p.consistency;
+  r->p.randomness = p.randomness;
+  r->p.momentum = p.momentum;
+  r->p.jump = p.jump;
+  r->p.radius = p.radius;
+  r->p.reserved = 0;
+  r->p.seed = p.seed;
+  r->sq_thresh = sq_threshold(p.radius);
+  // field_2d.rs:401: (dist/self.discretization).floor() as i32
+  const float q = std::floor(p.radius / r->g.d);
+  int32_t dd;
+  if (q != q) dd = 0;
+  else if (q >= 2147483648.0f) dd = INT32_MAX;
+  else if (q <= -2147483648.0f) dd = INT32_MIN;
+  else dd = (int32_t)q;
+  r->disc_dist = dd;
+  r->lds_ok = (r->g.tor != 0 && p.radius > 0.0f && dd >= 0 && dd <= r->g.tx && dd <= r->g.ty &&
+               r->g.tx + 2 * dd <= kMaxStageCols)
+                  ? 1
+                  : 0;
+}
+
+}  // namespace kg

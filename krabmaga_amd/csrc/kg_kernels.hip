@@ -1,0 +1,511 @@
+// kg_kernels.hip -- HIP kernels (gfx950, wave64) of the agent-step path.
+//
+//   init      Flocker::init on the device            tests/model/flockers/state.rs:41-56
+//   count     bin histogram of appended agents       field_2d.rs:838-846 (set_object_location)
+//   scan      exclusive prefix over bin counts       (bag layout, field_2d.rs:307)
+//   scatter   agents to their bins (rebinning)       field_2d.rs:905-921 (lazy_update)
+//   flock     fused neighbour query + Bird::step     field_2d.rs:386-440, bird.rs:38-154
+//             + bin histogram of the new positions
+//   query/... single-replica API queries             field_2d.rs:472-808
+#include "kg_phases.hpp"
+
+namespace kg {
+
+// ---------------------------------------------------------------------------
+// replica descriptor update (stream ordered)
+
+__global__ void k_set_rep(RepDev* dst, RepDev v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
+}
+
+hipError_t launch_set_rep(RepDev* dst, const RepDev& v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_rep, dim3(1), dim3(64), 0, s, dst, v);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// init
+
+__global__ void k_init_uniform(Buf A, uint32_t base, uint32_t n, float w, float h, uint64_t seed,
+                               int heading_mode) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float x, y, hx, hy;
+  init_agent(i, w, h, seed, heading_mode, &x, &y, &hx, &hy);
+  A.x[base + i] = x;
+  A.y[base + i] = y;
+  A.hx[base + i] = hx;
+  A.hy[base + i] = hy;
+  A.id[base + i] = i;
+}
+
+hipError_t launch_init_uniform(Buf A, uint32_t base, uint32_t n, float w, float h, uint64_t seed,
+                               int heading_mode, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_init_uniform, dim3((n + 255) / 256), dim3(256), 0, s, A, base, n, w, h, seed,
+                     heading_mode);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// count / scatter of appended agents (set_object_location and init)
+
+__global__ void k_count_append(const RepDev* __restrict__ reps, int nreps,
+                               const uint32_t* __restrict__ app_prefix, uint32_t napp, Buf A,
+                               uint32_t* __restrict__ count, uint32_t* __restrict__ err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < napp) count_append_one(reps, nreps, app_prefix, A, i, count, err);
+}
+
+hipError_t launch_count_append(const WorkList& wl, Buf A, uint32_t* count, uint32_t* err,
+                               hipStream_t s) {
+  if (wl.napp == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_append, dim3((wl.napp + 255) / 256), dim3(256), 0, s, wl.reps,
+                     wl.nreps, wl.app_prefix, wl.napp, A, count, err);
+  return hipGetLastError();
+}
+
+__global__ void k_scatter_append(const RepDev* __restrict__ reps, int nreps,
+                                 const uint32_t* __restrict__ app_prefix, uint32_t napp, Buf A,
+                                 uint32_t* __restrict__ cursor, Buf R2) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < napp) scatter_append_one(reps, nreps, app_prefix, A, i, cursor, R2);
+}
+
+hipError_t launch_scatter_append(const WorkList& wl, Buf A, uint32_t* cursor, Buf R2,
+                                 hipStream_t s) {
+  if (wl.napp == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter_append, dim3((wl.napp + 255) / 256), dim3(256), 0, s, wl.reps,
+                     wl.nreps, wl.app_prefix, wl.napp, A, cursor, R2);
+  return hipGetLastError();
+}
+
+// Rebinning of the write buffer the flock kernel produced: it is aligned with
+// the current read buffer, so work item (replica, tile) moves exactly the
+// agents of that tile's old range.
+__global__ __launch_bounds__(256) void k_scatter_aligned(const RepDev* __restrict__ reps, int nreps,
+                                                         const uint32_t* __restrict__ tile_prefix,
+                                                         Buf R, Buf W,
+                                                         const uint32_t* __restrict__ offset_cur,
+                                                         uint32_t* __restrict__ cursor, Buf R2) {
+  const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+  const TileRange tr = tile_range(reps, nreps, tile_prefix, item, offset_cur);
+  const RepDev& rp = reps[tr.r];
+  for (uint32_t i = tr.lo + threadIdx.x; i < tr.hi; i += blockDim.x)
+    scatter_aligned_one(rp, R, W, i, cursor, R2);
+}
+
+hipError_t launch_scatter_aligned(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
+                                  uint32_t* cursor, Buf R2, hipStream_t s) {
+  if (wl.ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter_aligned, dim3(wl.ntiles), dim3(256), 0, s, wl.reps, wl.nreps,
+                     wl.tile_prefix, R, W, offset_cur, cursor, R2);
+  return hipGetLastError();
+}
+
+__global__ void k_materialize(Buf R, Buf W, uint32_t n, Buf A) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  A.x[i] = W.x[i];
+  A.y[i] = W.y[i];
+  A.hx[i] = W.hx[i];
+  A.hy[i] = W.hy[i];
+  A.id[i] = R.id[i];
+}
+
+hipError_t launch_materialize(Buf R, Buf W, uint32_t n, Buf A, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_materialize, dim3((n + 255) / 256), dim3(256), 0, s, R, W, n, A);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// scan: single-pass exclusive prefix with decoupled look-back.
+//
+// Status word per scan tile: [epoch:30][flag:2][value:32]; flag 1 = tile
+// aggregate, 2 = inclusive prefix.  The epoch (bumped by the last tile of
+// every launch) makes stale words of earlier launches invalid, so nothing
+// has to be cleared between launches (graph replays included).  Tile indices
+// come from an atomic counter so a tile only waits on tiles already running.
+// The kernel also zeroes the counts (for the next histogram), fills the
+// scatter cursors and advances the schedule-step counter.
+
+__global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ count,
+                                                       uint32_t* __restrict__ offset,
+                                                       uint32_t* __restrict__ cursor, uint32_t n,
+                                                       uint64_t* __restrict__ status,
+                                                       uint32_t* tile_ctr, uint32_t* epoch_ptr,
+                                                       uint64_t* gstep) {
+  __shared__ uint32_t s_tile, s_epoch, s_prefix;
+  __shared__ uint32_t s_warp[kScanThreads / kWave];
+  const int t = threadIdx.x;
+  if (t == 0) {
+    const uint32_t ep = __hip_atomic_load(epoch_ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_tile = __hip_atomic_fetch_add(tile_ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_epoch = ep & 0x3FFFFFFFu;
+  }
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
+  const uint32_t base = tile * kScanTile + (uint32_t)t * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t idx = base + k;
+    v[k] = idx < n ? count[idx] : 0u;
+    sum += v[k];
+  }
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) count[base + k] = 0u;
+  uint32_t total;
+  const uint32_t excl = block_exclusive_scan<kScanThreads>(sum, s_warp, &total);
+  if (t == 0) {
+    const uint64_t ep = (uint64_t)s_epoch << 34;
+    uint32_t prefix = 0;
+    if (tile == 0) {
+      __hip_atomic_store(&status[0], ep | (2ull << 32) | total, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      __hip_atomic_store(&status[tile], ep | (1ull << 32) | total, __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      int64_t j = (int64_t)tile - 1;
+      for (;;) {
+        const uint64_t st =
+            __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t flag = (uint32_t)(st >> 32) & 3u;
+        if ((st & ~0x3FFFFFFFFull) != ep || flag == 0) continue;
+        prefix += (uint32_t)st;
+        if (flag == 2) break;
+        --j;
+      }
+      __hip_atomic_store(&status[tile], ep | (2ull << 32) | (uint32_t)(prefix + total),
+                         __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_prefix = prefix;
+  }
+  __syncthreads();
+  uint32_t run = s_prefix + excl;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint32_t idx = base + k;
+    if (idx < n) {
+      offset[idx] = run;
+      cursor[idx] = run;
+    }
+    run += v[k];
+  }
+  if (tile == ntiles - 1 && t == 0) {
+    offset[n] = s_prefix + total;
+    if (gstep) *gstep += 1;
+    __hip_atomic_store(tile_ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(epoch_ptr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint32_t nbins,
+                       const ScanState& ss, uint64_t* gstep, hipStream_t s) {
+  const uint32_t ntiles = (nbins + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(kScanThreads), 0, s, count, offset, cursor, nbins,
+                     ss.status, ss.tile_ctr, ss.epoch, gstep);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// flock: fused neighbour query + Bird::step, one block per (replica, tile);
+// phase bodies in kg_phases.hpp.
+
+size_t flock_lds_bytes(int cap) { return flock_smem_bytes(cap); }
+
+__global__ __launch_bounds__(kFlockThreads) void k_flock(
+    const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
+    const uint32_t* __restrict__ offset, Buf W, uint32_t* __restrict__ count_next,
+    const uint64_t* __restrict__ gstep, uint32_t* __restrict__ err, int lds_cap) {
+  extern __shared__ __align__(16) uint8_t smem_raw[];
+  __shared__ uint32_t s_seg_start[3 * kMaxStageCols];
+  __shared__ uint32_t s_seg_len[3 * kMaxStageCols];
+  __shared__ uint32_t s_col_base[kMaxStageCols + 1];
+  __shared__ uint32_t s_warp[kFlockThreads / kWave];
+  __shared__ uint32_t s_nown;
+
+  const int t = threadIdx.x;
+  const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+  const FlockTile ft = flock_tile(reps, nreps, tile_prefix, item, *gstep);
+  const RepDev& rp = reps[ft.r];
+
+  bool use_lds = rp.lds_ok != 0 && ft.ncols <= kMaxStageCols;
+  uint32_t L = 0;
+  if (use_lds) {
+    const uint32_t len =
+        t < ft.ncols ? flock_segments(rp, ft, t, offset, s_seg_start, s_seg_len) : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan<kFlockThreads>(len, s_warp, &total);
+    if (t < ft.ncols) s_col_base[t] = ex;
+    if (t == 0) {
+      s_col_base[ft.ncols] = total;
+      s_nown = 0;
+    }
+    L = total;
+    use_lds = total <= (uint32_t)lds_cap;
+    __syncthreads();
+  }
+  if (use_lds) {
+    const FlockSmem S = flock_carve(smem_raw, lds_cap);
+    for (uint32_t k = t; k < L; k += kFlockThreads)
+      flock_stage(rp, ft, k, R, s_seg_start, s_seg_len, s_col_base, S);
+    __syncthreads();
+    for (uint32_t k = t; k < L; k += kFlockThreads) flock_rank(ft, k, s_col_base, S, &s_nown);
+    __syncthreads();
+    const uint32_t nown = s_nown;
+    for (uint32_t o = t; o < nown; o += kFlockThreads)
+      flock_agent_lds(rp, ft, S.own[o], s_col_base, S, W, count_next, err);
+    return;
+  }
+  // global path
+  const uint32_t lo = offset[ft.tb0], hi = offset[ft.tb1];
+  for (uint32_t gi = lo + t; gi < hi; gi += kFlockThreads)
+    flock_agent_global(rp, ft.step, gi, R, offset, W, count_next, err);
+}
+
+hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W,
+                        uint32_t* count_next, const uint64_t* gstep, uint32_t* err, int lds_cap,
+                        hipStream_t s) {
+  if (wl.ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_flock, dim3(wl.ntiles), dim3(kFlockThreads), flock_lds_bytes(lds_cap), s,
+                     wl.reps, wl.nreps, wl.tile_prefix, R, offset, W, count_next, gstep, err,
+                     lds_cap);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// single-replica API queries
+
+__global__ void k_query(const RepDev* __restrict__ rep, Buf R, const uint32_t* __restrict__ offset,
+                        float x, float y, float dist, float T, int relax, uint32_t* out,
+                        uint32_t cap, uint32_t* nout) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const Geom g = rep->g;
+  uint32_t n = 0;
+  query_global(g, R, offset + rep->bin_base, x, y, dist, T, relax != 0, [&](uint32_t e) {
+    if (n < cap) out[n] = R.id[e];
+    ++n;
+  });
+  *nout = n;
+}
+
+hipError_t launch_query(const RepDev* rep, Buf R, const uint32_t* offset, float x, float y,
+                        float dist, float T, int relax, uint32_t* out, uint32_t cap,
+                        uint32_t* nout, hipStream_t s) {
+  hipLaunchKernelGGL(k_query, dim3(1), dim3(64), 0, s, rep, R, offset, x, y, dist, T, relax, out,
+                     cap, nout);
+  return hipGetLastError();
+}
+
+__global__ void k_neighbor_count(const RepDev* __restrict__ rep, Buf R,
+                                 const uint32_t* __restrict__ offset, uint32_t n, float dist,
+                                 float T, uint32_t* counts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Geom g = rep->g;
+  uint32_t c = 0;
+  query_global(g, R, offset + rep->bin_base, R.x[i], R.y[i], dist, T, false,
+               [&](uint32_t) { ++c; });
+  counts[i] = c;
+}
+
+hipError_t launch_neighbor_count(const RepDev* rep, Buf R, const uint32_t* offset, uint32_t n,
+                                 float dist, float T, uint32_t* counts, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_neighbor_count, dim3((n + 127) / 128), dim3(128), 0, s, rep, R, offset, n,
+                     dist, T, counts);
+  return hipGetLastError();
+}
+
+__global__ void k_neighbor_fill(const RepDev* __restrict__ rep, Buf R,
+                                const uint32_t* __restrict__ offset, uint32_t n, float dist,
+                                float T, const uint32_t* __restrict__ row_ptr, uint32_t* ids) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Geom g = rep->g;
+  uint32_t k = row_ptr[i];
+  query_global(g, R, offset + rep->bin_base, R.x[i], R.y[i], dist, T, false,
+               [&](uint32_t e) { ids[k++] = R.id[e]; });
+}
+
+hipError_t launch_neighbor_fill(const RepDev* rep, Buf R, const uint32_t* offset, uint32_t n,
+                                float dist, float T, const uint32_t* row_ptr, uint32_t* ids,
+                                hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_neighbor_fill, dim3((n + 127) / 128), dim3(128), 0, s, rep, R, offset, n,
+                     dist, T, row_ptr, ids);
+  return hipGetLastError();
+}
+
+// get_objects on the read buffer: the bag of (x, y) in ascending id.
+__global__ void k_objects_read(const RepDev* __restrict__ rep, Buf R,
+                               const uint32_t* __restrict__ offset, float x, float y,
+                               uint32_t* out, uint32_t cap, uint32_t* nout) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const Geom g = rep->g;
+  int32_t bx, by;
+  discretize(g, x, y, &bx, &by);
+  uint32_t n = 0;
+  if (cell_valid(g, bx, by)) {
+    const uint32_t b = rep->bin_base + bin_of(g, bx, by);
+    const uint32_t s = offset[b], e = offset[b + 1];
+    int64_t last = -1;
+    for (;;) {
+      uint32_t best = 0xFFFFFFFFu;
+      bool any = false;
+      for (uint32_t q = s; q < e; ++q) {
+        const uint32_t qid = R.id[q];
+        if ((int64_t)qid <= last || (any && qid >= best)) continue;
+        int32_t qx, qy;
+        discretize(g, R.x[q], R.y[q], &qx, &qy);
+        if (qx != bx || qy != by) continue;
+        best = qid;
+        any = true;
+      }
+      if (!any) break;
+      for (uint32_t q = s; q < e; ++q) {
+        if (R.id[q] != best) continue;
+        int32_t qx, qy;
+        discretize(g, R.x[q], R.y[q], &qx, &qy);
+        if (qx != bx || qy != by) continue;
+        if (n < cap) out[n] = best;
+        ++n;
+      }
+      last = best;
+    }
+  }
+  *nout = n;
+}
+
+hipError_t launch_objects_read(const RepDev* rep, Buf R, const uint32_t* offset, float x, float y,
+                               uint32_t* out, uint32_t cap, uint32_t* nout, hipStream_t s) {
+  hipLaunchKernelGGL(k_objects_read, dim3(1), dim3(64), 0, s, rep, R, offset, x, y, out, cap, nout);
+  return hipGetLastError();
+}
+
+// get_objects_unbuffered: the agents of buffer B (n entries) in the bag of
+// (x, y), in B's order (insertion order) or, with by_id, in ascending id.
+__global__ void k_objects_write(const RepDev* __restrict__ rep, Buf B, uint32_t n, int by_id,
+                                float x, float y, uint32_t* out, uint32_t cap, uint32_t* nout) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const Geom g = rep->g;
+  int32_t bx, by;
+  discretize(g, x, y, &bx, &by);
+  uint32_t m = 0;
+  if (!by_id) {
+    for (uint32_t q = 0; q < n; ++q) {
+      int32_t qx, qy;
+      discretize(g, B.x[q], B.y[q], &qx, &qy);
+      if (qx != bx || qy != by) continue;
+      if (m < cap) out[m] = B.id[q];
+      ++m;
+    }
+  } else {
+    int64_t last = -1;
+    for (;;) {
+      uint32_t best = 0xFFFFFFFFu;
+      bool any = false;
+      for (uint32_t q = 0; q < n; ++q) {
+        const uint32_t qid = B.id[q];
+        if ((int64_t)qid <= last || (any && qid >= best)) continue;
+        int32_t qx, qy;
+        discretize(g, B.x[q], B.y[q], &qx, &qy);
+        if (qx != bx || qy != by) continue;
+        best = qid;
+        any = true;
+      }
+      if (!any) break;
+      for (uint32_t q = 0; q < n; ++q) {
+        if (B.id[q] != best) continue;
+        int32_t qx, qy;
+        discretize(g, B.x[q], B.y[q], &qx, &qy);
+        if (qx != bx || qy != by) continue;
+        if (m < cap) out[m] = best;
+        ++m;
+      }
+      last = best;
+    }
+  }
+  *nout = m;
+}
+
+hipError_t launch_objects_write(const RepDev* rep, Buf B, uint32_t n, int by_id, float x, float y,
+                                uint32_t* out, uint32_t cap, uint32_t* nout, hipStream_t s) {
+  hipLaunchKernelGGL(k_objects_write, dim3(1), dim3(64), 0, s, rep, B, n, by_id, x, y, out, cap,
+                     nout);
+  return hipGetLastError();
+}
+
+// Number of distinct occupied cells of the read buffer: an agent counts if
+// no other agent of its cell precedes it in (id, index) order.
+__global__ void k_count_cells(const RepDev* __restrict__ rep, Buf R,
+                              const uint32_t* __restrict__ offset, uint32_t n,
+                              uint32_t* distinct) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Geom g = rep->g;
+  int32_t cx, cy;
+  discretize(g, R.x[i], R.y[i], &cx, &cy);
+  const uint32_t b = rep->bin_base + bin_of(g, cx, cy);
+  const uint32_t s = offset[b], e = offset[b + 1];
+  const uint32_t id = R.id[i];
+  for (uint32_t q = s; q < e; ++q) {
+    if (q == i) continue;
+    const uint32_t qid = R.id[q];
+    if (qid > id || (qid == id && q > i)) continue;
+    int32_t qx, qy;
+    discretize(g, R.x[q], R.y[q], &qx, &qy);
+    if (qx == cx && qy == cy) return;
+  }
+  atomicAdd(distinct, 1u);
+}
+
+hipError_t launch_count_cells(const RepDev* rep, Buf R, const uint32_t* offset, uint32_t n,
+                              uint32_t* distinct, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_cells, dim3((n + 255) / 256), dim3(256), 0, s, rep, R, offset, n,
+                     distinct);
+  return hipGetLastError();
+}
+
+// remove_object_location on an insertion-ordered buffer: stable compaction
+// dropping every entry with this id in the bag of (x, y).
+__global__ void k_remove(const RepDev* __restrict__ rep, Buf A, uint32_t n, uint32_t id, float x,
+                         float y, uint32_t* n_after) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const Geom g = rep->g;
+  int32_t bx, by;
+  discretize(g, x, y, &bx, &by);
+  uint32_t k = 0;
+  for (uint32_t q = 0; q < n; ++q) {
+    bool drop = false;
+    if (A.id[q] == id) {
+      int32_t qx, qy;
+      discretize(g, A.x[q], A.y[q], &qx, &qy);
+      drop = (qx == bx && qy == by);
+    }
+    if (drop) continue;
+    if (k != q) {
+      A.x[k] = A.x[q];
+      A.y[k] = A.y[q];
+      A.hx[k] = A.hx[q];
+      A.hy[k] = A.hy[q];
+      A.id[k] = A.id[q];
+    }
+    ++k;
+  }
+  *n_after = k;
+}
+
+hipError_t launch_remove(const RepDev* rep, Buf A, uint32_t n, uint32_t id, float x, float y,
+                         uint32_t* n_after, hipStream_t s) {
+  hipLaunchKernelGGL(k_remove, dim3(1), dim3(64), 0, s, rep, A, n, id, x, y, n_after);
+  return hipGetLastError();
+}
+
+}  // namespace kg

@@ -1,0 +1,47 @@
+// kg_launch.hpp -- host-side launchers of the agent-step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kg_types.hpp"
+
+namespace kg {
+
+// Bytes of dynamic LDS the flock kernel needs for `cap` staged agents.
+size_t flock_lds_bytes(int cap);
+
+hipError_t launch_set_rep(RepDev* dst, const RepDev& v, hipStream_t s);
+hipError_t launch_init_uniform(Buf A, uint32_t base, uint32_t n, float w, float h, uint64_t seed,
+                               int heading_mode, hipStream_t s);
+hipError_t launch_count_append(const WorkList& wl, Buf A, uint32_t* count, uint32_t* err,
+                               hipStream_t s);
+hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint32_t nbins,
+                       const ScanState& ss, uint64_t* gstep, hipStream_t s);
+hipError_t launch_scatter_aligned(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
+                                  uint32_t* cursor, Buf R2, hipStream_t s);
+hipError_t launch_scatter_append(const WorkList& wl, Buf A, uint32_t* cursor, Buf R2,
+                                 hipStream_t s);
+hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W,
+                        uint32_t* count_next, const uint64_t* gstep, uint32_t* err, int lds_cap,
+                        hipStream_t s);
+hipError_t launch_materialize(Buf R, Buf W, uint32_t n, Buf A, hipStream_t s);
+
+// Single-replica queries (API and parity use; not on the hot path).
+hipError_t launch_query(const RepDev* rep, Buf R, const uint32_t* offset, float x, float y,
+                        float dist, float T, int relax, uint32_t* out, uint32_t cap,
+                        uint32_t* nout, hipStream_t s);
+hipError_t launch_neighbor_count(const RepDev* rep, Buf R, const uint32_t* offset, uint32_t n,
+                                 float dist, float T, uint32_t* counts, hipStream_t s);
+hipError_t launch_neighbor_fill(const RepDev* rep, Buf R, const uint32_t* offset, uint32_t n,
+                                float dist, float T, const uint32_t* row_ptr, uint32_t* ids,
+                                hipStream_t s);
+hipError_t launch_objects_read(const RepDev* rep, Buf R, const uint32_t* offset, float x, float y,
+                               uint32_t* out, uint32_t cap, uint32_t* nout, hipStream_t s);
+hipError_t launch_objects_write(const RepDev* rep, Buf B, uint32_t n, int by_id, float x, float y,
+                                uint32_t* out, uint32_t cap, uint32_t* nout, hipStream_t s);
+hipError_t launch_count_cells(const RepDev* rep, Buf R, const uint32_t* offset, uint32_t n,
+                              uint32_t* distinct, hipStream_t s);
+hipError_t launch_remove(const RepDev* rep, Buf A, uint32_t n, uint32_t id, float x, float y,
+                         uint32_t* n_after, hipStream_t s);
+
+}  // namespace kg

@@ -1,0 +1,349 @@
+// kg_phases.hpp -- the bodies of the binning and flock kernels, written as
+// __host__ __device__ functions of (block, thread) so that a host driver can
+// execute them block by block, phase by phase, with the exact index
+// arithmetic and f32 operations of the device build.
+#pragma once
+
+#include "kg_types.hpp"
+
+namespace kg {
+
+KG_HD uint32_t atom_add(uint32_t* p, uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return atomicAdd(p, v);
+#else
+  const uint32_t o = *p;
+  *p = o + v;
+  return o;
+#endif
+}
+
+KG_HD void atom_or(uint32_t* p, uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  atomicOr(p, v);
+#else
+  *p |= v;
+#endif
+}
+
+// Spreads consecutive work items over the 8 XCDs so that each XCD (and its
+// L2) sees a contiguous run of tiles: hardware dispatches block b to XCD
+// b % 8.
+KG_HD uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+  const uint32_t q = nb / 8u;
+  if (b >= q * 8u) return b;
+  return (b % 8u) * q + b / 8u;
+}
+
+// Last r with prefix[r] <= item (prefix has nreps + 1 entries).
+KG_HD int find_rep(const uint32_t* prefix, int nreps, uint32_t item) {
+  int lo = 0, hi = nreps;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= item) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ---- set_object_location path: histogram and scatter of appended agents ----
+
+KG_HD void count_append_one(const RepDev* reps, int nreps, const uint32_t* app_prefix, Buf A,
+                            uint32_t i, uint32_t* count, uint32_t* err) {
+  const int r = find_rep(app_prefix, nreps, i);
+  const Geom& g = reps[r].g;
+  int32_t cx, cy;
+  discretize(g, A.x[i], A.y[i], &cx, &cy);
+  if (!cell_valid(g, cx, cy)) {
+    atom_or(err, 1u);
+    return;
+  }
+  atom_add(&count[reps[r].bin_base + bin_of(g, cx, cy)], 1u);
+}
+
+KG_HD void scatter_append_one(const RepDev* reps, int nreps, const uint32_t* app_prefix, Buf A,
+                              uint32_t i, uint32_t* cursor, Buf R2) {
+  const int r = find_rep(app_prefix, nreps, i);
+  const Geom& g = reps[r].g;
+  const float x = A.x[i], y = A.y[i];
+  int32_t cx, cy;
+  discretize(g, x, y, &cx, &cy);
+  if (!cell_valid(g, cx, cy)) return;
+  const uint32_t pos = atom_add(&cursor[reps[r].bin_base + bin_of(g, cx, cy)], 1u);
+  R2.x[pos] = x;
+  R2.y[pos] = y;
+  R2.hx[pos] = A.hx[i];
+  R2.hy[pos] = A.hy[i];
+  R2.id[pos] = A.id[i];
+}
+
+// ---- rebinning of the flock kernel's output (aligned with R) ---------------
+
+struct TileRange {
+  int r;
+  uint32_t lo, hi;  // read-buffer range of the tile
+};
+
+KG_HD TileRange tile_range(const RepDev* reps, int nreps, const uint32_t* tile_prefix,
+                           uint32_t item, const uint32_t* offset) {
+  TileRange tr;
+  tr.r = find_rep(tile_prefix, nreps, item);
+  const uint32_t tile = item - tile_prefix[tr.r];
+  const Geom& g = reps[tr.r].g;
+  const uint32_t b0 = reps[tr.r].bin_base + (tile << (g.tx_log2 + g.tr_log2));
+  const uint32_t b1 = b0 + (1u << (g.tx_log2 + g.tr_log2));
+  tr.lo = offset[b0];
+  tr.hi = offset[b1];
+  return tr;
+}
+
+KG_HD void scatter_aligned_one(const RepDev& rp, Buf R, Buf W, uint32_t i, uint32_t* cursor,
+                               Buf R2) {
+  const Geom& g = rp.g;
+  const float x = W.x[i], y = W.y[i];
+  int32_t cx, cy;
+  discretize(g, x, y, &cx, &cy);
+  if (!cell_valid(g, cx, cy)) return;  // reported by the flock kernel
+  const uint32_t pos = atom_add(&cursor[rp.bin_base + bin_of(g, cx, cy)], 1u);
+  R2.x[pos] = x;
+  R2.y[pos] = y;
+  R2.hx[pos] = W.hx[i];
+  R2.hy[pos] = W.hy[i];
+  R2.id[pos] = R.id[i];
+}
+
+// ---- flock: fused neighbour query + Bird::step, one block per tile ---------
+//
+// LDS tile path (toroidal fields whose window reaches at most one tile
+// further): the tile's own bins plus the halo cells the own agents' windows
+// reach are staged into LDS, each staged column is sorted by (cell y, id) --
+// the reference's bag-scan order -- and every own agent scans its window from
+// LDS.  Tiles whose staged set exceeds the LDS capacity, and fields outside
+// the LDS path's reach, take the global path (query_global per agent).
+
+struct FlockTile {
+  int r;
+  uint32_t tile;
+  int32_t tcy;
+  int32_t c0, c1, y0, y1;          // own cells [c0,c1) x [y0,y1)
+  int32_t sc_lo, ry_lo, ry_hi;     // staged columns start at sc_lo; staged rows
+  int32_t ncols;                   // staged columns
+  uint32_t tb0, tb1;               // the tile's bins
+  uint64_t step;
+};
+
+KG_HD FlockTile flock_tile(const RepDev* reps, int nreps, const uint32_t* tile_prefix,
+                           uint32_t item, uint64_t gstep) {
+  FlockTile ft;
+  ft.r = find_rep(tile_prefix, nreps, item);
+  const RepDev& rp = reps[ft.r];
+  const Geom& g = rp.g;
+  const int32_t dd = rp.disc_dist;
+  ft.tile = item - tile_prefix[ft.r];
+  const int32_t tcx = (int32_t)(ft.tile % (uint32_t)g.ntx);
+  ft.tcy = (int32_t)(ft.tile / (uint32_t)g.ntx);
+  ft.c0 = tcx << g.tx_log2;
+  ft.c1 = imin(ft.c0 + g.tx, g.dw);
+  ft.y0 = ft.tcy * g.ty;
+  ft.y1 = imin(ft.y0 + g.ty, g.dh);
+  ft.tb0 = rp.bin_base + (ft.tile << (g.tx_log2 + g.tr_log2));
+  ft.tb1 = ft.tb0 + (1u << (g.tx_log2 + g.tr_log2));
+  // staged cells: own cells plus every cell an own agent's clamped window
+  // reaches (field_2d.rs:411-416)
+  ft.sc_lo = imax(0, ft.c0 - dd);
+  const int32_t sc_hi = imax(ft.c1 - 1, imin(g.max_x - 1, ft.c1 - 1 + dd));
+  ft.ry_lo = imax(0, ft.y0 - dd);
+  ft.ry_hi = imax(ft.y1 - 1, imin(g.max_y - 1, ft.y1 - 1 + dd));
+  ft.ncols = sc_hi - ft.sc_lo + 1;
+  ft.step = rp.step_base + gstep;
+  return ft;
+}
+
+// Staged column t (< ncols): up to three read-buffer runs -- the band of the
+// tile row below, the tile's own row band, the band of the tile row above.
+// Returns the column's total length.
+KG_HD uint32_t flock_segments(const RepDev& rp, const FlockTile& ft, int t,
+                              const uint32_t* offset, uint32_t* seg_start, uint32_t* seg_len) {
+  const Geom& g = rp.g;
+  const int32_t col = ft.sc_lo + t;
+  const int32_t ttx = col >> g.tx_log2;
+  const int32_t lc = col & (g.tx - 1);
+  uint32_t len = 0;
+  for (int k = 0; k < 3; ++k) {
+    uint32_t st = 0, ln = 0;
+    const int32_t tty = ft.tcy - 1 + k;
+    if (tty >= 0 && tty < g.nty) {
+      const int32_t by0 = tty * g.ty;
+      const int32_t lo_row = imax(ft.ry_lo, by0);
+      const int32_t hi_row = imin(ft.ry_hi, by0 + g.ty - 1);
+      if (lo_row <= hi_row) {
+        const uint32_t tt = (uint32_t)tty * (uint32_t)g.ntx + (uint32_t)ttx;
+        const uint32_t cb = rp.bin_base + col_bin0(g, tt, lc);
+        const uint32_t b_lo = cb + (uint32_t)((lo_row - by0) >> g.hb_log2);
+        const uint32_t b_hi = cb + (uint32_t)((hi_row - by0) >> g.hb_log2) + 1u;
+        st = offset[b_lo];
+        ln = offset[b_hi] - st;
+      }
+    }
+    seg_start[3 * t + k] = st;
+    seg_len[3 * t + k] = ln;
+    len += ln;
+  }
+  return len;
+}
+
+struct FlockSmem {
+  float *x, *y, *hx, *hy;
+  uint32_t *id, *gi;
+  int32_t* cy;
+  uint16_t *perm, *col, *own;
+};
+
+KG_HD size_t flock_smem_bytes(int cap) {
+  return (size_t)cap * (7 * sizeof(uint32_t) + 3 * sizeof(uint16_t));
+}
+
+KG_HD FlockSmem flock_carve(uint8_t* p, int cap) {
+  FlockSmem s;
+  s.x = (float*)p;
+  s.y = s.x + cap;
+  s.hx = s.y + cap;
+  s.hy = s.hx + cap;
+  s.id = (uint32_t*)(s.hy + cap);
+  s.gi = s.id + cap;
+  s.cy = (int32_t*)(s.gi + cap);
+  s.perm = (uint16_t*)(s.cy + cap);
+  s.col = s.perm + cap;
+  s.own = s.col + cap;
+  return s;
+}
+
+// Stages item k (< total staged) into LDS.
+KG_HD void flock_stage(const RepDev& rp, const FlockTile& ft, uint32_t k, Buf R,
+                       const uint32_t* seg_start, const uint32_t* seg_len,
+                       const uint32_t* col_base, FlockSmem S) {
+  int lo = 0, hi = ft.ncols;  // column c with col_base[c] <= k < col_base[c+1]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (col_base[mid] <= k) lo = mid;
+    else hi = mid;
+  }
+  uint32_t off = k - col_base[lo];
+  int seg = 0;
+  while (off >= seg_len[3 * lo + seg]) {
+    off -= seg_len[3 * lo + seg];
+    ++seg;
+  }
+  const uint32_t gi = seg_start[3 * lo + seg] + off;
+  const float y = R.y[gi];
+  S.x[k] = R.x[gi];
+  S.y[k] = y;
+  S.hx[k] = R.hx[gi];
+  S.hy[k] = R.hy[gi];
+  S.id[k] = R.id[gi];
+  S.gi[k] = gi;
+  S.cy[k] = f2i(floorf(y / rp.g.d));
+  S.col[k] = (uint16_t)lo;
+}
+
+// Ranks item k inside its staged column by (cell y, id, slot) and, if it is
+// one of the tile's own agents, appends it to the own list.
+KG_HD void flock_rank(const FlockTile& ft, uint32_t k, const uint32_t* col_base, FlockSmem S,
+                      uint32_t* nown) {
+  const int c = S.col[k];
+  const uint32_t lo = col_base[c], hi = col_base[c + 1];
+  const int32_t ky = S.cy[k];
+  const uint32_t kid = S.id[k];
+  uint32_t rank = 0;
+  for (uint32_t j = lo; j < hi; ++j) {
+    const int32_t jy = S.cy[j];
+    const uint32_t jid = S.id[j];
+    rank += (jy < ky) || (jy == ky && (jid < kid || (jid == kid && j < k)));
+  }
+  S.perm[lo + rank] = (uint16_t)k;
+  const int32_t cx = ft.sc_lo + c;
+  if (cx >= ft.c0 && cx < ft.c1 && ky >= ft.y0 && ky < ft.y1) {
+    const uint32_t o = atom_add(nown, 1u);
+    S.own[o] = (uint16_t)k;
+  }
+}
+
+// Writes the agent's new state into the write buffer and counts its new bin.
+KG_HD void flock_emit(const Geom& g, uint32_t bin_base, Buf W, uint32_t gi, float nx, float ny,
+                      float nhx, float nhy, uint32_t* count_next, uint32_t* err) {
+  W.x[gi] = nx;
+  W.y[gi] = ny;
+  W.hx[gi] = nhx;
+  W.hy[gi] = nhy;
+  int32_t cx, cy;
+  discretize(g, nx, ny, &cx, &cy);
+  if (cell_valid(g, cx, cy)) atom_add(&count_next[bin_base + bin_of(g, cx, cy)], 1u);
+  else atom_or(err, 1u);
+}
+
+// Bird::step for own agent k of the staged tile, neighbours from LDS.
+KG_HD void flock_agent_lds(const RepDev& rp, const FlockTile& ft, uint32_t k,
+                           const uint32_t* col_base, FlockSmem S, Buf W, uint32_t* count_next,
+                           uint32_t* err) {
+  const Geom& g = rp.g;
+  const float T = rp.sq_thresh;
+  const int32_t dd = rp.disc_dist;
+  const float qx = S.x[k], qy = S.y[k];
+  const uint32_t qid = S.id[k];
+  const int32_t cx = ft.sc_lo + S.col[k];
+  const int32_t cy = S.cy[k];
+  // clamped window (field_2d.rs:401-416)
+  const int32_t min_i = imax(0, cx - dd), max_i = imin(cx + dd, g.max_x - 1);
+  const int32_t min_j = imax(0, cy - dd), max_j = imin(cy + dd, g.max_y - 1);
+  FlockAcc acc;
+  for (int32_t i = min_i; i <= max_i; ++i) {
+    const int c = i - ft.sc_lo;
+    uint32_t lo = col_base[c], hi = col_base[c + 1];
+    while (lo < hi) {  // first slot with cell y >= min_j
+      const uint32_t mid = (lo + hi) >> 1;
+      if (S.cy[S.perm[mid]] < min_j) lo = mid + 1;
+      else hi = mid;
+    }
+    const uint32_t end = col_base[c + 1];
+    int32_t cur = -1;
+    int chk = 0;
+    for (uint32_t q = lo; q < end; ++q) {
+      const uint32_t e = S.perm[q];
+      const int32_t ey_c = S.cy[e];
+      if (ey_c > max_j) break;
+      if (ey_c != cur) {
+        cur = ey_c;
+        chk = check_circle_sq(g, i, ey_c, qx, qy, T);
+      }
+      if (chk == -1) continue;
+      // field_2d.rs:432 (toroidal distance) doubles as bird.rs:64-65
+      const float dx = toroidal_distance(qx, S.x[e], g.w);
+      const float dy = toroidal_distance(qy, S.y[e], g.h);
+      if (chk == 1 || dx * dx + dy * dy <= T) acc.add(S.id[e] == qid, dx, dy, S.hx[e], S.hy[e]);
+    }
+  }
+  float nx, ny, nhx, nhy;
+  flock_finish(acc, rp.p, qid, ft.step, qx, qy, S.hx[k], S.hy[k], g.w, &nx, &ny, &nhx, &nhy);
+  flock_emit(g, rp.bin_base, W, S.gi[k], nx, ny, nhx, nhy, count_next, err);
+}
+
+// Bird::step for read-buffer agent gi, neighbours straight from HBM.
+KG_HD void flock_agent_global(const RepDev& rp, uint64_t step, uint32_t gi, Buf R,
+                              const uint32_t* offset, Buf W, uint32_t* count_next,
+                              uint32_t* err) {
+  const Geom& g = rp.g;
+  const float qx = R.x[gi], qy = R.y[gi];
+  const uint32_t qid = R.id[gi];
+  FlockAcc acc;
+  query_global(g, R, offset + rp.bin_base, qx, qy, rp.p.radius, rp.sq_thresh, false,
+               [&](uint32_t e) {
+                 const float dx = toroidal_distance(qx, R.x[e], g.w);
+                 const float dy = toroidal_distance(qy, R.y[e], g.h);
+                 acc.add(R.id[e] == qid, dx, dy, R.hx[e], R.hy[e]);
+               });
+  float nx, ny, nhx, nhy;
+  flock_finish(acc, rp.p, qid, step, qx, qy, R.hx[gi], R.hy[gi], g.w, &nx, &ny, &nhx, &nhy);
+  flock_emit(g, rp.bin_base, W, gi, nx, ny, nhx, nhy, count_next, err);
+}
+
+}  // namespace kg

@@ -1,0 +1,153 @@
+// emu_check.cpp -- runs the device kernel phases on the CPU (kg_emu.cpp)
+// under AddressSanitizer/UBSan and compares every run bit for bit with the
+// oracle.  Exit status 0 = all scenarios identical and no sanitizer report.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "kg_geom.hpp"
+#include "oracle.h"
+
+extern "C" {
+struct emu_field;
+emu_field* emu_create(float w, float h, float d, int tor, uint32_t cap, uint32_t bin_rows,
+                      int lds_cap);
+void emu_destroy(emu_field* f);
+void emu_options(emu_field* f, int force_global, uint64_t shuffle_seed);
+void emu_stats(emu_field* f, uint64_t* lds_tiles, uint64_t* global_tiles);
+int emu_set(emu_field* f, const uint32_t* ids, const float* x, const float* y, const float* hx,
+            const float* hy, uint32_t n);
+int emu_lazy_update(emu_field* f);
+int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step);
+uint32_t emu_read(emu_field* f, uint32_t* ids, float* x, float* y, float* hx, float* hy);
+}
+
+struct Scenario {
+  const char* name;
+  uint32_t n;
+  float side, d;
+  int tor;
+  uint32_t steps;
+  uint64_t seed;
+  uint32_t bin_rows;
+  int lds_cap;
+  int force_global;
+  uint64_t shuffle;
+  int layout;  // 0 uniform, 1 clustered, 2 hugging the far edges (padding cells)
+  float radius;
+};
+
+static void make_positions(const Scenario& s, std::vector<float>& x, std::vector<float>& y,
+                           std::vector<float>& hx, std::vector<float>& hy) {
+  x.resize(s.n);
+  y.resize(s.n);
+  hx.resize(s.n);
+  hy.resize(s.n);
+  or_init_uniform(s.n, s.side, s.side, s.seed, 1, x.data(), y.data(), hx.data(), hy.data());
+  std::mt19937_64 rng(s.seed * 31 + 7);
+  if (s.layout == 1) {
+    std::normal_distribution<double> nd(0.0, s.side / 40.0);
+    std::uniform_real_distribution<double> ud(0.0, s.side);
+    double cx[6], cy[6];
+    for (int k = 0; k < 6; ++k) {
+      cx[k] = ud(rng);
+      cy[k] = ud(rng);
+    }
+    for (uint32_t i = 0; i < s.n; ++i) {
+      if (i % 5 == 0) continue;  // 20 % stay uniform
+      const int k = (int)(i % 6);
+      x[i] = or_toroidal_transform((float)(cx[k] + nd(rng)), s.side);
+      y[i] = or_toroidal_transform((float)(cy[k] + nd(rng)), s.side);
+    }
+  } else if (s.layout == 2) {
+    for (uint32_t i = 0; i < s.n; i += 3) {
+      // the largest floats below the side: x/d may round up to ceil(w/d),
+      // the padding column/row the reference's window never scans
+      const int k = (int)(i % 7);
+      float v = s.side;
+      for (int j = 0; j <= k; ++j) v = std::nextafter(v, 0.0f);
+      if (i % 2) x[i] = v;
+      else y[i] = v;
+    }
+  }
+}
+
+static int run(const Scenario& s) {
+  std::vector<float> x, y, hx, hy;
+  make_positions(s, x, y, hx, hy);
+  or_flock_params op{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.7f, s.radius, s.seed};
+  or_flocker* fl = or_flocker_new(s.n, s.side, s.side, s.d, s.tor, x.data(), y.data(), hx.data(),
+                                  hy.data(), &op);
+  emu_field* e = emu_create(s.side, s.side, s.d, s.tor, s.n, s.bin_rows, s.lds_cap);
+  if (!e) {
+    std::printf("%-28s emu_create failed\n", s.name);
+    return 1;
+  }
+  emu_options(e, s.force_global, s.shuffle);
+  std::vector<uint32_t> ids(s.n);
+  for (uint32_t i = 0; i < s.n; ++i) ids[i] = i;
+  emu_set(e, ids.data(), x.data(), y.data(), hx.data(), hy.data(), s.n);
+  int rc = emu_lazy_update(e);
+  kg_flock_params kp{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.7f, s.radius, 0u, s.seed};
+  for (uint32_t k = 0; k < s.steps && rc == 0; ++k) {
+    rc = emu_flock_step(e, &kp, k);
+    if (rc == 0) rc = emu_lazy_update(e);
+    or_flocker_step(fl);
+  }
+  std::vector<uint32_t> gid(s.n);
+  std::vector<float> gx(s.n), gy(s.n), ghx(s.n), ghy(s.n);
+  const uint32_t m = emu_read(e, gid.data(), gx.data(), gy.data(), ghx.data(), ghy.data());
+  std::vector<float> rx(s.n), ry(s.n), rhx(s.n), rhy(s.n);
+  or_flocker_get(fl, rx.data(), ry.data(), rhx.data(), rhy.data());
+  uint64_t lds = 0, glob = 0;
+  emu_stats(e, &lds, &glob);
+  uint32_t bad = 0;
+  std::vector<int> seen(s.n, 0);
+  for (uint32_t k = 0; k < m; ++k) {
+    const uint32_t i = gid[k];
+    if (i >= s.n || seen[i]++) {
+      ++bad;
+      continue;
+    }
+    if (std::memcmp(&gx[k], &rx[i], 4) || std::memcmp(&gy[k], &ry[i], 4) ||
+        std::memcmp(&ghx[k], &rhx[i], 4) || std::memcmp(&ghy[k], &rhy[i], 4))
+      ++bad;
+  }
+  if (m != s.n) bad += s.n;
+  std::printf("%-28s rc=%d agents=%u lds_tiles=%llu global_tiles=%llu mismatches=%u\n", s.name,
+              rc, m, (unsigned long long)lds, (unsigned long long)glob, bad);
+  emu_destroy(e);
+  or_flocker_free(fl);
+  return (rc != 0 || bad != 0) ? 1 : 0;
+}
+
+int main() {
+  const float D = 10.0f / 1.5f;
+  const Scenario sc[] = {
+      {"flockers-200", 1000, 200.0f, D, 1, 40, 0, 0, 0, 0, 0, 0, 10.0f},
+      {"flockers-200-global", 1000, 200.0f, D, 1, 10, 0, 0, 0, 1, 0, 0, 10.0f},
+      {"flockers-200-shuffled", 1000, 200.0f, D, 1, 20, 0, 0, 0, 0, 99, 0, 10.0f},
+      {"flockers-200-hb16-cap64", 1000, 200.0f, D, 1, 10, 1, 16, 64, 0, 0, 0, 10.0f},
+      {"sparse-2048", 10000, 2048.0f, D, 1, 6, 2, 0, 0, 0, 5, 0, 10.0f},
+      {"dense-60", 3000, 60.0f, D, 1, 8, 3, 0, 0, 0, 0, 0, 10.0f},
+      {"clustered", 6000, 700.0f, D, 1, 8, 4, 0, 0, 0, 0, 1, 10.0f},
+      {"clustered-cap128", 6000, 700.0f, D, 1, 4, 4, 0, 128, 0, 3, 1, 10.0f},
+      {"padding-edges", 3000, 200.0f, D, 1, 12, 5, 0, 0, 0, 0, 2, 10.0f},
+      {"padding-edges-global", 3000, 200.0f, D, 1, 6, 5, 0, 0, 1, 0, 2, 10.0f},
+      {"radius-5x5", 2500, 160.0f, 4.0f, 1, 6, 6, 0, 0, 0, 0, 0, 10.0f},
+      {"fine-cells-global", 500, 10.0f, 0.5f, 1, 5, 7, 0, 0, 0, 0, 0, 10.0f},
+      {"non-toroidal", 1500, 150.0f, D, 0, 6, 8, 0, 0, 0, 0, 0, 10.0f},
+      {"partial-cells", 1200, 151.3f, 6.1f, 1, 8, 9, 0, 0, 0, 0, 0, 10.0f},
+      // w = 1024, d = 1.1596829: the largest float below w discretizes to
+      // ceil(w/d) = 883, the padding column/row no window scans
+      {"padding-reachable", 4000, 1024.0f, 1.1596828699111938f, 1, 10, 10, 0, 0, 0, 0, 2, 2.0f},
+      {"padding-reachable-global", 4000, 1024.0f, 1.1596828699111938f, 1, 4, 10, 0, 0, 1, 0, 2,
+       2.0f},
+  };
+  int fails = 0;
+  for (const Scenario& s : sc) fails += run(s);
+  std::printf("%d scenario(s) failed\n", fails);
+  return fails ? 1 : 0;
+}
