@@ -140,10 +140,17 @@ int kg_flock_step(kg_field *f, const kg_flock_params *p, uint64_t step);
 /* The simulate! loop (src/lib.rs:1163-1173) for a flockers state:
  * `nsteps` calls of Schedule::step starting at schedule step `first_step`,
  * each = [lazy_update if the step is 0] + kg_flock_step + lazy_update
- * (schedule.rs:349-351, 395-412).  With use_graph != 0 one step is captured
- * in a HIP graph and replayed.  Asynchronous: call kg_field_sync to wait. */
+ * (schedule.rs:349-351, 395-412).  Asynchronous: call kg_field_sync to wait.
+ *
+ * Toroidal fields whose window (floor(radius/d) cells) fits one tile of
+ * halo run each whole step as ONE kernel launch on a tile-resident layout;
+ * other fields run flock + scan + scatter.  Results are identical.  `flags`:
+ * KG_RUN_GRAPH replays launches from HIP graphs; KG_RUN_BINNED forces the
+ * three-kernel path. */
+#define KG_RUN_GRAPH 1
+#define KG_RUN_BINNED 2
 int kg_schedule_run(kg_field *f, const kg_flock_params *p, uint64_t first_step, uint32_t nsteps,
-                    int use_graph);
+                    int flags);
 
 /* Copies a buffer (0 = read, 1 = write) to the host in storage order.  Any
  * output pointer may be NULL.  *n receives the agent count; KG_E_CAPACITY
@@ -166,7 +173,9 @@ void *kg_field_stream(kg_field *f);
 #define KG_KERNEL_SCATTER 2        /* rebinning of the flock output */
 #define KG_KERNEL_COUNT_APPEND 3   /* histogram of set_object_location agents */
 #define KG_KERNEL_SCATTER_APPEND 4 /* binning of set_object_location agents */
-#define KG_KERNEL_COUNT 5
+#define KG_KERNEL_RT_STEP 5        /* tile-resident schedule step */
+#define KG_KERNEL_RT_LOAD 6        /* entry into the tile-resident layout */
+#define KG_KERNEL_COUNT 7
 int kg_field_profile(kg_field *f, int enable);
 int kg_field_profile_read(kg_field *f, int kernel, double *total_ms, uint64_t *launches);
 

@@ -1,40 +1,26 @@
 // kg_field.cpp -- C ABI of one device-resident Field2D<Bird> (krabgpu.h).
 //
 // A kg_field is a replica table of one: the same kernels drive the batched
-// ensemble.  Buffers (capacity `cap` agents each, SoA):
-//   R, R2  read buffer and the rebinning target (swapped by lazy_update)
-//   W      write buffer produced by the flock kernel, aligned with R
-//   A      write buffer filled by set_object_location / init, insertion order
+// ensemble.  The read buffer lives in one of two layouts:
+//   bin-grouped (API layout)   R, R2 read buffer and rebinning target,
+//                              W     write buffer produced by the flock
+//                                    kernel, aligned with R,
+//                              A     write buffer filled by
+//                                    set_object_location / init;
+//   tile-resident (kg_rt.hpp)  used by kg_schedule_run: one launch per
+//                              schedule step.
+// API calls that read or write the buffers move the field back to the
+// bin-grouped layout first.
 #include <algorithm>
 #include <cstring>
 #include <map>
 #include <vector>
 
-#include "kg_common.hpp"
+#include "kg_rt_state.hpp"
 
 using namespace kg;
 
 namespace {
-
-struct DevBuf {
-  Buf b{};
-  hipError_t alloc(uint32_t cap) {
-    hipError_t e;
-    if ((e = dmalloc(&b.x, cap)) != hipSuccess) return e;
-    if ((e = dmalloc(&b.y, cap)) != hipSuccess) return e;
-    if ((e = dmalloc(&b.hx, cap)) != hipSuccess) return e;
-    if ((e = dmalloc(&b.hy, cap)) != hipSuccess) return e;
-    return dmalloc(&b.id, cap);
-  }
-  void release() {
-    (void)hipFree(b.x);
-    (void)hipFree(b.y);
-    (void)hipFree(b.hx);
-    (void)hipFree(b.hy);
-    (void)hipFree(b.id);
-    b = Buf{};
-  }
-};
 
 struct GraphKey {
   uint8_t params[sizeof(kg_flock_params)];
@@ -76,6 +62,12 @@ struct kg_field {
   size_t density_estimation = 0;
 
   std::map<GraphKey, hipGraphExec_t> graphs;
+
+  // tile-resident layout
+  RtState rt;
+  bool rt_mode = false;     // the read buffer lives in the tile-resident layout
+  bool rt_replan = false;   // its capacities were exceeded: re-plan before the next run
+  size_t lds_limit = 65536;
 
   // HIP-event timing of individual kernel launches (kg_field_profile)
   bool prof_on = false;
@@ -167,8 +159,21 @@ int check_poison(kg_field* f) {
   return KG_OK;
 }
 
+int ensure_atomic(kg_field* f);
+
 // Reads the device error flag (synchronises the stream).
+int sync_only(kg_field* f) {
+  KG_HIP(hipStreamSynchronize(f->stream));
+  uint32_t e = 0;
+  KG_HIP(hipMemcpy(&e, f->err, sizeof(e), hipMemcpyDeviceToHost));
+  if (e) f->poisoned = true;
+  return check_poison(f);
+}
+
+// Brings the read buffer into the bin-grouped layout, synchronises and
+// reads the device error flag.
 int sync_and_check(kg_field* f) {
+  if (int rc = ensure_atomic(f)) return rc;
   KG_HIP(hipStreamSynchronize(f->stream));
   uint32_t e = 0;
   KG_HIP(hipMemcpy(&e, f->err, sizeof(e), hipMemcpyDeviceToHost));
@@ -236,6 +241,101 @@ int set_params(kg_field* f, const kg_flock_params* p, uint64_t step_base) {
   return upload_rep(f);
 }
 
+// ---- tile-resident layout -------------------------------------------------
+
+// Reads the tile-resident flags (synchronises): a location out of the grid
+// poisons the field; exceeded capacities ask for a new plan.
+int rt_check_flags(kg_field* f) {
+  if (!f->rt_mode) return KG_OK;
+  KG_HIP(hipStreamSynchronize(f->stream));
+  uint32_t fl = 0;
+  KG_HIP(hipMemcpy(&fl, f->rt.flags, sizeof(fl), hipMemcpyDeviceToHost));
+  if (fl & kRtFlagRange) f->poisoned = true;
+  if (fl & (kRtFlagSpill | kRtFlagGlobal | kRtFlagStayCap)) f->rt_replan = true;
+  return check_poison(f);
+}
+
+// Moves the bin-grouped read buffer into the tile-resident layout, with
+// capacities planned from the current per-tile counts.
+int rt_enter(kg_field* f) {
+  const uint32_t nt = f->g.ntiles;
+  const uint32_t nbt = (uint32_t)(f->g.tx * f->g.tr);
+  std::vector<uint32_t> off((size_t)f->g.nbins + 1);
+  KG_HIP(hipStreamSynchronize(f->stream));
+  KG_HIP(hipMemcpy(off.data(), f->offset, off.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  std::vector<uint32_t> counts(nt), nb(nt, nbt);
+  for (uint32_t t = 0; t < nt; ++t) counts[t] = off[(size_t)(t + 1) * nbt] - off[(size_t)t * nbt];
+  RtPlan plan;
+  rt_plan(counts, nb, &plan);
+  if (int rc = f->rt.setup(plan, f->cap, f->stream)) return rc;
+  f->rt.caps = rt_caps(plan, f->g, f->h_rep.disc_dist, f->lds_limit);
+  if (int rc = f->rt.reset(1, f->stream)) return rc;
+  {
+    ProfScope ps(f, KG_KERNEL_RT_LOAD);
+    KG_HIP(launch_rt_step(f->worklist(), f->rt.dev(), f->rt.caps, 1, f->R.b, f->offset,
+                          f->rt.G - 1, f->stream));
+  }
+  f->rt_mode = true;
+  f->rt_replan = false;
+  return KG_OK;
+}
+
+// Moves the tile-resident read buffer back to the bin-grouped layout.
+int ensure_atomic(kg_field* f) {
+  if (!f->rt_mode) return KG_OK;
+  if (int rc = rt_check_flags(f)) return rc;
+  KG_HIP(launch_fill_u32(f->scratch, 0u, 1, f->stream));
+  KG_HIP(launch_rt_unload(f->worklist(), f->rt.dev(), f->A.b, f->scratch, f->stream));
+  uint32_t n = 0;
+  KG_HIP(hipMemcpyAsync(&n, f->scratch, sizeof(n), hipMemcpyDeviceToHost, f->stream));
+  KG_HIP(hipStreamSynchronize(f->stream));
+  f->rt_mode = false;
+  f->nA = n;
+  f->nW = 0;
+  f->w_aligned = false;
+  return lazy_update_impl(f, nullptr);
+}
+
+// `nsteps` tile-resident schedule steps, one launch each (graph replays of
+// 8 launches when use_graph is set and profiling is off).
+int rt_steps(kg_field* f, uint32_t nsteps, int use_graph) {
+  const RtDev D = f->rt.dev();
+  const WorkList wl = f->worklist();
+  uint32_t left = nsteps;
+  constexpr int kChunk = 8;
+  if (use_graph && !f->prof_on && left >= kChunk) {
+    hipGraphExec_t exec = nullptr;
+    for (auto& gch : f->rt.graphs)
+      if (gch.first == kChunk) exec = gch.second;
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      KG_HIP(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e = hipSuccess;
+      for (int k = 0; k < kChunk && e == hipSuccess; ++k)
+        e = launch_rt_step(wl, D, f->rt.caps, 0, f->R.b, f->offset, 0, f->stream);
+      const hipError_t e2 = hipStreamEndCapture(f->stream, &graph);
+      if (e != hipSuccess || e2 != hipSuccess)
+        return set_error(KG_E_HIP, "graph capture: %s",
+                         hipGetErrorString(e != hipSuccess ? e : e2));
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) return set_error(KG_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+      f->rt.graphs.push_back({kChunk, exec});
+    }
+    while (left >= (uint32_t)kChunk) {
+      KG_HIP(hipGraphLaunch(exec, f->stream));
+      left -= kChunk;
+    }
+  }
+  while (left > 0) {
+    ProfScope ps(f, KG_KERNEL_RT_STEP);
+    KG_HIP(launch_rt_step(wl, D, f->rt.caps, 0, f->R.b, f->offset, 0, f->stream));
+    --left;
+  }
+  f->rt.G += nsteps;
+  return KG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -252,12 +352,19 @@ int kg_field_create(const kg_field_desc* desc, kg_field** out) {
   f->desc = *desc;
   f->cap = desc->capacity;
   std::string why;
-  if (!make_geom(desc->width, desc->height, desc->discretization, desc->toroidal, desc->capacity,
-                 desc->bin_rows, &f->g, &why)) {
+  if (!make_geom_rt(desc->width, desc->height, desc->discretization, desc->toroidal,
+                    desc->capacity, 0, 0, desc->bin_rows, &f->g, &why)) {
     delete f;
     return set_error(KG_E_INVALID, "%s", why.c_str());
   }
   f->lds_cap = clamp_lds_cap(pick_lds_cap(f->g, desc->capacity));
+  {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        v > 8192)
+      f->lds_limit = (size_t)v - 6144;  // static LDS of the tile-resident kernel
+  }
   f->h_rep.g = f->g;
   f->h_rep.bin_base = 0;
   kg_flock_params dflt{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.7f, 10.0f, 0u, 0ull};
@@ -315,6 +422,7 @@ void kg_field_destroy(kg_field* f) {
     (void)hipEventDestroy(pe.second.second);
   }
   for (hipEvent_t e : f->ev_pool) (void)hipEventDestroy(e);
+  f->rt.release();
   f->R.release();
   f->R2.release();
   f->W.release();
@@ -341,6 +449,7 @@ int kg_set_object_locations(kg_field* f, const uint32_t* ids, const float* x, co
   clear_error();
   if (!f) return set_error(KG_E_INVALID, "field is NULL");
   if (int rc = check_poison(f)) return rc;
+  if (int rc = ensure_atomic(f)) return rc;
   if (n == 0) return KG_OK;
   if (!ids || !x || !y) return set_error(KG_E_INVALID, "ids, x and y must not be NULL");
   if (f->w_aligned)
@@ -376,6 +485,7 @@ int kg_init_uniform(kg_field* f, uint32_t n, uint64_t seed, int heading_mode) {
   clear_error();
   if (!f) return set_error(KG_E_INVALID, "field is NULL");
   if (int rc = check_poison(f)) return rc;
+  if (int rc = ensure_atomic(f)) return rc;
   if (heading_mode != 0 && heading_mode != 1)
     return set_error(KG_E_INVALID, "heading_mode must be 0 or 1");
   if (f->w_aligned)
@@ -392,6 +502,7 @@ int kg_remove_object_location(kg_field* f, uint32_t id, float x, float y) {
   clear_error();
   if (!f) return set_error(KG_E_INVALID, "field is NULL");
   if (int rc = check_poison(f)) return rc;
+  if (int rc = ensure_atomic(f)) return rc;
   if (f->w_aligned) {
     KG_HIP(launch_materialize(f->R.b, f->W.b, f->nW, f->A.b, f->stream));
     f->nA = f->nW;
@@ -412,6 +523,7 @@ int kg_lazy_update(kg_field* f) {
   clear_error();
   if (!f) return set_error(KG_E_INVALID, "field is NULL");
   if (int rc = check_poison(f)) return rc;
+  if (int rc = ensure_atomic(f)) return rc;
   return lazy_update_impl(f, nullptr);
 }
 
@@ -571,6 +683,7 @@ int kg_flock_step(kg_field* f, const kg_flock_params* p, uint64_t step) {
   clear_error();
   if (!f) return set_error(KG_E_INVALID, "field is NULL");
   if (int rc = check_poison(f)) return rc;
+  if (int rc = ensure_atomic(f)) return rc;
   if (f->nW || f->nA || f->w_aligned)
     return set_error(KG_E_STATE, "the write buffer must be empty before a flock step");
   if (int rc = set_params(f, p, step)) return rc;
@@ -590,13 +703,35 @@ int kg_schedule_run(kg_field* f, const kg_flock_params* p, uint64_t first_step, 
   if (nsteps == 0) return KG_OK;
   if (f->w_aligned) return set_error(KG_E_STATE, "a flock step output is pending");
   // schedule.rs:349-351: step 0 first calls state.update(0)
-  if (first_step == 0)
+  if (first_step == 0) {
+    if (int rc = ensure_atomic(f)) return rc;
     if (int rc = lazy_update_impl(f, nullptr)) return rc;
+  }
   if (f->nA) return set_error(KG_E_STATE, "the write buffer must be empty");
+  if (!p) return set_error(KG_E_INVALID, "params is NULL");
+  {
+    RepDev probe = f->h_rep;
+    set_rep_params(&probe, *p);
+    if (!(use_graph & KG_RUN_BINNED) && probe.lds_ok) {
+      // tile-resident path: one launch per schedule step
+      if (f->rt_mode)
+        if (int rc = rt_check_flags(f)) return rc;
+      if (f->rt_mode && (f->rt_replan || probe.disc_dist != f->h_rep.disc_dist)) {
+        if (int rc = ensure_atomic(f)) return rc;
+      }
+      if (int rc = set_params(f, p, 0)) return rc;
+      if (!f->rt_mode)
+        if (int rc = rt_enter(f)) return rc;
+      // schedule step of the next launch = step_base + *gstep
+      if (int rc = set_params(f, p, first_step - f->rt.G)) return rc;
+      return rt_steps(f, nsteps, use_graph & KG_RUN_GRAPH);
+    }
+  }
+  if (int rc = ensure_atomic(f)) return rc;
   if (int rc = set_params(f, p, first_step)) return rc;
   KG_HIP(hipMemsetAsync(f->gstep, 0, sizeof(uint64_t), f->stream));
   uint32_t left = nsteps;
-  if (use_graph && left >= 2 && !f->prof_on) {
+  if ((use_graph & KG_RUN_GRAPH) && left >= 2 && !f->prof_on) {
     GraphKey key;
     std::memcpy(key.params, p, sizeof(kg_flock_params));
     key.parity = f->parity;
@@ -664,7 +799,8 @@ int kg_read_agents(kg_field* f, int buffer, uint32_t* ids, float* x, float* y, f
 int kg_field_sync(kg_field* f) {
   clear_error();
   if (!f) return set_error(KG_E_INVALID, "field is NULL");
-  return sync_and_check(f);
+  if (int rc = sync_only(f)) return rc;
+  return rt_check_flags(f);
 }
 
 void* kg_field_stream(kg_field* f) { return f ? (void*)f->stream : nullptr; }

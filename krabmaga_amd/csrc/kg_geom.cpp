@@ -135,3 +135,109 @@ void set_rep_params(RepDev* r, const kg_flock_params& p) {
 }
 
 }  // namespace kg
+
+namespace kg {
+
+void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& nb, RtPlan* plan) {
+  const size_t nt = counts.size();
+  plan->sbase.resize(nt);
+  plan->scap.resize(nt);
+  plan->ibase.resize(nt);
+  plan->icap.resize(nt);
+  plan->tboff.resize(nt);
+  uint64_t s = 0, ib = 0, bo = 0;
+  plan->max_scap = plan->max_icap = plan->max_nb = 0;
+  for (size_t t = 0; t < nt; ++t) {
+    const uint32_t c = counts[t];
+    // room for fluctuation: a quarter more than now, at least 16
+    uint32_t sc = c + std::max<uint32_t>(16u, c / 4u);
+    sc = (sc + 15u) & ~15u;
+    uint32_t ic = std::max<uint32_t>(16u, c / 8u + 8u);
+    ic = (ic + 7u) & ~7u;
+    plan->sbase[t] = (uint32_t)s;
+    plan->scap[t] = sc;
+    plan->ibase[t] = (uint32_t)ib;
+    plan->icap[t] = ic;
+    plan->tboff[t] = (uint32_t)bo;
+    s += sc;
+    ib += ic;
+    bo += nb[t] + 1u;
+    plan->max_scap = std::max(plan->max_scap, sc);
+    plan->max_icap = std::max(plan->max_icap, ic);
+    plan->max_nb = std::max(plan->max_nb, nb[t]);
+  }
+  plan->stay_total = s;
+  plan->inbox_total = ib;
+  plan->boff_total = bo;
+}
+
+RtCaps rt_caps(const RtPlan& plan, const Geom& g, int32_t dd, size_t lds_limit) {
+  RtCaps c;
+  // staged stayers per own stayer: the tile plus dd halo columns each side
+  // and one whole bin (hb rows) above and below
+  const double halo = std::min(9.0, (double)(g.tx + 2 * dd) * (double)(g.ty + 2 * g.hb) /
+                                        ((double)g.tx * (double)g.ty));
+  c.nb = (int)std::max<uint32_t>(plan.max_nb, 1u);
+  c.own = (int)std::min<uint32_t>((uint32_t)kRtMaxOwn, (plan.max_scap + plan.max_icap + 63u) & ~63u);
+  c.runs = (int)std::min<double>(16384.0, std::ceil(plan.max_scap * halo * 1.25 / 64.0) * 64.0);
+  c.extras = (int)std::min<uint32_t>(1024u, std::max<uint32_t>(128u, (3u * plan.max_icap + 63u) & ~63u));
+  while (rt_smem_bytes(c) > lds_limit && (c.runs > 128 || c.own > 64)) {
+    if (c.runs > 128) c.runs -= 64;
+    if (rt_smem_bytes(c) > lds_limit && c.own > 64) c.own -= 64;
+  }
+  return c;
+}
+
+bool make_geom_rt(float w, float h, float d, int tor, uint64_t expected_agents, uint32_t tx,
+                  uint32_t ty, uint32_t hb, Geom* g, std::string* why) {
+  if (!make_geom(w, h, d, tor, expected_agents, hb, g, why)) return false;
+  const double cells = (double)g->dw * (double)g->dh;
+  const double dens = expected_agents ? (double)expected_agents / cells : 0.1;
+  if (!tx || !ty) {
+    // tiles of about 448 agents, four times taller than wide
+    const double tile_cells = std::max(16.0, 448.0 / std::max(dens, 1e-6));
+    double wdt = std::sqrt(tile_cells / 4.0);
+    int tl = 0;
+    while ((1 << (tl + 1)) <= wdt && tl < 6) ++tl;
+    tx = 1u << tl;
+    int yl = 0;
+    while ((double)(1 << (yl + 1)) <= tile_cells / tx && yl < 12) ++yl;
+    ty = 1u << yl;
+  }
+  if (!hb) {
+    hb = 1u << g->hb_log2;
+    while (hb > ty) hb >>= 1;
+  }
+  if ((tx & (tx - 1)) || (ty & (ty - 1)) || (hb & (hb - 1)) || hb > ty || tx > 128) {
+    *why = "tile sizes must be powers of two with bin rows <= tile rows and tile columns <= 128";
+    return false;
+  }
+  int a = 0;
+  while ((1u << a) < tx) ++a;
+  int b = 0;
+  while ((1u << b) < hb) ++b;
+  int c = 0;
+  while ((1u << c) < ty) ++c;
+  while (a > 0 && (1 << (a - 1)) >= g->dw) --a;
+  while (c > b && (1 << (c - 1)) >= g->dh) --c;
+  g->tx_log2 = a;
+  g->hb_log2 = b;
+  g->tr_log2 = c - b;
+  g->tx = 1 << a;
+  g->hb = 1 << b;
+  g->tr = 1 << g->tr_log2;
+  g->ty = g->tr * g->hb;
+  g->ntx = (g->dw + g->tx - 1) / g->tx;
+  g->nty = (g->dh + g->ty - 1) / g->ty;
+  const uint64_t ntiles = (uint64_t)g->ntx * (uint64_t)g->nty;
+  const uint64_t nbins = ntiles * (uint64_t)g->tx * (uint64_t)g->tr;
+  if (nbins >= (1ull << 31) || g->tx * g->tr > kRtMaxBins) {
+    *why = "tile-resident layout: too many bins";
+    return false;
+  }
+  g->ntiles = (uint32_t)ntiles;
+  g->nbins = (uint32_t)nbins;
+  return true;
+}
+
+}  // namespace kg

@@ -2,9 +2,10 @@
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "../../include/krabgpu.h"
-#include "kg_types.hpp"
+#include "kg_rt.hpp"
 
 namespace kg {
 
@@ -26,5 +27,22 @@ void set_rep_params(RepDev* r, const kg_flock_params& p);
 int pick_lds_cap(const Geom& g, uint64_t expected_agents);
 
 bool params_valid(const kg_flock_params& p, std::string* why);
+
+// Tile-resident layout plan: per-tile stayer and inbox capacities and bases
+// from per-tile agent counts; nb[t] is tx * tr of tile t's replica.
+struct RtPlan {
+  std::vector<uint32_t> sbase, scap, ibase, icap, tboff;
+  uint64_t stay_total = 0, inbox_total = 0, boff_total = 0;
+  uint32_t max_scap = 0, max_icap = 0, max_nb = 0;
+};
+void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& nb, RtPlan* plan);
+
+// LDS capacities of the tile-resident kernel for a plan, the tile geometry
+// and the window reach dd, limited to `lds_limit` bytes.
+RtCaps rt_caps(const RtPlan& plan, const Geom& g, int32_t dd, size_t lds_limit);
+
+// Geometry for the tile-resident layout: tiles of about `own_target` agents.
+bool make_geom_rt(float w, float h, float d, int tor, uint64_t expected_agents, uint32_t tx,
+                  uint32_t ty, uint32_t hb, Geom* g, std::string* why);
 
 }  // namespace kg

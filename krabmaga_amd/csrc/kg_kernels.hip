@@ -8,6 +8,7 @@
 //             + bin histogram of the new positions
 //   query/... single-replica API queries             field_2d.rs:472-808
 #include "kg_phases.hpp"
+#include "kg_rt.hpp"
 
 namespace kg {
 
@@ -505,6 +506,295 @@ __global__ void k_remove(const RepDev* __restrict__ rep, Buf A, uint32_t n, uint
 hipError_t launch_remove(const RepDev* rep, Buf A, uint32_t n, uint32_t id, float x, float y,
                          uint32_t* n_after, hipStream_t s) {
   hipLaunchKernelGGL(k_remove, dim3(1), dim3(64), 0, s, rep, A, n, id, x, y, n_after);
+  return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------------------
+// tile-resident step (phase bodies in kg_rt.hpp)
+
+size_t rt_lds_bytes(const RtCaps& c) { return rt_smem_bytes(c); }
+
+__global__ __launch_bounds__(kRtThreads) void k_rt_step(
+    const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, RtDev D,
+    RtCaps caps, int load, Buf Ra, const uint32_t* __restrict__ offset_a, uint64_t load_g) {
+  extern __shared__ __align__(16) uint8_t smem_raw[];
+  __shared__ uint32_t s_seg_start[3 * kMaxStageCols];
+  __shared__ uint32_t s_seg_len[3 * kMaxStageCols];
+  __shared__ uint32_t s_col_base[kMaxStageCols + 1];
+  __shared__ uint32_t s_ecol_base[kMaxStageCols + 1];
+  __shared__ uint32_t s_src_base[10];
+  __shared__ uint32_t s_src_pre[11];
+  __shared__ uint32_t s_warp[kRtThreads / kWave];
+  __shared__ uint32_t s_ne, s_nown, s_nstay, s_fail;
+
+  const int t = threadIdx.x;
+  const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+  const uint64_t g = load ? load_g : *D.gstep;
+  const RtTile rt = rt_tile(reps, nreps, tile_prefix, item, g);
+  const RepDev& rp = reps[rt.r];
+  const RtSmem S = rt_carve(smem_raw, caps);
+  const int nb_tile = rp.g.tx * rp.g.tr;
+  const int pout = rt.pin ^ 1;
+
+  float nx[kRtOwnPerThread], ny[kRtOwnPerThread], nhx[kRtOwnPerThread], nhy[kRtOwnPerThread];
+  uint32_t nid[kRtOwnPerThread];
+  uint32_t nown = 0;
+  bool global_path = false;
+  if (t == 0) {
+    s_ne = 0;
+    s_nown = 0;
+    s_nstay = 0;
+    s_fail = 0;
+  }
+
+  if (!load) {
+    const bool lds_ok = rp.lds_ok != 0 && rt.ncols <= kMaxStageCols && nb_tile <= caps.nb;
+    uint32_t len = 0;
+    if (lds_ok && t < rt.ncols) len = rt_segments(rp, rt, D, t, s_seg_start, s_seg_len);
+    if (t < 10) {
+      uint32_t b;
+      const uint32_t n = rt_extra_count(rp, rt, D, t, &b);
+      s_src_base[t] = b;
+      s_src_pre[t + 1] = n;
+    }
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan<kRtThreads>(len, s_warp, &total);
+    if (lds_ok && t < rt.ncols) s_col_base[t] = ex;
+    if (t == 0) {
+      s_col_base[rt.ncols] = total;
+      s_src_pre[0] = 0;
+      for (int k = 0; k < 10; ++k) s_src_pre[k + 1] += s_src_pre[k];
+    }
+    __syncthreads();
+    global_path = !lds_ok || total > (uint32_t)caps.runs;
+    if (!global_path) {
+      const uint32_t next = s_src_pre[10];
+      for (uint32_t j = t; j < next; j += kRtThreads)
+        if (!rt_gather_extra(rp, rt, D, j, s_src_base, s_src_pre, caps.extras, &s_ne, S))
+          s_fail = 1u;
+      for (uint32_t k = t; k < total; k += kRtThreads)
+        rt_stage(rp, rt, D, k, s_seg_start, s_seg_len, s_col_base, S);
+      __syncthreads();
+      global_path = s_fail != 0;
+      if (!global_path) {
+        const uint32_t ne = s_ne;
+        for (uint32_t j = t; j < ne; j += kRtThreads) rt_rank_extra(j, ne, S);
+        for (int c = t; c <= rt.ncols; c += kRtThreads) s_ecol_base[c] = rt_extra_col_base(c, ne, S);
+        for (uint32_t k = t; k < total; k += kRtThreads)
+          if (rt_own(rt, rt.sc_lo + S.scol[k], S.scy[k])) {
+            const uint32_t o = atomicAdd(&s_nown, 1u);
+            if (o < (uint32_t)caps.own) S.own[o] = (uint16_t)k;
+          }
+        for (uint32_t j = t; j < ne; j += kRtThreads)
+          if (rt_own(rt, rt.sc_lo + S.ecol[j], S.ecy[j])) {
+            const uint32_t o = atomicAdd(&s_nown, 1u);
+            if (o < (uint32_t)caps.own) S.own[o] = (uint16_t)(0x8000u | j);
+          }
+        __syncthreads();
+        nown = s_nown;
+        global_path = nown > (uint32_t)caps.own;
+        if (!global_path) {
+#pragma unroll
+          for (int jj = 0; jj < kRtOwnPerThread; ++jj) {
+            const uint32_t o = t + jj * kRtThreads;
+            if (o < nown)
+              rt_agent_lds(rp, rt, S.own[o], s_col_base, s_ecol_base, S, &nx[jj], &ny[jj],
+                           &nhx[jj], &nhy[jj], &nid[jj]);
+          }
+        }
+      }
+    }
+  } else {
+    // load: the tile's agents from the bin-grouped read buffer, unchanged
+    const uint32_t tile = rt.gt - rt.tbase;
+    const uint32_t b0 = rp.bin_base + (tile << (rp.g.tx_log2 + rp.g.tr_log2));
+    const uint32_t lo = offset_a[b0], hi = offset_a[b0 + (uint32_t)nb_tile];
+    nown = hi - lo;
+    global_path = nown > (uint32_t)caps.own || nb_tile > caps.nb;
+    if (global_path) {
+      for (uint32_t i = lo + t; i < hi; i += kRtThreads) {
+        int32_t cx, cy;
+        discretize(rp.g, Ra.x[i], Ra.y[i], &cx, &cy);
+        rt_push(rp, rt, D, rt.tbase + tile_of(rp.g, cx, cy), Ra.x[i], Ra.y[i], Ra.hx[i], Ra.hy[i],
+                Ra.id[i]);
+      }
+    } else {
+#pragma unroll
+      for (int jj = 0; jj < kRtOwnPerThread; ++jj) {
+        const uint32_t o = t + jj * kRtThreads;
+        if (o < nown) {
+          nx[jj] = Ra.x[lo + o];
+          ny[jj] = Ra.y[lo + o];
+          nhx[jj] = Ra.hx[lo + o];
+          nhy[jj] = Ra.hy[lo + o];
+          nid[jj] = Ra.id[lo + o];
+        }
+      }
+    }
+  }
+  __syncthreads();
+
+  if (global_path) {
+    if (!load) {
+      if (t == 0) atomicOr(D.flags, kRtFlagGlobal);
+      for (uint32_t o = t;; o += kRtThreads) {
+        float x, y, hx, hy;
+        uint32_t id;
+        bool skip;
+        if (!rt_own_global(rp, rt, D, o, &x, &y, &hx, &hy, &id, &skip)) break;
+        if (skip) continue;
+        float ax, ay, ahx, ahy;
+        rt_agent_global(rp, rt, D, x, y, hx, hy, id, &ax, &ay, &ahx, &ahy);
+        int32_t cx, cy;
+        discretize(rp.g, ax, ay, &cx, &cy);
+        if (!cell_valid(rp.g, cx, cy)) {
+          atomicOr(D.flags, kRtFlagRange);
+          continue;
+        }
+        rt_push(rp, rt, D, rt.tbase + tile_of(rp.g, cx, cy), ax, ay, ahx, ahy, id);
+      }
+    }
+    for (int b = t; b <= nb_tile; b += kRtThreads) D.boff[pout][D.tboff[rt.gt] + b] = 0u;
+    if (t == 0) D.cnt[pout][rt.gt] = 0u;
+  } else {
+#pragma unroll
+    for (int jj = 0; jj < kRtOwnPerThread; ++jj) {
+      const uint32_t o = t + jj * kRtThreads;
+      if (o < nown)
+        rt_classify(rp, rt, D, false, nx[jj], ny[jj], nhx[jj], nhy[jj], nid[jj], &s_nstay, S);
+    }
+    __syncthreads();
+    const uint32_t nstay = s_nstay;
+    if (nstay > D.scap[rt.gt]) {
+      if (t == 0) atomicOr(D.flags, kRtFlagStayCap);
+      for (uint32_t s = t; s < nstay; s += kRtThreads)
+        rt_push(rp, rt, D, rt.gt, S.ox[s], S.oy[s], S.ohx[s], S.ohy[s], S.oid[s]);
+      for (int b = t; b <= nb_tile; b += kRtThreads) D.boff[pout][D.tboff[rt.gt] + b] = 0u;
+      if (t == 0) D.cnt[pout][rt.gt] = 0u;
+    } else {
+      for (int b = t; b <= nb_tile; b += kRtThreads) S.hist[b] = 0u;
+      __syncthreads();
+      for (uint32_t s = t; s < nstay; s += kRtThreads) atomicAdd(&S.hist[S.obin[s]], 1u);
+      __syncthreads();
+      // exclusive scan of the nb_tile bin counts, 4 per thread
+      uint32_t v[4], sum = 0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int b = 4 * t + k;
+        v[k] = b < nb_tile ? S.hist[b] : 0u;
+        sum += v[k];
+      }
+      uint32_t tot;
+      uint32_t run = block_exclusive_scan<kRtThreads>(sum, s_warp, &tot);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int b = 4 * t + k;
+        if (b < nb_tile) {
+          S.bbase[b] = run;
+          S.bcur[b] = run;
+        }
+        run += v[k];
+      }
+      if (t == 0) S.bbase[nb_tile] = nstay;
+      __syncthreads();
+      for (uint32_t s = t; s < nstay; s += kRtThreads) {
+        const uint32_t pos = atomicAdd(&S.bcur[S.obin[s]], 1u);
+        S.operm[pos] = (uint16_t)s;
+      }
+      __syncthreads();
+      for (uint32_t s = t; s < nstay; s += kRtThreads) rt_write_stayer(rt, D, s, S);
+      for (int b = t; b <= nb_tile; b += kRtThreads)
+        D.boff[pout][D.tboff[rt.gt] + b] = S.bbase[b];
+      if (t == 0) D.cnt[pout][rt.gt] = nstay;
+    }
+  }
+
+  if (!load && t == 0) {
+    D.icnt[rt.ph_reset][rt.gt] = 0u;
+    if (rt.gt == 0) *D.scnt[rt.ph_reset] = 0u;
+    const uint32_t prev = __hip_atomic_fetch_add(D.done, 1u, __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == gridDim.x - 1) {
+      *D.gstep = g + 1;
+      __hip_atomic_store(D.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+hipError_t launch_rt_step(const WorkList& wl, const RtDev& D, const RtCaps& caps, int load,
+                          Buf Ra, const uint32_t* offset_a, uint64_t load_g, hipStream_t s) {
+  if (wl.ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rt_step, dim3(wl.ntiles), dim3(kRtThreads), rt_lds_bytes(caps), s, wl.reps,
+                     wl.nreps, wl.tile_prefix, D, caps, load, Ra, offset_a, load_g);
+  return hipGetLastError();
+}
+
+// Writes every agent of the tile-resident layout (stayers, inbox and spill
+// entries of the current parity) into the append buffer A; *n_out counts.
+__global__ void k_rt_unload(const RepDev* __restrict__ reps, int nreps,
+                            const uint32_t* __restrict__ tile_prefix, RtDev D, Buf A,
+                            uint32_t* n_out) {
+  const uint32_t item = blockIdx.x;
+  const uint64_t g = *D.gstep;
+  const RtTile rt = rt_tile(reps, nreps, tile_prefix, item, g);
+  const RepDev& rp = reps[rt.r];
+  const uint32_t ns = D.cnt[rt.pin][rt.gt];
+  uint32_t ib;
+  const uint32_t ni = rt_extra_count(rp, rt, D, 4, &ib);
+  uint32_t sb;
+  const uint32_t nsp = (rt.gt == 0) ? rt_extra_count(rp, rt, D, 9, &sb) : 0u;
+  for (uint32_t o = threadIdx.x; o < ns + ni + nsp; o += blockDim.x) {
+    const Buf* B;
+    uint32_t gi;
+    if (o < ns) {
+      B = &D.stay[rt.pin];
+      gi = D.sbase[rt.gt] + o;
+    } else if (o < ns + ni) {
+      B = &D.inbox[rt.pin];
+      gi = ib + (o - ns);
+    } else {
+      B = &D.spill[rt.pin];
+      gi = o - ns - ni;
+    }
+    const uint32_t k = atomicAdd(n_out, 1u);
+    A.x[k] = B->x[gi];
+    A.y[k] = B->y[gi];
+    A.hx[k] = B->hx[gi];
+    A.hy[k] = B->hy[gi];
+    A.id[k] = B->id[gi];
+  }
+}
+
+hipError_t launch_rt_unload(const WorkList& wl, const RtDev& D, Buf A, uint32_t* n_out,
+                            hipStream_t s) {
+  if (wl.ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rt_unload, dim3(wl.ntiles), dim3(256), 0, s, wl.reps, wl.nreps,
+                     wl.tile_prefix, D, A, n_out);
+  return hipGetLastError();
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+}  // namespace kg
+
+namespace kg {
+
+__global__ void k_set_u64(uint64_t* p, uint64_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+hipError_t launch_set_u64(uint64_t* p, uint64_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, p, v);
   return hipGetLastError();
 }
 

@@ -3,7 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
-#include "kg_types.hpp"
+#include "kg_rt.hpp"
 
 namespace kg {
 
@@ -25,6 +25,16 @@ hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W
                         uint32_t* count_next, const uint64_t* gstep, uint32_t* err, int lds_cap,
                         hipStream_t s);
 hipError_t launch_materialize(Buf R, Buf W, uint32_t n, Buf A, hipStream_t s);
+
+// Tile-resident step (one launch per schedule step); with load != 0 the
+// launch instead moves the bin-grouped read buffer (Ra, offset_a) into the
+// tile-resident layout that step `load_g + 1` reads.
+size_t rt_lds_bytes(const RtCaps& c);
+hipError_t launch_rt_step(const WorkList& wl, const RtDev& D, const RtCaps& caps, int load,
+                          Buf Ra, const uint32_t* offset_a, uint64_t load_g, hipStream_t s);
+hipError_t launch_rt_unload(const WorkList& wl, const RtDev& D, Buf A, uint32_t* n_out,
+                            hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 
 // Single-replica queries (API and parity use; not on the hot path).
 hipError_t launch_query(const RepDev* rep, Buf R, const uint32_t* offset, float x, float y,

@@ -1,0 +1,115 @@
+// kg_rt_state.cpp -- allocation and reset of the tile-resident layout.
+#include "kg_rt_state.hpp"
+
+namespace kg {
+
+RtDev RtState::dev() const {
+  RtDev D;
+  D.sbase = sbase;
+  D.scap = scap;
+  D.ibase = ibase;
+  D.icap = icap;
+  D.tboff = tboff;
+  for (int p = 0; p < 2; ++p) {
+    D.cnt[p] = cnt[p];
+    D.boff[p] = boff[p];
+    D.stay[p] = stay[p].b;
+    D.inbox[p] = inbox[p].b;
+    D.spill[p] = spill[p].b;
+    D.spill_rep[p] = spill_rep[p];
+  }
+  for (int k = 0; k < 3; ++k) {
+    D.icnt[k] = icnt[k];
+    D.scnt[k] = scnt + k;
+  }
+  D.spill_cap = (uint32_t)spill_alloc;
+  D.flags = flags;
+  D.done = done;
+  D.gstep = gstep;
+  return D;
+}
+
+void RtState::drop_graphs() {
+  for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+  graphs.clear();
+}
+
+void RtState::release() {
+  drop_graphs();
+  for (uint32_t* p : {sbase, scap, ibase, icap, tboff, cnt[0], cnt[1], boff[0], boff[1], icnt[0],
+                      icnt[1], icnt[2], scnt, spill_rep[0], spill_rep[1], flags, done})
+    (void)hipFree(p);
+  (void)hipFree(gstep);
+  for (int p = 0; p < 2; ++p) {
+    stay[p].release();
+    inbox[p].release();
+    spill[p].release();
+  }
+  *this = RtState();
+}
+
+int RtState::setup(const RtPlan& pl, uint64_t spill_cap, hipStream_t s) {
+  const uint64_t nt = pl.sbase.size();
+  drop_graphs();
+  if (!allocated || pl.stay_total > stay_alloc || pl.inbox_total > inbox_alloc ||
+      pl.boff_total > boff_alloc || spill_cap > spill_alloc || nt > tiles_alloc) {
+    release();
+    // grow with headroom so that re-planning rarely reallocates
+    stay_alloc = pl.stay_total + pl.stay_total / 8 + 64;
+    inbox_alloc = pl.inbox_total + pl.inbox_total / 8 + 64;
+    boff_alloc = pl.boff_total;
+    spill_alloc = spill_cap;
+    tiles_alloc = nt;
+    hipError_t e = hipSuccess;
+    auto ok = [&](hipError_t r) {
+      if (e == hipSuccess) e = r;
+    };
+    ok(dmalloc(&sbase, nt));
+    ok(dmalloc(&scap, nt));
+    ok(dmalloc(&ibase, nt));
+    ok(dmalloc(&icap, nt));
+    ok(dmalloc(&tboff, nt));
+    for (int p = 0; p < 2; ++p) {
+      ok(dmalloc(&cnt[p], nt));
+      ok(dmalloc(&boff[p], boff_alloc));
+      ok(stay[p].alloc(stay_alloc));
+      ok(inbox[p].alloc(inbox_alloc));
+      ok(spill[p].alloc(spill_alloc));
+      ok(dmalloc(&spill_rep[p], spill_alloc));
+    }
+    for (int k = 0; k < 3; ++k) ok(dmalloc(&icnt[k], nt));
+    ok(dmalloc(&scnt, 3));
+    ok(dmalloc(&flags, 1));
+    ok(dmalloc(&done, 1));
+    ok(dmalloc(&gstep, 1));
+    if (e != hipSuccess) {
+      release();
+      return set_error(KG_E_NOMEM, "tile-resident layout allocation failed: %s",
+                       hipGetErrorString(e));
+    }
+    allocated = true;
+  }
+  plan = pl;
+  ntiles = (uint32_t)nt;
+  const size_t bytes = nt * sizeof(uint32_t);
+  KG_HIP(hipMemcpyAsync(sbase, plan.sbase.data(), bytes, hipMemcpyHostToDevice, s));
+  KG_HIP(hipMemcpyAsync(scap, plan.scap.data(), bytes, hipMemcpyHostToDevice, s));
+  KG_HIP(hipMemcpyAsync(ibase, plan.ibase.data(), bytes, hipMemcpyHostToDevice, s));
+  KG_HIP(hipMemcpyAsync(icap, plan.icap.data(), bytes, hipMemcpyHostToDevice, s));
+  KG_HIP(hipMemcpyAsync(tboff, plan.tboff.data(), bytes, hipMemcpyHostToDevice, s));
+  // the plan vectors are members and stay unchanged until the copies ran
+  KG_HIP(hipStreamSynchronize(s));
+  return KG_OK;
+}
+
+int RtState::reset(uint64_t g, hipStream_t s) {
+  for (int k = 0; k < 3; ++k) KG_HIP(launch_fill_u32(icnt[k], 0u, ntiles, s));
+  KG_HIP(launch_fill_u32(scnt, 0u, 3, s));
+  KG_HIP(launch_fill_u32(flags, 0u, 1, s));
+  KG_HIP(launch_fill_u32(done, 0u, 1, s));
+  KG_HIP(launch_set_u64(gstep, g, s));
+  G = g;
+  return KG_OK;
+}
+
+}  // namespace kg

@@ -1,0 +1,59 @@
+// kg_rt_state.hpp -- device memory of the tile-resident layout (kg_rt.hpp)
+// and the host mirror of its step counter, shared by the plain field and
+// the ensemble engine.
+#pragma once
+
+#include "kg_common.hpp"
+
+namespace kg {
+
+struct DevBuf {
+  Buf b{};
+  hipError_t alloc(size_t cap) {
+    hipError_t e;
+    if ((e = dmalloc(&b.x, cap)) != hipSuccess) return e;
+    if ((e = dmalloc(&b.y, cap)) != hipSuccess) return e;
+    if ((e = dmalloc(&b.hx, cap)) != hipSuccess) return e;
+    if ((e = dmalloc(&b.hy, cap)) != hipSuccess) return e;
+    return dmalloc(&b.id, cap);
+  }
+  void release() {
+    (void)hipFree(b.x);
+    (void)hipFree(b.y);
+    (void)hipFree(b.hx);
+    (void)hipFree(b.hy);
+    (void)hipFree(b.id);
+    b = Buf{};
+  }
+};
+
+struct RtState {
+  bool allocated = false;
+  RtPlan plan;
+  RtCaps caps{};
+  uint32_t ntiles = 0;
+  uint64_t stay_alloc = 0, inbox_alloc = 0, boff_alloc = 0, spill_alloc = 0, tiles_alloc = 0;
+  uint32_t *sbase = nullptr, *scap = nullptr, *ibase = nullptr, *icap = nullptr, *tboff = nullptr;
+  uint32_t *cnt[2] = {nullptr, nullptr}, *boff[2] = {nullptr, nullptr};
+  uint32_t* icnt[3] = {nullptr, nullptr, nullptr};
+  uint32_t* scnt = nullptr;  // 3 counters
+  DevBuf stay[2], inbox[2], spill[2];
+  uint32_t* spill_rep[2] = {nullptr, nullptr};
+  uint32_t* flags = nullptr;
+  uint32_t* done = nullptr;
+  uint64_t* gstep = nullptr;
+  uint64_t G = 0;  // value of *gstep once queued work has run
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;  // (steps, exec)
+
+  RtDev dev() const;
+  // (Re)allocates for `plan` (growing only) and uploads the plan tables.
+  int setup(const RtPlan& plan, uint64_t spill_cap, hipStream_t s);
+  // Zeroes the counters and flags and sets the step counter to G.
+  int reset(uint64_t G, hipStream_t s);
+  void drop_graphs();
+  void release();
+};
+
+hipError_t launch_set_u64(uint64_t* p, uint64_t v, hipStream_t s);
+
+}  // namespace kg

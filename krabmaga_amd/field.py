@@ -156,10 +156,14 @@ class Field2D:
         p = params.c()
         check(self._L.kg_flock_step(self._h, ctypes.byref(p), step))
 
-    def schedule_run(self, params, first_step, nsteps, use_graph=True):
+    def schedule_run(self, params, first_step, nsteps, use_graph=True, binned=False):
+        """`nsteps` Schedule::step calls from `first_step` (simulate!'s loop).
+
+        binned=True forces the three-kernel bin-grouped path instead of the
+        one-launch tile-resident step (same results)."""
         p = params.c()
-        check(self._L.kg_schedule_run(self._h, ctypes.byref(p), first_step, nsteps,
-                                      int(bool(use_graph))))
+        flags = (1 if use_graph else 0) | (2 if binned else 0)
+        check(self._L.kg_schedule_run(self._h, ctypes.byref(p), first_step, nsteps, flags))
 
     def read_agents(self, buffer=0):
         n = ctypes.c_size_t()

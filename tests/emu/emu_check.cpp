@@ -22,7 +22,28 @@ int emu_set(emu_field* f, const uint32_t* ids, const float* x, const float* y, c
 int emu_lazy_update(emu_field* f);
 int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step);
 uint32_t emu_read(emu_field* f, uint32_t* ids, float* x, float* y, float* hx, float* hy);
+emu_field* emu_create_rt(float w, float h, float d, int tor, uint32_t cap, uint32_t tx,
+                         uint32_t ty, uint32_t hb);
+int emu_rt_enter(emu_field* f, int caps_runs, int caps_own, int caps_extras, int tight);
+int emu_rt_steps(emu_field* f, const kg_flock_params* p, uint64_t step0, uint32_t nsteps);
+uint32_t emu_rt_flags(emu_field* f);
+uint32_t emu_rt_read(emu_field* f, uint32_t* ids, float* x, float* y, float* hx, float* hy);
 }
+
+struct RtScenario {
+  const char* name;
+  uint32_t n;
+  float side, d;
+  uint32_t steps;
+  uint64_t seed;
+  uint32_t tx, ty, hb;  // 0 = automatic
+  int runs, own, extras; // LDS capacities, 0 = planned
+  int tight;            // capacities below the counts
+  uint64_t shuffle;
+  int layout;
+  float radius;
+  float jump;
+};
 
 struct Scenario {
   const char* name;
@@ -123,6 +144,61 @@ static int run(const Scenario& s) {
   return (rc != 0 || bad != 0) ? 1 : 0;
 }
 
+static int compare(const char* name, uint32_t n, const std::vector<uint32_t>& gid,
+                   const std::vector<float>& gx, const std::vector<float>& gy,
+                   const std::vector<float>& ghx, const std::vector<float>& ghy, uint32_t m,
+                   or_flocker* fl, const char* extra) {
+  std::vector<float> rx(n), ry(n), rhx(n), rhy(n);
+  or_flocker_get(fl, rx.data(), ry.data(), rhx.data(), rhy.data());
+  uint32_t bad = 0;
+  std::vector<int> seen(n, 0);
+  for (uint32_t k = 0; k < m; ++k) {
+    const uint32_t i = gid[k];
+    if (i >= n || seen[i]++) {
+      ++bad;
+      continue;
+    }
+    if (std::memcmp(&gx[k], &rx[i], 4) || std::memcmp(&gy[k], &ry[i], 4) ||
+        std::memcmp(&ghx[k], &rhx[i], 4) || std::memcmp(&ghy[k], &rhy[i], 4))
+      ++bad;
+  }
+  if (m != n) bad += n;
+  std::printf("%-28s agents=%u %s mismatches=%u\n", name, m, extra, bad);
+  return bad ? 1 : 0;
+}
+
+static int run_rt(const RtScenario& s) {
+  Scenario base{s.name, s.n, s.side, s.d, 1, s.steps, s.seed, 0, 0, 0, 0, s.layout, s.radius};
+  std::vector<float> x, y, hx, hy;
+  make_positions(base, x, y, hx, hy);
+  or_flock_params op{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, s.jump, s.radius, s.seed};
+  or_flocker* fl = or_flocker_new(s.n, s.side, s.side, s.d, 1, x.data(), y.data(), hx.data(),
+                                  hy.data(), &op);
+  emu_field* e = emu_create_rt(s.side, s.side, s.d, 1, s.n, s.tx, s.ty, s.hb);
+  if (!e) {
+    std::printf("%-28s emu_create_rt failed\n", s.name);
+    return 1;
+  }
+  emu_options(e, 0, s.shuffle);
+  std::vector<uint32_t> ids(s.n);
+  for (uint32_t i = 0; i < s.n; ++i) ids[i] = i;
+  emu_set(e, ids.data(), x.data(), y.data(), hx.data(), hy.data(), s.n);
+  int rc = emu_lazy_update(e);  // schedule.rs:349-351
+  if (rc == 0) rc = emu_rt_enter(e, s.runs, s.own, s.extras, s.tight);
+  kg_flock_params kp{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, s.jump, s.radius, 0u, s.seed};
+  if (rc == 0) rc = emu_rt_steps(e, &kp, 0, s.steps);
+  for (uint32_t k = 0; k < s.steps; ++k) or_flocker_step(fl);
+  std::vector<uint32_t> gid(s.n);
+  std::vector<float> gx(s.n), gy(s.n), ghx(s.n), ghy(s.n);
+  const uint32_t m = emu_rt_read(e, gid.data(), gx.data(), gy.data(), ghx.data(), ghy.data());
+  char extra[96];
+  std::snprintf(extra, sizeof(extra), "rc=%d flags=0x%x", rc, emu_rt_flags(e));
+  const int bad = compare(s.name, s.n, gid, gx, gy, ghx, ghy, m, fl, extra);
+  emu_destroy(e);
+  or_flocker_free(fl);
+  return (rc != 0 || bad) ? 1 : 0;
+}
+
 int main() {
   const float D = 10.0f / 1.5f;
   const Scenario sc[] = {
@@ -148,6 +224,22 @@ int main() {
   };
   int fails = 0;
   for (const Scenario& s : sc) fails += run(s);
+  const RtScenario rs[] = {
+      {"rt-flockers-200", 1000, 200.0f, D, 60, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 10.0f, 0.7f},
+      {"rt-flockers-200-shuffled", 1000, 200.0f, D, 30, 0, 0, 0, 0, 0, 0, 0, 0, 77, 0, 10.0f, 0.7f},
+      {"rt-small-tiles", 1500, 200.0f, D, 30, 1, 4, 8, 2, 0, 0, 0, 0, 0, 0, 10.0f, 0.7f},
+      {"rt-sparse", 10000, 2048.0f, D, 10, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 10.0f, 0.7f},
+      {"rt-dense", 3000, 60.0f, D, 10, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 10.0f, 0.7f},
+      {"rt-clustered", 6000, 700.0f, D, 10, 4, 0, 0, 0, 0, 0, 0, 0, 5, 1, 10.0f, 0.7f},
+      {"rt-clustered-global", 6000, 700.0f, D, 6, 4, 0, 0, 0, 96, 64, 8, 0, 0, 1, 10.0f, 0.7f},
+      {"rt-tight-capacity", 4000, 400.0f, D, 12, 6, 0, 0, 0, 0, 0, 0, 1, 3, 0, 10.0f, 0.7f},
+      {"rt-padding-edges", 3000, 200.0f, D, 12, 5, 0, 0, 0, 0, 0, 0, 0, 0, 2, 10.0f, 0.7f},
+      {"rt-padding-reachable", 4000, 1024.0f, 1.1596828699111938f, 10, 10, 0, 0, 0, 0, 0, 0, 0,
+       0, 2, 2.0f, 0.7f},
+      {"rt-radius-5x5", 2500, 160.0f, 4.0f, 8, 6, 0, 0, 0, 0, 0, 0, 0, 0, 0, 10.0f, 0.7f},
+      {"rt-big-jumps", 2000, 300.0f, D, 10, 8, 0, 0, 0, 0, 0, 0, 0, 0, 0, 10.0f, 9.0f},
+  };
+  for (const RtScenario& s : rs) fails += run_rt(s);
   std::printf("%d scenario(s) failed\n", fails);
   return fails ? 1 : 0;
 }

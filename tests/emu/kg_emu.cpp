@@ -17,6 +17,7 @@
 
 #include "kg_geom.hpp"
 #include "kg_phases.hpp"
+#include "kg_rt.hpp"
 
 using namespace kg;
 
@@ -37,7 +38,45 @@ struct HostBuf {
 
 }  // namespace
 
+struct EmuRt {
+  bool active = false;
+  RtPlan plan;
+  RtCaps caps{};
+  HostBuf stay[2], inbox[2], spill[2];
+  std::vector<uint32_t> spill_rep[2];
+  std::vector<uint32_t> cnt[2], boff[2], icnt[3];
+  uint32_t scnt[3] = {0, 0, 0};
+  uint32_t flags = 0, done = 0;
+  uint64_t gstep = 0;
+  RtDev dev() {
+    RtDev D;
+    D.sbase = plan.sbase.data();
+    D.scap = plan.scap.data();
+    D.ibase = plan.ibase.data();
+    D.icap = plan.icap.data();
+    D.tboff = plan.tboff.data();
+    for (int p = 0; p < 2; ++p) {
+      D.cnt[p] = cnt[p].data();
+      D.boff[p] = boff[p].data();
+      D.stay[p] = stay[p].view();
+      D.inbox[p] = inbox[p].view();
+      D.spill[p] = spill[p].view();
+      D.spill_rep[p] = spill_rep[p].data();
+    }
+    for (int k = 0; k < 3; ++k) {
+      D.icnt[k] = icnt[k].data();
+      D.scnt[k] = &scnt[k];
+    }
+    D.spill_cap = (uint32_t)spill[0].x.size();
+    D.flags = &flags;
+    D.done = &done;
+    D.gstep = &gstep;
+    return D;
+  }
+};
+
 struct emu_field {
+  EmuRt rt;
   Geom g{};
   RepDev rep{};
   uint32_t cap = 0;
@@ -54,11 +93,30 @@ struct emu_field {
 
 extern "C" {
 
+emu_field* emu_reset_geom(emu_field* f, float w, float h, float d, int tor, uint32_t cap,
+                          uint32_t bin_rows, int lds_cap, int keep_geom, int);
+
 emu_field* emu_create(float w, float h, float d, int tor, uint32_t cap, uint32_t bin_rows,
                       int lds_cap) {
+  return emu_reset_geom(new emu_field(), w, h, d, tor, cap, bin_rows, lds_cap, 0, 0);
+}
+
+// Recreates the field with the tile-resident geometry (tx, ty, hb; 0 = auto).
+emu_field* emu_create_rt(float w, float h, float d, int tor, uint32_t cap, uint32_t tx,
+                         uint32_t ty, uint32_t hb) {
   emu_field* f = new emu_field();
   std::string why;
-  if (!make_geom(w, h, d, tor, cap, bin_rows, &f->g, &why)) {
+  if (!make_geom_rt(w, h, d, tor, cap, tx, ty, hb, &f->g, &why)) {
+    delete f;
+    return nullptr;
+  }
+  return emu_reset_geom(f, w, h, d, tor, cap, 0, 0, 1, 0);
+}
+
+emu_field* emu_reset_geom(emu_field* f, float w, float h, float d, int tor, uint32_t cap,
+                          uint32_t bin_rows, int lds_cap, int keep_geom, int) {
+  std::string why;
+  if (!keep_geom && !make_geom(w, h, d, tor, cap, bin_rows, &f->g, &why)) {
     delete f;
     return nullptr;
   }
@@ -79,6 +137,240 @@ emu_field* emu_create(float w, float h, float d, int tor, uint32_t cap, uint32_t
 }
 
 void emu_destroy(emu_field* f) { delete f; }
+
+// ---- tile-resident path ---------------------------------------------------
+
+static std::vector<uint32_t> order(uint32_t lo, uint32_t hi, uint64_t seed);
+
+static void emu_rt_block(emu_field* f, uint32_t item, int load, uint64_t load_g,
+                         std::vector<uint8_t>& smem) {
+  EmuRt& E = f->rt;
+  RtDev D = E.dev();
+  const RtCaps caps = E.caps;
+  const uint32_t prefix[2] = {0u, f->g.ntiles};
+  const uint64_t g = load ? load_g : E.gstep;
+  const RtTile rt = rt_tile(&f->rep, 1, prefix, item, g);
+  const RepDev& rp = f->rep;
+  std::fill(smem.begin(), smem.end(), 0x5A);
+  const RtSmem S = rt_carve(smem.data(), caps);
+  const int nb_tile = rp.g.tx * rp.g.tr;
+  const int pout = rt.pin ^ 1;
+  std::vector<float> nx(kRtMaxOwn), ny(kRtMaxOwn), nhx(kRtMaxOwn), nhy(kRtMaxOwn);
+  std::vector<uint32_t> nid(kRtMaxOwn);
+  uint32_t seg_start[3 * kMaxStageCols], seg_len[3 * kMaxStageCols];
+  uint32_t col_base[kMaxStageCols + 1], ecol_base[kMaxStageCols + 1];
+  uint32_t src_base[10], src_pre[11];
+  uint32_t ne = 0, nown_c = 0, nstay = 0, fail = 0, nown = 0;
+  bool global_path = false;
+  Buf Ra = f->R.view();
+  if (!load) {
+    const bool lds_ok = rp.lds_ok != 0 && rt.ncols <= kMaxStageCols && nb_tile <= caps.nb;
+    uint32_t total = 0;
+    for (int t = 0; t < rt.ncols && lds_ok; ++t) {
+      col_base[t] = total;
+      total += rt_segments(rp, rt, D, t, seg_start, seg_len);
+    }
+    col_base[rt.ncols] = total;
+    src_pre[0] = 0;
+    for (int k = 0; k < 10; ++k) src_pre[k + 1] = src_pre[k] + rt_extra_count(rp, rt, D, k, &src_base[k]);
+    global_path = !lds_ok || total > (uint32_t)caps.runs;
+    if (!global_path) {
+      for (uint32_t j = 0; j < src_pre[10]; ++j)
+        if (!rt_gather_extra(rp, rt, D, j, src_base, src_pre, caps.extras, &ne, S)) fail = 1;
+      for (uint32_t k = 0; k < total; ++k)
+        rt_stage(rp, rt, D, k, seg_start, seg_len, col_base, S);
+      global_path = fail != 0;
+      if (!global_path) {
+        for (uint32_t j = 0; j < ne; ++j) rt_rank_extra(j, ne, S);
+        for (int c = 0; c <= rt.ncols; ++c) ecol_base[c] = rt_extra_col_base(c, ne, S);
+        for (uint32_t k = 0; k < total; ++k)
+          if (rt_own(rt, rt.sc_lo + S.scol[k], S.scy[k])) {
+            const uint32_t o = nown_c++;
+            if (o < (uint32_t)caps.own) S.own[o] = (uint16_t)k;
+          }
+        for (uint32_t j = 0; j < ne; ++j)
+          if (rt_own(rt, rt.sc_lo + S.ecol[j], S.ecy[j])) {
+            const uint32_t o = nown_c++;
+            if (o < (uint32_t)caps.own) S.own[o] = (uint16_t)(0x8000u | j);
+          }
+        nown = nown_c;
+        global_path = nown > (uint32_t)caps.own;
+        if (!global_path)
+          for (uint32_t o = 0; o < nown; ++o)
+            rt_agent_lds(rp, rt, S.own[o], col_base, ecol_base, S, &nx[o], &ny[o], &nhx[o],
+                         &nhy[o], &nid[o]);
+      }
+    }
+  } else {
+    const uint32_t tile = rt.gt - rt.tbase;
+    const uint32_t b0 = rp.bin_base + (tile << (rp.g.tx_log2 + rp.g.tr_log2));
+    const uint32_t lo = f->offset[b0], hi = f->offset[b0 + (uint32_t)nb_tile];
+    nown = hi - lo;
+    global_path = nown > (uint32_t)caps.own || nb_tile > caps.nb;
+    if (global_path) {
+      for (uint32_t i = lo; i < hi; ++i) {
+        int32_t cx, cy;
+        discretize(rp.g, Ra.x[i], Ra.y[i], &cx, &cy);
+        rt_push(rp, rt, D, rt.tbase + tile_of(rp.g, cx, cy), Ra.x[i], Ra.y[i], Ra.hx[i], Ra.hy[i],
+                Ra.id[i]);
+      }
+    } else {
+      for (uint32_t o = 0; o < nown; ++o) {
+        nx[o] = Ra.x[lo + o];
+        ny[o] = Ra.y[lo + o];
+        nhx[o] = Ra.hx[lo + o];
+        nhy[o] = Ra.hy[lo + o];
+        nid[o] = Ra.id[lo + o];
+      }
+    }
+  }
+  std::fill(smem.begin(), smem.end(), 0x3C);  // phase B overwrites phase A
+  if (global_path) {
+    if (!load) {
+      *D.flags |= kRtFlagGlobal;
+      for (uint32_t o = 0;; ++o) {
+        float x, y, hx, hy;
+        uint32_t id;
+        bool skip;
+        if (!rt_own_global(rp, rt, D, o, &x, &y, &hx, &hy, &id, &skip)) break;
+        if (skip) continue;
+        float ax, ay, ahx, ahy;
+        rt_agent_global(rp, rt, D, x, y, hx, hy, id, &ax, &ay, &ahx, &ahy);
+        int32_t cx, cy;
+        discretize(rp.g, ax, ay, &cx, &cy);
+        if (!cell_valid(rp.g, cx, cy)) {
+          *D.flags |= kRtFlagRange;
+          continue;
+        }
+        rt_push(rp, rt, D, rt.tbase + tile_of(rp.g, cx, cy), ax, ay, ahx, ahy, id);
+      }
+    }
+    for (int b = 0; b <= nb_tile; ++b) D.boff[pout][D.tboff[rt.gt] + b] = 0u;
+    D.cnt[pout][rt.gt] = 0u;
+  } else {
+    for (uint32_t o = 0; o < nown; ++o)
+      rt_classify(rp, rt, D, false, nx[o], ny[o], nhx[o], nhy[o], nid[o], &nstay, S);
+    if (nstay > D.scap[rt.gt]) {
+      *D.flags |= kRtFlagStayCap;
+      for (uint32_t s2 = 0; s2 < nstay; ++s2)
+        rt_push(rp, rt, D, rt.gt, S.ox[s2], S.oy[s2], S.ohx[s2], S.ohy[s2], S.oid[s2]);
+      for (int b = 0; b <= nb_tile; ++b) D.boff[pout][D.tboff[rt.gt] + b] = 0u;
+      D.cnt[pout][rt.gt] = 0u;
+    } else {
+      for (int b = 0; b <= nb_tile; ++b) S.hist[b] = 0u;
+      for (uint32_t s2 = 0; s2 < nstay; ++s2) S.hist[S.obin[s2]]++;
+      uint32_t run = 0;
+      for (int b = 0; b < nb_tile; ++b) {
+        S.bbase[b] = run;
+        S.bcur[b] = run;
+        run += S.hist[b];
+      }
+      S.bbase[nb_tile] = nstay;
+      for (uint32_t s2 = 0; s2 < nstay; ++s2) S.operm[S.bcur[S.obin[s2]]++] = (uint16_t)s2;
+      for (uint32_t s2 = 0; s2 < nstay; ++s2) rt_write_stayer(rt, D, s2, S);
+      for (int b = 0; b <= nb_tile; ++b) D.boff[pout][D.tboff[rt.gt] + b] = S.bbase[b];
+      D.cnt[pout][rt.gt] = nstay;
+    }
+  }
+  if (!load) {
+    D.icnt[rt.ph_reset][rt.gt] = 0u;
+    if (rt.gt == 0) *D.scnt[rt.ph_reset] = 0u;
+  }
+}
+
+// Enters the tile-resident layout from the current read buffer.
+int emu_rt_enter(emu_field* f, int caps_runs, int caps_own, int caps_extras, int tight) {
+  if (f->w_aligned || f->nA) return -6;
+  EmuRt& E = f->rt;
+  const uint32_t nt = f->g.ntiles;
+  const uint32_t nbt = (uint32_t)(f->g.tx * f->g.tr);
+  std::vector<uint32_t> counts(nt), nb(nt, nbt);
+  for (uint32_t t = 0; t < nt; ++t) counts[t] = f->offset[(t + 1) * nbt] - f->offset[t * nbt];
+  rt_plan(counts, nb, &E.plan);
+  E.caps = rt_caps(E.plan, f->g, f->rep.disc_dist, 60000);
+  if (tight) {
+    // capacities far below the counts: exercises stay-capacity overflow,
+    // inbox overflow and the spill list
+    uint64_t s2 = 0, ib = 0;
+    for (uint32_t t = 0; t < nt; ++t) {
+      E.plan.scap[t] = std::max<uint32_t>(4u, counts[t] / 2u);
+      E.plan.icap[t] = 4u;
+      E.plan.sbase[t] = (uint32_t)s2;
+      E.plan.ibase[t] = (uint32_t)ib;
+      s2 += E.plan.scap[t];
+      ib += E.plan.icap[t];
+    }
+    E.plan.stay_total = s2;
+    E.plan.inbox_total = ib;
+  }
+  if (caps_runs > 0) E.caps.runs = caps_runs;
+  if (caps_own > 0) E.caps.own = caps_own;
+  if (caps_extras > 0) E.caps.extras = caps_extras;
+  for (int p = 0; p < 2; ++p) {
+    E.stay[p].resize(E.plan.stay_total);
+    E.inbox[p].resize(E.plan.inbox_total);
+    E.spill[p].resize(f->cap);
+    E.spill_rep[p].assign(f->cap, 0u);
+    E.cnt[p].assign(nt, 0u);
+    E.boff[p].assign(E.plan.boff_total, 0u);
+  }
+  for (int k = 0; k < 3; ++k) {
+    E.icnt[k].assign(nt, 0u);
+    E.scnt[k] = 0;
+  }
+  E.flags = 0;
+  E.done = 0;
+  E.gstep = 1;
+  std::vector<uint8_t> smem(rt_smem_bytes(E.caps));
+  for (uint32_t item : order(0, nt, f->shuffle_seed)) emu_rt_block(f, item, 1, E.gstep - 1, smem);
+  E.active = true;
+  return 0;
+}
+
+int emu_rt_steps(emu_field* f, const kg_flock_params* p, uint64_t step0, uint32_t nsteps) {
+  EmuRt& E = f->rt;
+  if (!E.active) return -6;
+  set_rep_params(&f->rep, *p);
+  f->rep.step_base = step0 - E.gstep;
+  std::vector<uint8_t> smem(rt_smem_bytes(E.caps));
+  for (uint32_t s2 = 0; s2 < nsteps; ++s2) {
+    for (uint32_t item : order(0, f->g.ntiles, f->shuffle_seed ? f->shuffle_seed + s2 : 0))
+      emu_rt_block(f, item, 0, 0, smem);
+    E.gstep += 1;
+  }
+  return (E.flags & kRtFlagRange) ? -3 : 0;
+}
+
+uint32_t emu_rt_flags(emu_field* f) { return f->rt.flags; }
+
+// All agents of the tile-resident layout (any order).
+uint32_t emu_rt_read(emu_field* f, uint32_t* ids, float* x, float* y, float* hx, float* hy) {
+  EmuRt& E = f->rt;
+  RtDev D = E.dev();
+  const uint32_t prefix[2] = {0u, f->g.ntiles};
+  uint32_t n = 0;
+  auto put = [&](const Buf& B, uint32_t gi) {
+    ids[n] = B.id[gi];
+    x[n] = B.x[gi];
+    y[n] = B.y[gi];
+    hx[n] = B.hx[gi];
+    hy[n] = B.hy[gi];
+    ++n;
+  };
+  for (uint32_t item = 0; item < f->g.ntiles; ++item) {
+    const RtTile rt = rt_tile(&f->rep, 1, prefix, item, E.gstep);
+    for (uint32_t o = 0; o < D.cnt[rt.pin][rt.gt]; ++o) put(D.stay[rt.pin], D.sbase[rt.gt] + o);
+    uint32_t ib;
+    const uint32_t ni = rt_extra_count(f->rep, rt, D, 4, &ib);
+    for (uint32_t o = 0; o < ni; ++o) put(D.inbox[rt.pin], ib + o);
+    if (item == 0) {
+      uint32_t sb;
+      const uint32_t nsp = rt_extra_count(f->rep, rt, D, 9, &sb);
+      for (uint32_t o = 0; o < nsp; ++o) put(D.spill[rt.pin], o);
+    }
+  }
+  return n;
+}
 
 // geom: max_x, max_y, dw, dh, tx, tr, hb, ty, ntx, nty, ntiles, nbins, lds_cap
 void emu_geom(emu_field* f, int64_t* out) {

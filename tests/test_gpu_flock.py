@@ -121,6 +121,42 @@ def test_multi_step_bit_exact(oracle, n, side, steps, seed):
     assert np.array_equal(bits(got), bits(fl.state()))
 
 
+@pytest.mark.parametrize("n,side,seed", [(1000, 200.0, 0), (60000, 4000.0, 3), (20000, 300.0, 5)])
+def test_tile_resident_and_binned_paths_agree(oracle, n, side, seed):
+    """The one-launch tile-resident step equals the three-kernel path and the oracle."""
+    from krabmaga_amd import FlockParams
+    p = FlockParams(seed=seed)
+    a = gpu_field(n, side, D, seed=seed)
+    b = gpu_field(n, side, D, seed=seed)
+    a.schedule_run(p, 0, 12)
+    b.schedule_run(p, 0, 12, binned=True)
+    sa, ia = a.state_by_id(0)
+    sb, ib = b.state_by_id(0)
+    assert np.array_equal(ia, np.arange(n)) and np.array_equal(ib, np.arange(n))
+    assert np.array_equal(bits(sa), bits(sb))
+    if n <= 20000:
+        fl = oracle_flocker(oracle, n, side, D, seed)
+        fl.step(12)
+        assert np.array_equal(bits(sa), bits(fl.state()))
+
+
+def test_run_split_across_calls_and_layouts(oracle):
+    """Runs resumed across calls, queries in between (layout round trips)."""
+    from krabmaga_amd import FlockParams
+    n, side, seed = 3000, 300.0, 8
+    p = FlockParams(seed=seed)
+    f = gpu_field(n, side, D, seed=seed)
+    f.schedule_run(p, 0, 5)
+    assert len(f.get_neighbors_within_distance(150.0, 150.0, 10.0)) >= 0  # back to bins
+    f.schedule_run(p, 5, 9, use_graph=False)
+    f.schedule_run(p, 14, 3, binned=True)
+    f.schedule_run(p, 17, 20)
+    s, _ = f.state_by_id(0)
+    fl = oracle_flocker(oracle, n, side, D, seed)
+    fl.step(37)
+    assert np.array_equal(bits(s), bits(fl.state()))
+
+
 def test_graph_and_direct_paths_agree():
     from krabmaga_amd import FlockParams
     p = FlockParams(seed=9, cohesion=1.3, jump=0.5)
