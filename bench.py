@@ -5,11 +5,12 @@ Workload (the plain GPU drop-in): 1,048,576 boids on a 20,480 x 20,480
 toroidal Field2D, discretization 10/1.5, neighbour radius 10, all weights 1,
 jump 0.7, positions uniform and headings unit vectors at a uniform angle
 from the seeded Philox stream.  One step = one Schedule::step of the
-flockers model = the fused query + Bird::step kernel and the rebinning
-(lazy_update) of all agents.  State is resident in HBM before timing starts.
+flockers model: neighbour query + Bird::step for every agent and the
+rebinning of the field (lazy_update).  State is resident in HBM before the
+timed region starts.
 
-Prints one JSON line (see the keys below).  `python bench.py --gpus N` with
-N > 1 exits: this build measures one GPU.
+Prints one JSON line.  `python bench.py --gpus N` with N > 1 exits: this
+build measures one GPU.
 """
 import argparse
 import ctypes
@@ -23,10 +24,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "agent-updates/s (1M flockers f32) & ensemble p50/p99 latency, batched vs plain"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-# algorithmic bytes per agent and launch (f32 x, y, hx, hy + u32 id)
-BYTES_FLOCK = 20 + 16    # read sorted state, write new x, y, hx, hy
-BYTES_SCATTER = 20 + 20  # read new state + id, write binned state + id
-BYTES_STEP = BYTES_FLOCK + BYTES_SCATTER
+# Algorithmic bytes per agent and launch (f32 x, y, hx, hy + u32 id = 20 B):
+BYTES_RT = 20 + 20       # tile-resident step: read the agent, write it back
+BYTES_FLOCK = 20 + 16    # three-kernel path, fused query + step
+BYTES_SCATTER = 20 + 20  # three-kernel path, rebinning
+BYTES_STEP = 76          # per agent-update of a whole step (binning 40 + query/step 36)
 
 
 class Hip:
@@ -39,7 +41,6 @@ class Hip:
         self.L.hipEventSynchronize.argtypes = [ctypes.c_void_p]
         self.L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
                                                ctypes.c_void_p]
-        self.L.hipEventDestroy.argtypes = [ctypes.c_void_p]
 
     def _ok(self, rc, what):
         if rc != 0:
@@ -63,8 +64,24 @@ class Hip:
         return ms.value
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def plain_workload():
     return dict(n=1 << 20, side=20480.0, d=10.0 / 1.5, radius=10.0, seed=1)
+
+
+def timed_run(hip, f, p, first, steps, binned):
+    e0, e1 = hip.event(), hip.event()
+    hip.sync()
+    t0 = time.perf_counter()
+    hip.record(e0, f.stream)
+    f.schedule_run(p, first, steps, binned=binned)
+    hip.record(e1, f.stream)
+    ms = hip.elapsed_ms(e0, e1)
+    hip.sync()
+    return ms, time.perf_counter() - t0
 
 
 def run_plain(args, hip):
@@ -76,34 +93,35 @@ def run_plain(args, hip):
     f.init_uniform(n, w["seed"], 1)
     f.schedule_run(p, 0, args.warmup)  # step 0 includes the first lazy_update
     f.sync()
-    print(f"[bench] plain warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
-
-    # timed region: K schedule steps replayed from a HIP graph
-    e0, e1 = hip.event(), hip.event()
-    hip.sync()
-    t0 = time.perf_counter()
-    hip.record(e0, f.stream)
-    f.schedule_run(p, args.warmup, args.steps)
-    hip.record(e1, f.stream)
-    ms = hip.elapsed_ms(e0, e1)
-    hip.sync()
-    wall = time.perf_counter() - t0
+    log(f"plain warmup done ({args.warmup} steps)")
+    ms, wall = timed_run(hip, f, p, args.warmup, args.steps, binned=False)
     f.sync()
-    print(f"[bench] plain timed region: {ms:.3f} ms for {args.steps} steps", file=sys.stderr,
-          flush=True)
+    log(f"tile-resident timed region: {ms:.3f} ms for {args.steps} steps")
 
-    # per-kernel durations: the same steps launched directly, each kernel
-    # bracketed by HIP events on the field's stream
+    # per-kernel durations: steps launched one by one, each launch bracketed
+    # by HIP events on the field's stream
+    first = args.warmup + args.steps
     kprof = min(args.steps, args.profile_steps)
     f.profile(True)
-    f.schedule_run(p, args.warmup + args.steps, kprof, use_graph=False)
-    kern = {}
+    f.schedule_run(p, first, kprof, use_graph=False)
+    rt_ms, rt_n = f.profile_read("rt_step")
+    f.profile(False)
+    first += kprof
+
+    # the three-kernel path on the same state, for comparison
+    nb = min(args.steps, args.binned_steps)
+    bms, _ = timed_run(hip, f, p, first, nb, binned=True)
+    first += nb
+    f.profile(True)
+    f.schedule_run(p, first, kprof, use_graph=False, binned=True)
+    kern = {"rt_step": {"avg_us": 1e3 * rt_ms / max(rt_n, 1), "launches": rt_n}}
     for k in ("flock", "scan", "scatter"):
         tot, cnt = f.profile_read(k)
         kern[k] = {"avg_us": 1e3 * tot / max(cnt, 1), "launches": cnt}
     f.profile(False)
     f.close()
-    return w, ms, wall, kern
+    log(f"three-kernel path: {bms / nb:.4f} ms/step")
+    return w, ms, wall, kern, bms / nb
 
 
 def cpu_baseline(args):
@@ -130,6 +148,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--binned-steps", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -139,12 +158,11 @@ def main():
         sys.exit(2)
 
     hip = Hip()
-    w, ms, wall, kern = run_plain(args, hip)
+    w, ms, wall, kern, binned_ms = run_plain(args, hip)
     n, steps = w["n"], args.steps
     value = n * steps / (ms * 1e-3)
-    dom = max(("flock", "scatter"), key=lambda k: kern[k]["avg_us"])
-    per_launch = {"flock": BYTES_FLOCK * n, "scatter": BYTES_SCATTER * n}[dom]
-    achieved = per_launch / (kern[dom]["avg_us"] * 1e-6)
+    rt_us = kern["rt_step"]["avg_us"]
+    achieved = BYTES_RT * n / (rt_us * 1e-6)
     out = {
         "metric": METRIC,
         "value": value,
@@ -165,12 +183,13 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
+            "kernel": "k_rt_step",
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": None,
+            "bytes_per_agent": BYTES_RT,
         },
         "step_roofline": {
             "bytes_per_agent_update": BYTES_STEP,
@@ -178,10 +197,14 @@ def main():
             "frac": BYTES_STEP * value / HBM_PEAK,
         },
         "kernels_us": {k: round(v["avg_us"], 3) for k, v in kern.items()},
+        "three_kernel_path": {
+            "ms_per_step": binned_ms,
+            "agent_updates_per_s": n / (binned_ms * 1e-3),
+        },
         "wall_s_timed_region": wall,
     }
     if not args.no_cpu:
-        print("[bench] CPU baseline (oracle, 1 thread) ...", file=sys.stderr, flush=True)
+        log("CPU baseline (oracle, 1 thread) ...")
         out["cpu_baseline"] = cpu_baseline(args)
     print(json.dumps(out))
 

@@ -33,7 +33,8 @@ EXPORTS = [
     "kg_field_profile", "kg_field_profile_read", "kg_last_error",
 ]
 
-KERNELS = {"flock": 0, "scan": 1, "scatter": 2, "count_append": 3, "scatter_append": 4}
+KERNELS = {"flock": 0, "scan": 1, "scatter": 2, "count_append": 3, "scatter_append": 4,
+           "rt_step": 5, "rt_load": 6}
 
 
 class KgError(RuntimeError):
