@@ -67,6 +67,7 @@ struct kg_field {
   RtState rt;
   bool rt_mode = false;     // the read buffer lives in the tile-resident layout
   bool rt_replan = false;   // its capacities were exceeded: re-plan before the next run
+  bool rt_unsuitable = false;  // the last plan did not fit the LDS budget
   size_t lds_limit = 65536;
 
   // HIP-event timing of individual kernel launches (kg_field_profile)
@@ -267,8 +268,15 @@ int rt_enter(kg_field* f) {
   for (uint32_t t = 0; t < nt; ++t) counts[t] = off[(size_t)(t + 1) * nbt] - off[(size_t)t * nbt];
   RtPlan plan;
   rt_plan(counts, nb, &plan);
+  bool fits = false;
+  const RtCaps caps = rt_caps(plan, f->g, f->h_rep.disc_dist, f->lds_limit, &fits);
+  if (!fits) {
+    // tiles too crowded for the LDS working set: stay bin-grouped
+    f->rt_unsuitable = true;
+    return KG_OK;
+  }
   if (int rc = f->rt.setup(plan, f->cap, f->stream)) return rc;
-  f->rt.caps = rt_caps(plan, f->g, f->h_rep.disc_dist, f->lds_limit);
+  f->rt.caps = caps;
   if (int rc = f->rt.reset(1, f->stream)) return rc;
   {
     ProfScope ps(f, KG_KERNEL_RT_LOAD);
@@ -713,18 +721,36 @@ int kg_schedule_run(kg_field* f, const kg_flock_params* p, uint64_t first_step, 
     RepDev probe = f->h_rep;
     set_rep_params(&probe, *p);
     if (!(use_graph & KG_RUN_BINNED) && probe.lds_ok) {
-      // tile-resident path: one launch per schedule step
+      // tile-resident path: one launch per schedule step; the capacity
+      // flags are read every kRtCheckEvery steps and the layout re-planned
+      // when a tile overflowed
+      constexpr uint32_t kRtCheckEvery = 128;
       if (f->rt_mode)
         if (int rc = rt_check_flags(f)) return rc;
-      if (f->rt_mode && (f->rt_replan || probe.disc_dist != f->h_rep.disc_dist)) {
+      if (f->rt_mode && (f->rt_replan || probe.disc_dist != f->h_rep.disc_dist))
         if (int rc = ensure_atomic(f)) return rc;
-      }
       if (int rc = set_params(f, p, 0)) return rc;
-      if (!f->rt_mode)
+      if (!f->rt_mode) {
+        f->rt_unsuitable = false;
         if (int rc = rt_enter(f)) return rc;
-      // schedule step of the next launch = step_base + *gstep
-      if (int rc = set_params(f, p, first_step - f->rt.G)) return rc;
-      return rt_steps(f, nsteps, use_graph & KG_RUN_GRAPH);
+      }
+      while (f->rt_mode && nsteps > 0) {
+        const uint32_t chunk = std::min(nsteps, kRtCheckEvery);
+        // schedule step of the next launch = step_base + *gstep
+        if (int rc = set_params(f, p, first_step - f->rt.G)) return rc;
+        if (int rc = rt_steps(f, chunk, use_graph & KG_RUN_GRAPH)) return rc;
+        first_step += chunk;
+        nsteps -= chunk;
+        if (nsteps == 0) break;
+        if (int rc = rt_check_flags(f)) return rc;
+        if (f->rt_replan) {
+          if (int rc = ensure_atomic(f)) return rc;
+          if (int rc = set_params(f, p, 0)) return rc;
+          if (int rc = rt_enter(f)) return rc;
+        }
+      }
+      if (nsteps == 0) return KG_OK;
+      // the plan did not fit: the remaining steps run bin-grouped
     }
   }
   if (int rc = ensure_atomic(f)) return rc;

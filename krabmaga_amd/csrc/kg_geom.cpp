@@ -149,10 +149,13 @@ void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& n
   plan->max_scap = plan->max_icap = plan->max_nb = 0;
   for (size_t t = 0; t < nt; ++t) {
     const uint32_t c = counts[t];
-    // room for fluctuation: a quarter more than now, at least 16
-    uint32_t sc = c + std::max<uint32_t>(16u, c / 4u);
+    // A tile's population wanders: after a few crossing times it is a fresh
+    // Poisson draw around the mean, and flocking adds clumping.  Half the
+    // count again (at least 32) keeps overflow rare; the inbox holds the
+    // arrivals of a step plus overflow.
+    uint32_t sc = c + std::max<uint32_t>(32u, c / 2u);
     sc = (sc + 15u) & ~15u;
-    uint32_t ic = std::max<uint32_t>(16u, c / 8u + 8u);
+    uint32_t ic = std::max<uint32_t>(32u, c / 4u + 16u);
     ic = (ic + 7u) & ~7u;
     plan->sbase[t] = (uint32_t)s;
     plan->scap[t] = sc;
@@ -171,20 +174,26 @@ void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& n
   plan->boff_total = bo;
 }
 
-RtCaps rt_caps(const RtPlan& plan, const Geom& g, int32_t dd, size_t lds_limit) {
+RtCaps rt_caps(const RtPlan& plan, const Geom& g, int32_t dd, size_t lds_limit, bool* fits) {
   RtCaps c;
   // staged stayers per own stayer: the tile plus dd halo columns each side
   // and one whole bin (hb rows) above and below
   const double halo = std::min(9.0, (double)(g.tx + 2 * dd) * (double)(g.ty + 2 * g.hb) /
                                         ((double)g.tx * (double)g.ty));
   c.nb = (int)std::max<uint32_t>(plan.max_nb, 1u);
-  c.own = (int)std::min<uint32_t>((uint32_t)kRtMaxOwn, (plan.max_scap + plan.max_icap + 63u) & ~63u);
-  c.runs = (int)std::min<double>(16384.0, std::ceil(plan.max_scap * halo * 1.25 / 64.0) * 64.0);
-  c.extras = (int)std::min<uint32_t>(1024u, std::max<uint32_t>(128u, (3u * plan.max_icap + 63u) & ~63u));
+  // own agents of a tile: its stayers plus a share of its inbox
+  const uint32_t want_own = (plan.max_scap + plan.max_icap / 2u + 63u) & ~63u;
+  const double want_runs = std::ceil(plan.max_scap * halo * 1.1 / 64.0) * 64.0;
+  c.own = (int)std::min<uint32_t>((uint32_t)kRtMaxOwn, want_own);
+  c.runs = (int)std::min<double>(16384.0, want_runs);
+  c.extras = (int)std::min<uint32_t>(1024u, std::max<uint32_t>(128u, (plan.max_icap + 63u) & ~63u));
   while (rt_smem_bytes(c) > lds_limit && (c.runs > 128 || c.own > 64)) {
     if (c.runs > 128) c.runs -= 64;
     if (rt_smem_bytes(c) > lds_limit && c.own > 64) c.own -= 64;
   }
+  if (fits)
+    *fits = (uint32_t)c.own >= want_own && (double)c.runs >= want_runs &&
+            rt_smem_bytes(c) <= lds_limit && c.nb <= kRtMaxBins;
   return c;
 }
 

@@ -39,7 +39,10 @@ void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& n
 
 // LDS capacities of the tile-resident kernel for a plan, the tile geometry
 // and the window reach dd, limited to `lds_limit` bytes.
-RtCaps rt_caps(const RtPlan& plan, const Geom& g, int32_t dd, size_t lds_limit);
+// *fits tells whether the plan's working set fits (else the tile-resident
+// path would fall back to its slow global mode on most tiles).
+RtCaps rt_caps(const RtPlan& plan, const Geom& g, int32_t dd, size_t lds_limit,
+               bool* fits = nullptr);
 
 // Geometry for the tile-resident layout: tiles of about `own_target` agents.
 bool make_geom_rt(float w, float h, float d, int tor, uint64_t expected_agents, uint32_t tx,

@@ -666,13 +666,8 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_step(
     }
     __syncthreads();
     const uint32_t nstay = s_nstay;
-    if (nstay > D.scap[rt.gt]) {
-      if (t == 0) atomicOr(D.flags, kRtFlagStayCap);
-      for (uint32_t s = t; s < nstay; s += kRtThreads)
-        rt_push(rp, rt, D, rt.gt, S.ox[s], S.oy[s], S.ohx[s], S.ohy[s], S.oid[s]);
-      for (int b = t; b <= nb_tile; b += kRtThreads) D.boff[pout][D.tboff[rt.gt] + b] = 0u;
-      if (t == 0) D.cnt[pout][rt.gt] = 0u;
-    } else {
+    const uint32_t scap = D.scap[rt.gt];
+    {
       for (int b = t; b <= nb_tile; b += kRtThreads) S.hist[b] = 0u;
       __syncthreads();
       for (uint32_t s = t; s < nstay; s += kRtThreads) atomicAdd(&S.hist[S.obin[s]], 1u);
@@ -703,10 +698,10 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_step(
         S.operm[pos] = (uint16_t)s;
       }
       __syncthreads();
-      for (uint32_t s = t; s < nstay; s += kRtThreads) rt_write_stayer(rt, D, s, S);
+      for (uint32_t s = t; s < nstay; s += kRtThreads) rt_write_stayer(rp, rt, D, s, S);
       for (int b = t; b <= nb_tile; b += kRtThreads)
-        D.boff[pout][D.tboff[rt.gt] + b] = S.bbase[b];
-      if (t == 0) D.cnt[pout][rt.gt] = nstay;
+        D.boff[pout][D.tboff[rt.gt] + b] = S.bbase[b] < scap ? S.bbase[b] : scap;
+      if (t == 0) D.cnt[pout][rt.gt] = nstay < scap ? nstay : scap;
     }
   }
 

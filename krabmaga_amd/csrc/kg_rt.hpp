@@ -455,7 +455,10 @@ KG_HD int64_t rt_classify(const RepDev& rp, const RtTile& rt, const RtDev& D, bo
 }
 
 // Places stayer s inside its bin by (cell y, id, slot) and writes it out.
-KG_HD void rt_write_stayer(const RtTile& rt, const RtDev& D, uint32_t s, RtSmem S) {
+// Stayers sorted past the tile's capacity go to its own inbox instead (the
+// tile keeps the first scap of its sorted stayers; boff is clipped).
+KG_HD void rt_write_stayer(const RepDev& rp, const RtTile& rt, const RtDev& D, uint32_t s,
+                           RtSmem S) {
   const uint32_t b = S.obin[s];
   const uint32_t lo = S.bbase[b], hi = S.bbase[b + 1];
   const int32_t cy = S.ocy[s];
@@ -467,8 +470,14 @@ KG_HD void rt_write_stayer(const RtTile& rt, const RtDev& D, uint32_t s, RtSmem 
     const uint32_t qid = S.oid[o];
     rank += (qy < cy) || (qy == cy && (qid < id || (qid == id && o < s)));
   }
+  const uint32_t pos = lo + rank;
+  if (pos >= D.scap[rt.gt]) {
+    atom_or(D.flags, kRtFlagStayCap);
+    rt_push(rp, rt, D, rt.gt, S.ox[s], S.oy[s], S.ohx[s], S.ohy[s], id);
+    return;
+  }
   const Buf& B = D.stay[rt.pin ^ 1];
-  const uint32_t gi = D.sbase[rt.gt] + lo + rank;
+  const uint32_t gi = D.sbase[rt.gt] + pos;
   B.x[gi] = S.ox[s]; B.y[gi] = S.oy[s]; B.hx[gi] = S.ohx[s]; B.hy[gi] = S.ohy[s]; B.id[gi] = id;
 }
 

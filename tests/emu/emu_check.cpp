@@ -200,6 +200,7 @@ static int run_rt(const RtScenario& s) {
 }
 
 int main() {
+  std::setvbuf(stdout, nullptr, _IOLBF, 0);
   const float D = 10.0f / 1.5f;
   const Scenario sc[] = {
       {"flockers-200", 1000, 200.0f, D, 1, 40, 0, 0, 0, 0, 0, 0, 10.0f},
@@ -229,10 +230,10 @@ int main() {
       {"rt-flockers-200-shuffled", 1000, 200.0f, D, 30, 0, 0, 0, 0, 0, 0, 0, 0, 77, 0, 10.0f, 0.7f},
       {"rt-small-tiles", 1500, 200.0f, D, 30, 1, 4, 8, 2, 0, 0, 0, 0, 0, 0, 10.0f, 0.7f},
       {"rt-sparse", 10000, 2048.0f, D, 10, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 10.0f, 0.7f},
-      {"rt-dense", 3000, 60.0f, D, 10, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 10.0f, 0.7f},
+      {"rt-dense", 1200, 60.0f, D, 3, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 10.0f, 0.7f},
       {"rt-clustered", 6000, 700.0f, D, 10, 4, 0, 0, 0, 0, 0, 0, 0, 5, 1, 10.0f, 0.7f},
-      {"rt-clustered-global", 6000, 700.0f, D, 6, 4, 0, 0, 0, 96, 64, 8, 0, 0, 1, 10.0f, 0.7f},
-      {"rt-tight-capacity", 4000, 400.0f, D, 12, 6, 0, 0, 0, 0, 0, 0, 1, 3, 0, 10.0f, 0.7f},
+      {"rt-clustered-global", 2000, 400.0f, D, 4, 4, 0, 0, 0, 96, 64, 8, 0, 0, 1, 10.0f, 0.7f},
+      {"rt-tight-capacity", 2000, 300.0f, D, 8, 6, 0, 0, 0, 0, 0, 0, 1, 3, 0, 10.0f, 0.7f},
       {"rt-padding-edges", 3000, 200.0f, D, 12, 5, 0, 0, 0, 0, 0, 0, 0, 0, 2, 10.0f, 0.7f},
       {"rt-padding-reachable", 4000, 1024.0f, 1.1596828699111938f, 10, 10, 0, 0, 0, 0, 0, 0, 0,
        0, 2, 2.0f, 0.7f},

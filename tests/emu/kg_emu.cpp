@@ -250,13 +250,8 @@ static void emu_rt_block(emu_field* f, uint32_t item, int load, uint64_t load_g,
   } else {
     for (uint32_t o = 0; o < nown; ++o)
       rt_classify(rp, rt, D, false, nx[o], ny[o], nhx[o], nhy[o], nid[o], &nstay, S);
-    if (nstay > D.scap[rt.gt]) {
-      *D.flags |= kRtFlagStayCap;
-      for (uint32_t s2 = 0; s2 < nstay; ++s2)
-        rt_push(rp, rt, D, rt.gt, S.ox[s2], S.oy[s2], S.ohx[s2], S.ohy[s2], S.oid[s2]);
-      for (int b = 0; b <= nb_tile; ++b) D.boff[pout][D.tboff[rt.gt] + b] = 0u;
-      D.cnt[pout][rt.gt] = 0u;
-    } else {
+    const uint32_t scap = D.scap[rt.gt];
+    {
       for (int b = 0; b <= nb_tile; ++b) S.hist[b] = 0u;
       for (uint32_t s2 = 0; s2 < nstay; ++s2) S.hist[S.obin[s2]]++;
       uint32_t run = 0;
@@ -267,9 +262,10 @@ static void emu_rt_block(emu_field* f, uint32_t item, int load, uint64_t load_g,
       }
       S.bbase[nb_tile] = nstay;
       for (uint32_t s2 = 0; s2 < nstay; ++s2) S.operm[S.bcur[S.obin[s2]]++] = (uint16_t)s2;
-      for (uint32_t s2 = 0; s2 < nstay; ++s2) rt_write_stayer(rt, D, s2, S);
-      for (int b = 0; b <= nb_tile; ++b) D.boff[pout][D.tboff[rt.gt] + b] = S.bbase[b];
-      D.cnt[pout][rt.gt] = nstay;
+      for (uint32_t s2 = 0; s2 < nstay; ++s2) rt_write_stayer(rp, rt, D, s2, S);
+      for (int b = 0; b <= nb_tile; ++b)
+        D.boff[pout][D.tboff[rt.gt] + b] = S.bbase[b] < scap ? S.bbase[b] : scap;
+      D.cnt[pout][rt.gt] = nstay < scap ? nstay : scap;
     }
   }
   if (!load) {
