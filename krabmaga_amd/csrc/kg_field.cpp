@@ -252,7 +252,9 @@ int rt_check_flags(kg_field* f) {
   uint32_t fl = 0;
   KG_HIP(hipMemcpy(&fl, f->rt.flags, sizeof(fl), hipMemcpyDeviceToHost));
   if (fl & kRtFlagRange) f->poisoned = true;
-  if (fl & (kRtFlagSpill | kRtFlagGlobal | kRtFlagStayCap)) f->rt_replan = true;
+  // stay overflow is absorbed by the inboxes; spilling or falling back to the
+  // global path costs every tile, so those ask for a new plan
+  if (fl & (kRtFlagSpill | kRtFlagGlobal)) f->rt_replan = true;
   return check_poison(f);
 }
 
@@ -267,7 +269,11 @@ int rt_enter(kg_field* f) {
   std::vector<uint32_t> counts(nt), nb(nt, nbt);
   for (uint32_t t = 0; t < nt; ++t) counts[t] = off[(size_t)(t + 1) * nbt] - off[(size_t)t * nbt];
   RtPlan plan;
-  rt_plan(counts, nb, &plan);
+  std::vector<double> expect;
+  uint64_t total = 0;
+  for (uint32_t c : counts) total += c;
+  rt_expected_counts(f->g, total, &expect);
+  rt_plan(counts, nb, expect, &plan);
   bool fits = false;
   const RtCaps caps = rt_caps(plan, f->g, f->h_rep.disc_dist, f->lds_limit, &fits);
   if (!fits) {
@@ -724,7 +730,7 @@ int kg_schedule_run(kg_field* f, const kg_flock_params* p, uint64_t first_step, 
       // tile-resident path: one launch per schedule step; the capacity
       // flags are read every kRtCheckEvery steps and the layout re-planned
       // when a tile overflowed
-      constexpr uint32_t kRtCheckEvery = 128;
+      constexpr uint32_t kRtCheckEvery = 64;
       if (f->rt_mode)
         if (int rc = rt_check_flags(f)) return rc;
       if (f->rt_mode && (f->rt_replan || probe.disc_dist != f->h_rep.disc_dist))

@@ -138,7 +138,8 @@ void set_rep_params(RepDev* r, const kg_flock_params& p) {
 
 namespace kg {
 
-void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& nb, RtPlan* plan) {
+void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& nb,
+             const std::vector<double>& expect, RtPlan* plan) {
   const size_t nt = counts.size();
   plan->sbase.resize(nt);
   plan->scap.resize(nt);
@@ -149,13 +150,14 @@ void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& n
   plan->max_scap = plan->max_icap = plan->max_nb = 0;
   for (size_t t = 0; t < nt; ++t) {
     const uint32_t c = counts[t];
-    // A tile's population wanders: after a few crossing times it is a fresh
-    // Poisson draw around the mean, and flocking adds clumping.  Half the
-    // count again (at least 32) keeps overflow rare; the inbox holds the
-    // arrivals of a step plus overflow.
-    uint32_t sc = c + std::max<uint32_t>(32u, c / 2u);
+    // Flocks build large-scale density waves: after a few hundred steps the
+    // fullest tile holds 2.5-4.5 times the mean of uniform tiles.  A tile
+    // gets half its count again, and at least 2.5 times its share of the
+    // agents by area; what still overflows goes to its inbox.
+    const uint32_t by_area = (uint32_t)std::min(4.0e9, std::ceil(2.5 * expect[t]));
+    uint32_t sc = std::max<uint32_t>(c + std::max<uint32_t>(32u, c / 2u), by_area + 32u);
     sc = (sc + 15u) & ~15u;
-    uint32_t ic = std::max<uint32_t>(32u, c / 4u + 16u);
+    uint32_t ic = std::max<uint32_t>(32u, sc / 4u);
     ic = (ic + 7u) & ~7u;
     plan->sbase[t] = (uint32_t)s;
     plan->scap[t] = sc;
@@ -174,6 +176,17 @@ void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& n
   plan->boff_total = bo;
 }
 
+void rt_expected_counts(const Geom& g, uint64_t agents, std::vector<double>* expect) {
+  expect->resize(g.ntiles);
+  const double per_cell = (double)agents / ((double)g.dw * (double)g.dh);
+  for (uint32_t t = 0; t < g.ntiles; ++t) {
+    const int32_t tcx = (int32_t)(t % (uint32_t)g.ntx), tcy = (int32_t)(t / (uint32_t)g.ntx);
+    const int32_t cols = std::min(g.tx, g.dw - tcx * g.tx);
+    const int32_t rows = std::min(g.ty, g.dh - tcy * g.ty);
+    (*expect)[t] = per_cell * (double)cols * (double)rows;
+  }
+}
+
 RtCaps rt_caps(const RtPlan& plan, const Geom& g, int32_t dd, size_t lds_limit, bool* fits) {
   RtCaps c;
   // staged stayers per own stayer: the tile plus dd halo columns each side
@@ -186,7 +199,7 @@ RtCaps rt_caps(const RtPlan& plan, const Geom& g, int32_t dd, size_t lds_limit, 
   const double want_runs = std::ceil(plan.max_scap * halo * 1.1 / 64.0) * 64.0;
   c.own = (int)std::min<uint32_t>((uint32_t)kRtMaxOwn, want_own);
   c.runs = (int)std::min<double>(16384.0, want_runs);
-  c.extras = (int)std::min<uint32_t>(1024u, std::max<uint32_t>(128u, (plan.max_icap + 63u) & ~63u));
+  c.extras = (int)std::min<uint32_t>(2048u, std::max<uint32_t>(128u, (2u * plan.max_icap + 63u) & ~63u));
   while (rt_smem_bytes(c) > lds_limit && (c.runs > 128 || c.own > 64)) {
     if (c.runs > 128) c.runs -= 64;
     if (rt_smem_bytes(c) > lds_limit && c.own > 64) c.own -= 64;

@@ -29,13 +29,18 @@ int pick_lds_cap(const Geom& g, uint64_t expected_agents);
 bool params_valid(const kg_flock_params& p, std::string* why);
 
 // Tile-resident layout plan: per-tile stayer and inbox capacities and bases
-// from per-tile agent counts; nb[t] is tx * tr of tile t's replica.
+// from per-tile agent counts and the counts expected from the tiles' areas;
+// nb[t] is tx * tr of tile t's replica.
 struct RtPlan {
   std::vector<uint32_t> sbase, scap, ibase, icap, tboff;
   uint64_t stay_total = 0, inbox_total = 0, boff_total = 0;
   uint32_t max_scap = 0, max_icap = 0, max_nb = 0;
 };
-void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& nb, RtPlan* plan);
+void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& nb,
+             const std::vector<double>& expect, RtPlan* plan);
+
+// Agents each tile of `g` would hold if `agents` were spread uniformly.
+void rt_expected_counts(const Geom& g, uint64_t agents, std::vector<double>* expect);
 
 // LDS capacities of the tile-resident kernel for a plan, the tile geometry
 // and the window reach dd, limited to `lds_limit` bytes.

@@ -282,7 +282,11 @@ int emu_rt_enter(emu_field* f, int caps_runs, int caps_own, int caps_extras, int
   const uint32_t nbt = (uint32_t)(f->g.tx * f->g.tr);
   std::vector<uint32_t> counts(nt), nb(nt, nbt);
   for (uint32_t t = 0; t < nt; ++t) counts[t] = f->offset[(t + 1) * nbt] - f->offset[t * nbt];
-  rt_plan(counts, nb, &E.plan);
+  std::vector<double> expect;
+  uint64_t total = 0;
+  for (uint32_t c : counts) total += c;
+  rt_expected_counts(f->g, total, &expect);
+  rt_plan(counts, nb, expect, &E.plan);
   E.caps = rt_caps(E.plan, f->g, f->rep.disc_dist, 60000);
   if (tight) {
     // capacities far below the counts: exercises stay-capacity overflow,
