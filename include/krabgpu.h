@@ -179,6 +179,17 @@ void *kg_field_stream(kg_field *f);
 int kg_field_profile(kg_field *f, int enable);
 int kg_field_profile_read(kg_field *f, int kernel, double *total_ms, uint64_t *launches);
 
+/* Layout statistics (no counterpart in the reference), 16 values:
+ * [0] tile-resident layout active, [1] last plan did not fit the LDS
+ * budget, [2] plans made, [3] OR of the tile-resident flags seen
+ * (1 spill, 2 global path, 4 out of grid, 8 stay overflow), [4] tiles,
+ * [5] tile columns, [6] tile rows, [7] bin rows, [8] LDS staged-agent
+ * capacity, [9] LDS own-agent capacity, [10] LDS extras capacity, [11] LDS
+ * bytes per work-group, [12] LDS limit per work-group, [13] largest tile
+ * capacity, [14] largest inbox capacity, [15] schedule steps run
+ * tile-resident. */
+int kg_field_stats(kg_field *f, uint64_t out[16]);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char *kg_last_error(void);
 

@@ -30,7 +30,7 @@ EXPORTS = [
     "kg_get_neighbors_within_relax_distance", "kg_get_objects", "kg_num_objects_at_location",
     "kg_num_empty_bags", "kg_nagents", "kg_count", "kg_neighbor_sets", "kg_flock_step",
     "kg_schedule_run", "kg_read_agents", "kg_field_sync", "kg_field_stream",
-    "kg_field_profile", "kg_field_profile_read", "kg_last_error",
+    "kg_field_profile", "kg_field_profile_read", "kg_field_stats", "kg_last_error",
 ]
 
 KERNELS = {"flock": 0, "scan": 1, "scatter": 2, "count_append": 3, "scatter_append": 4,
@@ -102,6 +102,7 @@ def lib():
     L.kg_field_profile.argtypes = [P, I]
     L.kg_field_profile_read.argtypes = [P, I, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_uint64)]
+    L.kg_field_stats.argtypes = [P, ctypes.POINTER(ctypes.c_uint64)]
     L.kg_last_error.argtypes = []
     L.kg_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:

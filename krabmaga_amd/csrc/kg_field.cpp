@@ -68,6 +68,7 @@ struct kg_field {
   bool rt_mode = false;     // the read buffer lives in the tile-resident layout
   bool rt_replan = false;   // its capacities were exceeded: re-plan before the next run
   bool rt_unsuitable = false;  // the last plan did not fit the LDS budget
+  uint64_t rt_plans = 0, rt_flags_seen = 0, rt_steps_run = 0;
   size_t lds_limit = 65536;
 
   // HIP-event timing of individual kernel launches (kg_field_profile)
@@ -251,6 +252,7 @@ int rt_check_flags(kg_field* f) {
   KG_HIP(hipStreamSynchronize(f->stream));
   uint32_t fl = 0;
   KG_HIP(hipMemcpy(&fl, f->rt.flags, sizeof(fl), hipMemcpyDeviceToHost));
+  f->rt_flags_seen |= fl;
   if (fl & kRtFlagRange) f->poisoned = true;
   // stay overflow is absorbed by the inboxes; spilling or falling back to the
   // global path costs every tile, so those ask for a new plan
@@ -276,6 +278,7 @@ int rt_enter(kg_field* f) {
   rt_plan(counts, nb, expect, &plan);
   bool fits = false;
   const RtCaps caps = rt_caps(plan, f->g, f->h_rep.disc_dist, f->lds_limit, &fits);
+  ++f->rt_plans;
   if (!fits) {
     // tiles too crowded for the LDS working set: stay bin-grouped
     f->rt_unsuitable = true;
@@ -347,6 +350,7 @@ int rt_steps(kg_field* f, uint32_t nsteps, int use_graph) {
     --left;
   }
   f->rt.G += nsteps;
+  f->rt_steps_run += nsteps;
   return KG_OK;
 }
 
@@ -856,6 +860,34 @@ int kg_field_profile_read(kg_field* f, int kernel, double* total_ms, uint64_t* l
   if (int rc = prof_collect(f)) return rc;
   *total_ms = f->prof_ms[kernel];
   *launches = f->prof_n[kernel];
+  return KG_OK;
+}
+
+int kg_field_stats(kg_field* f, uint64_t out[16]) {
+  clear_error();
+  if (!f || !out) return set_error(KG_E_INVALID, "NULL argument");
+  if (int rc = sync_only(f)) return rc;
+  uint32_t fl = 0;
+  if (f->rt.allocated) KG_HIP(hipMemcpy(&fl, f->rt.flags, sizeof(fl), hipMemcpyDeviceToHost));
+  f->rt_flags_seen |= fl;
+  const RtCaps& c = f->rt.caps;
+  const uint64_t v[16] = {f->rt_mode,
+                          f->rt_unsuitable,
+                          f->rt_plans,
+                          f->rt_flags_seen,
+                          f->g.ntiles,
+                          (uint64_t)f->g.tx,
+                          (uint64_t)f->g.ty,
+                          (uint64_t)f->g.hb,
+                          (uint64_t)c.runs,
+                          (uint64_t)c.own,
+                          (uint64_t)c.extras,
+                          f->rt.allocated ? rt_lds_bytes(c) : 0,
+                          f->lds_limit,
+                          f->rt.plan.max_scap,
+                          f->rt.plan.max_icap,
+                          f->rt_steps_run};
+  std::memcpy(out, v, sizeof(v));
   return KG_OK;
 }
 

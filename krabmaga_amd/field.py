@@ -198,6 +198,16 @@ class Field2D:
                                             ctypes.byref(n)))
         return ms.value, n.value
 
+    STAT_NAMES = ("rt_active", "rt_unsuitable", "rt_plans", "rt_flags", "tiles", "tile_cols",
+                  "tile_rows", "bin_rows", "lds_runs", "lds_own", "lds_extras", "lds_bytes",
+                  "lds_limit", "max_tile_cap", "max_inbox_cap", "rt_steps")
+
+    def stats(self):
+        """Layout statistics (see kg_field_stats)."""
+        out = (ctypes.c_uint64 * 16)()
+        check(self._L.kg_field_stats(self._h, out))
+        return dict(zip(self.STAT_NAMES, (int(v) for v in out)))
+
     def sync(self):
         check(self._L.kg_field_sync(self._h))
 
