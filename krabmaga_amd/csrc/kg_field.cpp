@@ -404,9 +404,7 @@ int kg_field_create(const kg_field_desc* desc, kg_field** out) {
   if ((e = dmalloc(&f->offset, nb + 1)) != hipSuccess) return fail(e);
   if ((e = dmalloc(&f->offset2, nb + 1)) != hipSuccess) return fail(e);
   if ((e = dmalloc(&f->cursor, nb)) != hipSuccess) return fail(e);
-  if ((e = dmalloc(&f->ss.status, nscan)) != hipSuccess) return fail(e);
-  if ((e = dmalloc(&f->ss.tile_ctr, 1)) != hipSuccess) return fail(e);
-  if ((e = dmalloc(&f->ss.epoch, 1)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->ss.partial, nscan + 1)) != hipSuccess) return fail(e);
   if ((e = dmalloc(&f->err, 1)) != hipSuccess) return fail(e);
   if ((e = dmalloc(&f->gstep, 1)) != hipSuccess) return fail(e);
   if ((e = dmalloc(&f->zero64, 1)) != hipSuccess) return fail(e);
@@ -416,8 +414,7 @@ int kg_field_create(const kg_field_desc* desc, kg_field** out) {
   // one-time clears (never inside a captured graph)
   const std::pair<void*, size_t> clears[] = {
       {f->count, sizeof(uint32_t) * nb},          {f->offset, sizeof(uint32_t) * (nb + 1)},
-      {f->offset2, sizeof(uint32_t) * (nb + 1)},  {f->ss.status, sizeof(uint64_t) * nscan},
-      {f->ss.tile_ctr, sizeof(uint32_t)},         {f->ss.epoch, sizeof(uint32_t)},
+      {f->offset2, sizeof(uint32_t) * (nb + 1)},  {f->ss.partial, sizeof(uint32_t) * (nscan + 1)},
       {f->err, sizeof(uint32_t)},                 {f->gstep, sizeof(uint64_t)},
       {f->zero64, sizeof(uint64_t)}};
   for (const auto& c : clears)
@@ -449,9 +446,7 @@ void kg_field_destroy(kg_field* f) {
   (void)hipFree(f->offset);
   (void)hipFree(f->offset2);
   (void)hipFree(f->cursor);
-  (void)hipFree(f->ss.status);
-  (void)hipFree(f->ss.tile_ctr);
-  (void)hipFree(f->ss.epoch);
+  (void)hipFree(f->ss.partial);
   (void)hipFree(f->err);
   (void)hipFree(f->gstep);
   (void)hipFree(f->zero64);

@@ -121,95 +121,87 @@ hipError_t launch_materialize(Buf R, Buf W, uint32_t n, Buf A, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// scan: single-pass exclusive prefix with decoupled look-back.
-//
-// Status word per scan tile: [epoch:30][flag:2][value:32]; flag 1 = tile
-// aggregate, 2 = inclusive prefix.  The epoch (bumped by the last tile of
-// every launch) makes stale words of earlier launches invalid, so nothing
-// has to be cleared between launches (graph replays included).  Tile indices
-// come from an atomic counter so a tile only waits on tiles already running.
-// The kernel also zeroes the counts (for the next histogram), fills the
-// scatter cursors and advances the schedule-step counter.
+// scan: exclusive prefix over the bin counts in three launches (tile sums,
+// one-block scan of the tile sums, tile scans).  No work-group waits on
+// another, so nothing depends on inter-work-group ordering.  The last launch
+// also zeroes the counts (for the next histogram), fills the scatter cursors
+// and advances the schedule-step counter.
 
-__global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* __restrict__ count,
-                                                       uint32_t* __restrict__ offset,
-                                                       uint32_t* __restrict__ cursor, uint32_t n,
-                                                       uint64_t* __restrict__ status,
-                                                       uint32_t* tile_ctr, uint32_t* epoch_ptr,
-                                                       uint64_t* gstep) {
-  __shared__ uint32_t s_tile, s_epoch, s_prefix;
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ count,
+                                                              uint32_t n,
+                                                              uint32_t* __restrict__ partial) {
   __shared__ uint32_t s_warp[kScanThreads / kWave];
-  const int t = threadIdx.x;
-  if (t == 0) {
-    const uint32_t ep = __hip_atomic_load(epoch_ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s_tile = __hip_atomic_fetch_add(tile_ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_epoch = ep & 0x3FFFFFFFu;
-  }
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) sum += count[base + k];
+  uint32_t total;
+  block_exclusive_scan<kScanThreads>(sum, s_warp, &total);
+  if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// One block: partial[0..m) becomes its exclusive prefix, partial[m] the total.
+__global__ __launch_bounds__(1024) void k_scan_partials(uint32_t* __restrict__ partial, uint32_t m,
+                                                        uint64_t* gstep) {
+  __shared__ uint32_t s_warp[1024 / kWave];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
   __syncthreads();
-  const uint32_t tile = s_tile;
-  const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
-  const uint32_t base = tile * kScanTile + (uint32_t)t * kScanItems;
+  for (uint32_t base = 0; base < m; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < m ? partial[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan<1024>(v, s_warp, &total);
+    const uint32_t carry = s_carry;
+    if (i < m) partial[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry = carry + total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[m] = s_carry;
+    if (gstep) *gstep += 1;
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict__ count,
+                                                            uint32_t* __restrict__ offset,
+                                                            uint32_t* __restrict__ cursor, uint32_t n,
+                                                            const uint32_t* __restrict__ partial) {
+  __shared__ uint32_t s_warp[kScanThreads / kWave];
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t sum = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    const uint32_t idx = base + k;
-    v[k] = idx < n ? count[idx] : 0u;
+    v[k] = base + k < n ? count[base + k] : 0u;
     sum += v[k];
   }
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k)
     if (base + k < n) count[base + k] = 0u;
   uint32_t total;
-  const uint32_t excl = block_exclusive_scan<kScanThreads>(sum, s_warp, &total);
-  if (t == 0) {
-    const uint64_t ep = (uint64_t)s_epoch << 34;
-    uint32_t prefix = 0;
-    if (tile == 0) {
-      __hip_atomic_store(&status[0], ep | (2ull << 32) | total, __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      __hip_atomic_store(&status[tile], ep | (1ull << 32) | total, __ATOMIC_RELEASE,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      int64_t j = (int64_t)tile - 1;
-      for (;;) {
-        const uint64_t st =
-            __hip_atomic_load(&status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t flag = (uint32_t)(st >> 32) & 3u;
-        if ((st & ~0x3FFFFFFFFull) != ep || flag == 0) continue;
-        prefix += (uint32_t)st;
-        if (flag == 2) break;
-        --j;
-      }
-      __hip_atomic_store(&status[tile], ep | (2ull << 32) | (uint32_t)(prefix + total),
-                         __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    s_prefix = prefix;
-  }
-  __syncthreads();
-  uint32_t run = s_prefix + excl;
+  uint32_t run = partial[blockIdx.x] + block_exclusive_scan<kScanThreads>(sum, s_warp, &total);
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    const uint32_t idx = base + k;
-    if (idx < n) {
-      offset[idx] = run;
-      cursor[idx] = run;
+    if (base + k < n) {
+      offset[base + k] = run;
+      cursor[base + k] = run;
     }
     run += v[k];
   }
-  if (tile == ntiles - 1 && t == 0) {
-    offset[n] = s_prefix + total;
-    if (gstep) *gstep += 1;
-    __hip_atomic_store(tile_ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_add(epoch_ptr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) offset[n] = partial[gridDim.x];
 }
 
 hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint32_t nbins,
                        const ScanState& ss, uint64_t* gstep, hipStream_t s) {
   const uint32_t ntiles = (nbins + kScanTile - 1) / kScanTile;
-  hipLaunchKernelGGL(k_scan, dim3(ntiles), dim3(kScanThreads), 0, s, count, offset, cursor, nbins,
-                     ss.status, ss.tile_ctr, ss.epoch, gstep);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kScanThreads), 0, s, count, nbins,
+                     ss.partial);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, ss.partial, ntiles, gstep);
+  hipLaunchKernelGGL(k_scan_down, dim3(ntiles), dim3(kScanThreads), 0, s, count, offset, cursor,
+                     nbins, ss.partial);
   return hipGetLastError();
 }
 
@@ -708,7 +700,9 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_step(
   if (!load && t == 0) {
     D.icnt[rt.ph_reset][rt.gt] = 0u;
     if (rt.gt == 0) *D.scnt[rt.ph_reset] = 0u;
-    const uint32_t prev = __hip_atomic_fetch_add(D.done, 1u, __ATOMIC_ACQ_REL,
+    // relaxed: the count carries no data; the next launch reads gstep
+    // after this kernel has completed
+    const uint32_t prev = __hip_atomic_fetch_add(D.done, 1u, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT);
     if (prev == gridDim.x - 1) {
       *D.gstep = g + 1;

@@ -35,9 +35,7 @@ struct WorkList {
 };
 
 struct ScanState {
-  uint64_t* status;    // one word per scan tile, zeroed once at allocation
-  uint32_t* tile_ctr;  // zeroed once; reset by the last scan tile
-  uint32_t* epoch;     // bumped by the last scan tile
+  uint32_t* partial;  // one sum per scan tile, plus the total
 };
 
 constexpr int kScanThreads = 256;
