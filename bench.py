@@ -96,7 +96,8 @@ def run_plain(args, hip):
     log(f"plain warmup done ({args.warmup} steps)")
     ms, wall = timed_run(hip, f, p, args.warmup, args.steps, binned=False)
     f.sync()
-    log(f"tile-resident timed region: {ms:.3f} ms for {args.steps} steps")
+    layout = f.stats()
+    log(f"tile-resident timed region: {ms:.3f} ms for {args.steps} steps; layout {layout}")
 
     # per-kernel durations: steps launched one by one, each launch bracketed
     # by HIP events on the field's stream
@@ -121,7 +122,7 @@ def run_plain(args, hip):
     f.profile(False)
     f.close()
     log(f"three-kernel path: {bms / nb:.4f} ms/step")
-    return w, ms, wall, kern, bms / nb
+    return w, ms, wall, kern, bms / nb, layout
 
 
 def cpu_baseline(args):
@@ -158,7 +159,7 @@ def main():
         sys.exit(2)
 
     hip = Hip()
-    w, ms, wall, kern, binned_ms = run_plain(args, hip)
+    w, ms, wall, kern, binned_ms, layout = run_plain(args, hip)
     n, steps = w["n"], args.steps
     value = n * steps / (ms * 1e-3)
     # dominant kernel of the timed path: the tile-resident step, or the
@@ -208,6 +209,8 @@ def main():
             "agent_updates_per_s": n / (binned_ms * 1e-3),
         },
         "wall_s_timed_region": wall,
+        "layout": {k: layout[k] for k in ("rt_active", "rt_plans", "rt_flags", "tiles",
+                                          "dense_tiles", "lds_bytes", "rt_steps")},
     }
     if not args.no_cpu:
         log("CPU baseline (oracle, 1 thread) ...")

@@ -185,9 +185,9 @@ int kg_field_profile_read(kg_field *f, int kernel, double *total_ms, uint64_t *l
  * (1 spill, 2 global path, 4 out of grid, 8 stay overflow), [4] tiles,
  * [5] tile columns, [6] tile rows, [7] bin rows, [8] LDS staged-agent
  * capacity, [9] LDS own-agent capacity, [10] LDS extras capacity, [11] LDS
- * bytes per work-group, [12] LDS limit per work-group, [13] largest tile
- * capacity, [14] largest inbox capacity, [15] schedule steps run
- * tile-resident. */
+ * bytes per work-group ([8]-[11] of the normal tile class), [12] LDS limit
+ * per work-group, [13] largest tile capacity, [14] tiles in the dense class,
+ * [15] schedule steps run tile-resident. */
 int kg_field_stats(kg_field *f, uint64_t out[16]);
 
 /* Message of the last failing call on this thread ("" if none). */

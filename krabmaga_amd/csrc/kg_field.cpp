@@ -12,6 +12,7 @@
 // API calls that read or write the buffers move the field back to the
 // bin-grouped layout first.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -50,6 +51,7 @@ struct kg_field {
   uint32_t* prefix = nullptr;  // [0..1] tile prefix, [2..3] append prefix
   RepDev* d_rep = nullptr;
   uint32_t* scratch = nullptr;  // small query outputs
+  uint32_t* tcount = nullptr;   // per-tile agent counts (tile-resident planning)
   RepDev h_rep{};
 
   uint32_t nR = 0, nW = 0, nA = 0;
@@ -256,8 +258,19 @@ int rt_check_flags(kg_field* f) {
   if (fl & kRtFlagRange) f->poisoned = true;
   // stay overflow is absorbed by the inboxes; spilling or falling back to the
   // global path costs every tile, so those ask for a new plan
-  if (fl & (kRtFlagSpill | kRtFlagGlobal)) f->rt_replan = true;
+  if (fl & (kRtFlagSpill | kRtFlagGlobal | kRtFlagDefer)) f->rt_replan = true;
   return check_poison(f);
+}
+
+// LDS budget of a normal-class work-group: four fit one CU's 160 KB.
+// KG_RT_LDS_NORMAL (bytes) overrides it for tuning.
+size_t rt_lds_normal() {
+  static const size_t v = [] {
+    const char* e = std::getenv("KG_RT_LDS_NORMAL");
+    const long n = e ? std::atol(e) : 0;
+    return n >= 4096 && n <= 160 * 1024 ? (size_t)n : (size_t)40 * 1024;
+  }();
+  return v;
 }
 
 // Moves the bin-grouped read buffer into the tile-resident layout, with
@@ -265,32 +278,34 @@ int rt_check_flags(kg_field* f) {
 int rt_enter(kg_field* f) {
   const uint32_t nt = f->g.ntiles;
   const uint32_t nbt = (uint32_t)(f->g.tx * f->g.tr);
-  std::vector<uint32_t> off((size_t)f->g.nbins + 1);
-  KG_HIP(hipStreamSynchronize(f->stream));
-  KG_HIP(hipMemcpy(off.data(), f->offset, off.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (!f->tcount) KG_HIP(dmalloc(&f->tcount, std::max<uint32_t>(nt, 1u)));
+  KG_HIP(launch_tile_counts(f->offset, nbt, nt, f->tcount, f->stream));
   std::vector<uint32_t> counts(nt), nb(nt, nbt);
-  for (uint32_t t = 0; t < nt; ++t) counts[t] = off[(size_t)(t + 1) * nbt] - off[(size_t)t * nbt];
+  KG_HIP(hipMemcpyAsync(counts.data(), f->tcount, nt * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                        f->stream));
+  KG_HIP(hipStreamSynchronize(f->stream));
   RtPlan plan;
   std::vector<double> expect;
   uint64_t total = 0;
   for (uint32_t c : counts) total += c;
   rt_expected_counts(f->g, total, &expect);
   rt_plan(counts, nb, expect, &plan);
-  bool fits = false;
-  const RtCaps caps = rt_caps(plan, f->g, f->h_rep.disc_dist, f->lds_limit, &fits);
+  RtClassPlan cls;
+  rt_classes(&plan, counts, f->g, f->h_rep.disc_dist, rt_lds_normal(), f->lds_limit, &cls);
   ++f->rt_plans;
-  if (!fits) {
+  if (!cls.fits) {
     // tiles too crowded for the LDS working set: stay bin-grouped
     f->rt_unsuitable = true;
     return KG_OK;
   }
-  if (int rc = f->rt.setup(plan, f->cap, f->stream)) return rc;
-  f->rt.caps = caps;
+  if (int rc = f->rt.setup(plan, cls, f->cap, f->stream)) return rc;
   if (int rc = f->rt.reset(1, f->stream)) return rc;
   {
     ProfScope ps(f, KG_KERNEL_RT_LOAD);
-    KG_HIP(launch_rt_step(f->worklist(), f->rt.dev(), f->rt.caps, 1, f->R.b, f->offset,
-                          f->rt.G - 1, f->stream));
+    const RtDev D = f->rt.dev();
+    for (int c = 0; c < 2; ++c)
+      KG_HIP(launch_rt_load(f->worklist(), D, f->rt.caps[c], c, f->rt.dense, f->rt.ndense,
+                            f->R.b, f->offset, f->rt.G - 1, f->stream));
   }
   f->rt_mode = true;
   f->rt_replan = false;
@@ -313,8 +328,20 @@ int ensure_atomic(kg_field* f) {
   return lazy_update_impl(f, nullptr);
 }
 
-// `nsteps` tile-resident schedule steps, one launch each (graph replays of
-// 8 launches when use_graph is set and profiling is off).
+// One tile-resident schedule step: the normal-class launch, then the
+// dense-class launch (the plan's dense tiles and any deferred normal tiles).
+hipError_t rt_launch_step(kg_field* f, const WorkList& wl, const RtDev& D, uint32_t koff) {
+  hipError_t e = launch_rt_step(wl, D, f->rt.caps[0], koff, 0, nullptr, 0, f->stream);
+  // the dense launch also runs the normal tiles deferred by the first
+  if (e == hipSuccess)
+    e = launch_rt_step(wl, D, f->rt.caps[1], koff, 1, f->rt.dense, f->rt.ndense, f->stream);
+  return e;
+}
+
+// `nsteps` tile-resident schedule steps (graph replays of 8 steps when
+// use_graph is set and profiling is off).  Step k of a batch runs schedule
+// step step_base + *gstep + k; one 1-thread kernel then advances *gstep by
+// the batch size, so no work-group waits on another.
 int rt_steps(kg_field* f, uint32_t nsteps, int use_graph) {
   const RtDev D = f->rt.dev();
   const WorkList wl = f->worklist();
@@ -328,8 +355,8 @@ int rt_steps(kg_field* f, uint32_t nsteps, int use_graph) {
       hipGraph_t graph = nullptr;
       KG_HIP(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
       hipError_t e = hipSuccess;
-      for (int k = 0; k < kChunk && e == hipSuccess; ++k)
-        e = launch_rt_step(wl, D, f->rt.caps, 0, f->R.b, f->offset, 0, f->stream);
+      for (int k = 0; k < kChunk && e == hipSuccess; ++k) e = rt_launch_step(f, wl, D, (uint32_t)k);
+      if (e == hipSuccess) e = launch_add_u64(D.gstep, kChunk, f->stream);
       const hipError_t e2 = hipStreamEndCapture(f->stream, &graph);
       if (e != hipSuccess || e2 != hipSuccess)
         return set_error(KG_E_HIP, "graph capture: %s",
@@ -344,10 +371,12 @@ int rt_steps(kg_field* f, uint32_t nsteps, int use_graph) {
       left -= kChunk;
     }
   }
-  while (left > 0) {
-    ProfScope ps(f, KG_KERNEL_RT_STEP);
-    KG_HIP(launch_rt_step(wl, D, f->rt.caps, 0, f->R.b, f->offset, 0, f->stream));
-    --left;
+  if (left > 0) {
+    for (uint32_t k = 0; k < left; ++k) {
+      ProfScope ps(f, KG_KERNEL_RT_STEP);
+      KG_HIP(rt_launch_step(f, wl, D, k));
+    }
+    KG_HIP(launch_add_u64(D.gstep, left, f->stream));
   }
   f->rt.G += nsteps;
   f->rt_steps_run += nsteps;
@@ -453,6 +482,7 @@ void kg_field_destroy(kg_field* f) {
   (void)hipFree(f->prefix);
   (void)hipFree(f->d_rep);
   (void)hipFree(f->scratch);
+  (void)hipFree(f->tcount);
   if (f->stream) (void)hipStreamDestroy(f->stream);
   delete f;
 }
@@ -865,7 +895,7 @@ int kg_field_stats(kg_field* f, uint64_t out[16]) {
   uint32_t fl = 0;
   if (f->rt.allocated) KG_HIP(hipMemcpy(&fl, f->rt.flags, sizeof(fl), hipMemcpyDeviceToHost));
   f->rt_flags_seen |= fl;
-  const RtCaps& c = f->rt.caps;
+  const RtCaps& c = f->rt.caps[0];
   const uint64_t v[16] = {f->rt_mode,
                           f->rt_unsuitable,
                           f->rt_plans,
@@ -880,7 +910,7 @@ int kg_field_stats(kg_field* f, uint64_t out[16]) {
                           f->rt.allocated ? rt_lds_bytes(c) : 0,
                           f->lds_limit,
                           f->rt.plan.max_scap,
-                          f->rt.plan.max_icap,
+                          f->rt.ndense,
                           f->rt_steps_run};
   std::memcpy(out, v, sizeof(v));
   return KG_OK;

@@ -146,6 +146,7 @@ void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& n
   plan->ibase.resize(nt);
   plan->icap.resize(nt);
   plan->tboff.resize(nt);
+  plan->meta.resize(nt);
   uint64_t s = 0, ib = 0, bo = 0;
   plan->max_scap = plan->max_icap = plan->max_nb = 0;
   for (size_t t = 0; t < nt; ++t) {
@@ -164,6 +165,7 @@ void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& n
     plan->ibase[t] = (uint32_t)ib;
     plan->icap[t] = ic;
     plan->tboff[t] = (uint32_t)bo;
+    plan->meta[t] = RtMeta{(uint32_t)s, sc, (uint32_t)ib, ic, 0u, {0u, 0u, 0u}};
     s += sc;
     ib += ic;
     bo += nb[t] + 1u;
@@ -187,27 +189,109 @@ void rt_expected_counts(const Geom& g, uint64_t agents, std::vector<double>* exp
   }
 }
 
-RtCaps rt_caps(const RtPlan& plan, const Geom& g, int32_t dd, size_t lds_limit, bool* fits) {
-  RtCaps c;
-  // staged stayers per own stayer: the tile plus dd halo columns each side
-  // and one whole bin (hb rows) above and below
-  const double halo = std::min(9.0, (double)(g.tx + 2 * dd) * (double)(g.ty + 2 * g.hb) /
-                                        ((double)g.tx * (double)g.ty));
-  c.nb = (int)std::max<uint32_t>(plan.max_nb, 1u);
-  // own agents of a tile: its stayers plus a share of its inbox
-  const uint32_t want_own = (plan.max_scap + plan.max_icap / 2u + 63u) & ~63u;
-  const double want_runs = std::ceil(plan.max_scap * halo * 1.1 / 64.0) * 64.0;
-  c.own = (int)std::min<uint32_t>((uint32_t)kRtMaxOwn, want_own);
-  c.runs = (int)std::min<double>(16384.0, want_runs);
-  c.extras = (int)std::min<uint32_t>(2048u, std::max<uint32_t>(128u, (2u * plan.max_icap + 63u) & ~63u));
-  while (rt_smem_bytes(c) > lds_limit && (c.runs > 128 || c.own > 64)) {
-    if (c.runs > 128) c.runs -= 64;
-    if (rt_smem_bytes(c) > lds_limit && c.own > 64) c.own -= 64;
+namespace {
+
+uint32_t round_up(double v, uint32_t q) {
+  const double r = std::ceil(std::max(v, 1.0) / q) * q;
+  return (uint32_t)std::min(r, 1.0e9);
+}
+
+}  // namespace
+
+void rt_needs(const RtPlan& plan, const std::vector<uint32_t>& counts, const Geom& g, int32_t dd,
+              std::vector<uint32_t>* own, std::vector<uint32_t>* staged) {
+  const uint32_t nt = (uint32_t)plan.scap.size();
+  // LDS holds the agents actually present, with room for 64 steps of growth
+  // (a tile past it takes the global path once and the host re-plans); the
+  // HBM capacities of the plan are far roomier.
+  std::vector<double> want(nt);
+  for (uint32_t t = 0; t < nt; ++t)
+    want[t] = std::min<double>(plan.scap[t], std::ceil(1.3 * counts[t]) + 32.0);
+  own->resize(nt);
+  staged->resize(nt);
+  // rows staged above / below a tile: whole bins covering dd rows
+  const double hrows = (double)g.hb * std::ceil((double)dd / (double)g.hb);
+  const double fx = std::min(1.0, (double)dd / (double)g.tx);
+  const double fy = std::min(1.0, hrows / (double)g.ty);
+  for (uint32_t t = 0; t < nt; ++t) {
+    const int32_t tcx = (int32_t)(t % (uint32_t)g.ntx), tcy = (int32_t)(t / (uint32_t)g.ntx);
+    double st = want[t];
+    for (int k = 0; k < 9; ++k) {
+      const int32_t ax = tcx - 1 + k % 3, ay = tcy - 1 + k / 3;
+      if (k == 4 || ax < 0 || ax >= g.ntx || ay < 0 || ay >= g.nty) continue;
+      const double w = (ax != tcx ? fx : 1.0) * (ay != tcy ? fy : 1.0);
+      st += w * want[(uint32_t)ay * (uint32_t)g.ntx + (uint32_t)ax];
+    }
+    (*own)[t] = (uint32_t)want[t];
+    (*staged)[t] = (uint32_t)std::min(1.0e9, std::ceil(st * 1.1));
   }
-  if (fits)
-    *fits = (uint32_t)c.own >= want_own && (double)c.runs >= want_runs &&
-            rt_smem_bytes(c) <= lds_limit && c.nb <= kRtMaxBins;
+}
+
+RtCaps rt_caps_for(const Geom& g, int32_t dd, uint32_t own, uint32_t staged, uint32_t icap) {
+  RtCaps c;
+  c.nb = g.tx * g.tr;
+  c.cols = std::min(kMaxStageCols, g.tx + 2 * dd);
+  // staged bins per column: the tile's tr plus whole bins covering dd rows
+  // above and below
+  c.nbc = g.tr + 2 * (int)((dd + g.hb - 1) / g.hb) + 1;
+  c.own = (int)round_up(own, 64);
+  c.runs = (int)round_up(staged, 64);
+  // extras per step: agents that crossed into the staged rectangle, a few
+  // percent of the tile's count; a quarter of the own capacity is ample
+  c.extras = (int)std::min<uint32_t>(2048u, std::max<uint32_t>(64u, round_up(0.25 * own, 64)));
+  (void)icap;
   return c;
+}
+
+void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts, const Geom& g, int32_t dd,
+                size_t lds_normal, size_t lds_limit, RtClassPlan* out) {
+  const uint32_t nt = (uint32_t)plan->scap.size();
+  std::vector<uint32_t> own, staged;
+  rt_needs(*plan, counts, g, dd, &own, &staged);
+  out->dense.clear();
+  out->fits = true;
+  if (nt == 0) {
+    out->caps[0] = out->caps[1] = rt_caps_for(g, dd, 64, 64, 32);
+    return;
+  }
+  // normal class: the largest own-count quantile whose capacities fit the
+  // normal LDS budget; the tiles above it are dense
+  std::vector<uint32_t> so(own);
+  std::sort(so.begin(), so.end());
+  RtCaps cn{};
+  uint32_t own_n = 0, st_n = 0;
+  for (double qtl : {1.0, 0.999, 0.995, 0.99, 0.98, 0.95, 0.9, 0.8, 0.5}) {
+    own_n = so[std::min<size_t>(nt - 1, (size_t)std::floor(qtl * (nt - 1)))];
+    st_n = 0;
+    uint32_t ic = 0;
+    for (uint32_t t = 0; t < nt; ++t)
+      if (own[t] <= own_n) {
+        st_n = std::max(st_n, staged[t]);
+        ic = std::max(ic, plan->icap[t]);
+      }
+    cn = rt_caps_for(g, dd, own_n, st_n, ic);
+    if (rt_smem_bytes(cn) <= lds_normal) break;
+  }
+  uint32_t own_d = 0, st_d = 0, ic_d = 0;
+  for (uint32_t t = 0; t < nt; ++t) {
+    const bool dense = own[t] > (uint32_t)cn.own || staged[t] > (uint32_t)cn.runs;
+    plan->meta[t].cls = dense ? 1u : 0u;
+    if (dense) {
+      out->dense.push_back(t);
+      own_d = std::max(own_d, own[t]);
+      st_d = std::max(st_d, staged[t]);
+      ic_d = std::max(ic_d, plan->icap[t]);
+    }
+  }
+  // the dense launch also takes normal tiles that outgrow their capacities
+  // before the next plan, whose extras may be most of their agents
+  RtCaps cd = rt_caps_for(g, dd, std::max<uint32_t>(own_d, 2u * (uint32_t)cn.own),
+                          std::max<uint32_t>(st_d, 2u * (uint32_t)cn.runs), ic_d);
+  cd.extras = std::min(2048, std::max(cd.extras, cd.own));
+  out->caps[0] = cn;
+  out->caps[1] = cd;
+  out->fits = rt_smem_bytes(cn) <= lds_limit && rt_smem_bytes(cd) <= lds_limit &&
+              cn.nb <= kRtMaxBins && cn.cols <= kMaxStageCols;
 }
 
 bool make_geom_rt(float w, float h, float d, int tor, uint64_t expected_agents, uint32_t tx,

@@ -33,6 +33,7 @@ bool params_valid(const kg_flock_params& p, std::string* why);
 // nb[t] is tx * tr of tile t's replica.
 struct RtPlan {
   std::vector<uint32_t> sbase, scap, ibase, icap, tboff;
+  std::vector<RtMeta> meta;  // (sbase, scap, ibase, icap) per tile, as uploaded
   uint64_t stay_total = 0, inbox_total = 0, boff_total = 0;
   uint32_t max_scap = 0, max_icap = 0, max_nb = 0;
 };
@@ -42,12 +43,25 @@ void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& n
 // Agents each tile of `g` would hold if `agents` were spread uniformly.
 void rt_expected_counts(const Geom& g, uint64_t agents, std::vector<double>* expect);
 
-// LDS capacities of the tile-resident kernel for a plan, the tile geometry
-// and the window reach dd, limited to `lds_limit` bytes.
-// *fits tells whether the plan's working set fits (else the tile-resident
-// path would fall back to its slow global mode on most tiles).
-RtCaps rt_caps(const RtPlan& plan, const Geom& g, int32_t dd, size_t lds_limit,
-               bool* fits = nullptr);
+// Per-tile LDS needs of a plan for the current per-tile counts: own agents
+// (the count with 30 % + 32 of headroom) and staged stayers (that plus the
+// neighbours' halo shares, with 10 % headroom).
+void rt_needs(const RtPlan& plan, const std::vector<uint32_t>& counts, const Geom& g, int32_t dd,
+              std::vector<uint32_t>* own, std::vector<uint32_t>* staged);
+
+// Capacities of one launch for the given needs.
+RtCaps rt_caps_for(const Geom& g, int32_t dd, uint32_t own, uint32_t staged, uint32_t icap);
+
+// Splits the tiles into a normal class, whose capacities fit `lds_normal`
+// bytes of LDS per work-group, and a dense class (sets plan->meta[].cls).
+// fits = both classes fit `lds_limit`.
+struct RtClassPlan {
+  RtCaps caps[2];
+  std::vector<uint32_t> dense;  // tiles of the dense class
+  bool fits = false;
+};
+void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts, const Geom& g, int32_t dd,
+                size_t lds_normal, size_t lds_limit, RtClassPlan* out);
 
 // Geometry for the tile-resident layout: tiles of about `own_target` agents.
 bool make_geom_rt(float w, float h, float d, int tor, uint64_t expected_agents, uint32_t tx,

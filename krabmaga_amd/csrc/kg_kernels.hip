@@ -507,215 +507,245 @@ hipError_t launch_remove(const RepDev* rep, Buf A, uint32_t n, uint32_t id, floa
 
 size_t rt_lds_bytes(const RtCaps& c) { return rt_smem_bytes(c); }
 
+// Output phase of a tile (after its stayers are classified and counted per
+// bin in LDS): sort the stayers by (bin, cell y, id), write them and the new
+// bin offsets.
+__device__ __forceinline__ void rt_output(const RepDev& rp, const RtTile& rt, const RtDev& D,
+                                          const RtMeta& me, uint32_t bo_own, int nb_tile,
+                                          uint32_t nstay, RtSmem S, uint32_t* s_warp) {
+  const int t = threadIdx.x;
+  const int pout = rt.pin ^ 1;
+  // exclusive scan of the nb_tile bin counts, 4 per thread
+  uint32_t v[4], sum = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int b = 4 * t + k;
+    v[k] = b < nb_tile ? S.hist[b] : 0u;
+    sum += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_exclusive_scan<kRtThreads>(sum, s_warp, &tot);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int b = 4 * t + k;
+    if (b < nb_tile) {
+      S.bbase[b] = run;
+      S.bcur[b] = run;
+    }
+    run += v[k];
+  }
+  if (t == 0) S.bbase[nb_tile] = nstay;
+  __syncthreads();
+  for (uint32_t s = t; s < nstay; s += kRtThreads) {
+    const uint32_t pos = atomicAdd(&S.bcur[S.obin[s]], 1u);
+    S.operm[pos] = (uint16_t)s;
+  }
+  __syncthreads();
+  const uint32_t scap = me.scap;
+  for (uint32_t s = t; s < nstay; s += kRtThreads) rt_write_stayer(rp, rt, D, me, s, S);
+  for (int b = t; b <= nb_tile; b += kRtThreads)
+    D.boff[pout][bo_own + b] = S.bbase[b] < scap ? S.bbase[b] : scap;
+  if (t == 0) D.cnt[pout][rt.gt] = nstay < scap ? nstay : scap;
+}
+
+// One schedule step of the tiles of class `cls`: cls 0 covers every tile
+// (dense tiles return at once), cls 1 the tiles in `list`.
 __global__ __launch_bounds__(kRtThreads) void k_rt_step(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, RtDev D,
-    RtCaps caps, int load, Buf Ra, const uint32_t* __restrict__ offset_a, uint64_t load_g) {
+    RtCaps caps, uint32_t koff, int cls, const uint32_t* __restrict__ list, uint32_t nlist) {
   extern __shared__ __align__(16) uint8_t smem_raw[];
   __shared__ uint32_t s_seg_start[3 * kMaxStageCols];
   __shared__ uint32_t s_seg_len[3 * kMaxStageCols];
   __shared__ uint32_t s_col_base[kMaxStageCols + 1];
-  __shared__ uint32_t s_ecol_base[kMaxStageCols + 1];
   __shared__ uint32_t s_src_base[10];
   __shared__ uint32_t s_src_pre[11];
   __shared__ uint32_t s_warp[kRtThreads / kWave];
-  __shared__ uint32_t s_ne, s_nown, s_nstay, s_fail;
+  __shared__ uint32_t s_ne, s_nown, s_nstay, s_fail, s_defer;
 
   const int t = threadIdx.x;
-  const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
-  const uint64_t g = load ? load_g : *D.gstep;
+  // schedule step: launch koff of a batch that starts at *gstep
+  const uint64_t g = *D.gstep + koff;
+  const int par = (int)(g & 1u);
+  uint32_t item;
+  bool deferred = false;
+  if (cls == 0) {
+    item = xcd_remap(blockIdx.x, gridDim.x);
+  } else if (blockIdx.x < nlist) {
+    item = list[blockIdx.x];
+  } else {
+    // deferred normal tiles of this step; block nlist clears the list that
+    // the next step's normal launch fills
+    if (blockIdx.x == nlist && t == 0) D.ovf[par ^ 1][0] = 0u;
+    const uint32_t i = blockIdx.x - nlist;
+    const uint32_t n = D.ovf[par][0];
+    if (i >= n || i >= kRtOvfMax) return;
+    item = D.ovf[par][1 + i];
+    deferred = true;
+  }
   const RtTile rt = rt_tile(reps, nreps, tile_prefix, item, g);
+  const RtMeta me = D.meta[rt.gt];
+  if (!deferred && (int)me.cls != cls) return;
   const RepDev& rp = reps[rt.r];
   const RtSmem S = rt_carve(smem_raw, caps);
   const int nb_tile = rp.g.tx * rp.g.tr;
   const int pout = rt.pin ^ 1;
-
-  float nx[kRtOwnPerThread], ny[kRtOwnPerThread], nhx[kRtOwnPerThread], nhy[kRtOwnPerThread];
-  uint32_t nid[kRtOwnPerThread];
-  uint32_t nown = 0;
-  bool global_path = false;
+  const uint32_t bo_own = rt_boff_at(rp, rt, rt.gt);
+  const RtSegs sg = rt_segs(rp.g, rt);
   if (t == 0) {
     s_ne = 0;
     s_nown = 0;
     s_nstay = 0;
     s_fail = 0;
+    s_defer = 0;
   }
 
-  if (!load) {
-    const bool lds_ok = rp.lds_ok != 0 && rt.ncols <= kMaxStageCols && nb_tile <= caps.nb;
-    uint32_t len = 0;
-    if (lds_ok && t < rt.ncols) len = rt_segments(rp, rt, D, t, s_seg_start, s_seg_len);
-    if (t < 10) {
-      uint32_t b;
-      const uint32_t n = rt_extra_count(rp, rt, D, t, &b);
-      s_src_base[t] = b;
-      s_src_pre[t + 1] = n;
-    }
-    uint32_t total;
-    const uint32_t ex = block_exclusive_scan<kRtThreads>(len, s_warp, &total);
-    if (lds_ok && t < rt.ncols) s_col_base[t] = ex;
-    if (t == 0) {
-      s_col_base[rt.ncols] = total;
-      s_src_pre[0] = 0;
-      for (int k = 0; k < 10; ++k) s_src_pre[k + 1] += s_src_pre[k];
-    }
-    __syncthreads();
-    global_path = !lds_ok || total > (uint32_t)caps.runs;
-    if (!global_path) {
-      const uint32_t next = s_src_pre[10];
-      for (uint32_t j = t; j < next; j += kRtThreads)
-        if (!rt_gather_extra(rp, rt, D, j, s_src_base, s_src_pre, caps.extras, &s_ne, S))
-          s_fail = 1u;
-      for (uint32_t k = t; k < total; k += kRtThreads)
-        rt_stage(rp, rt, D, k, s_seg_start, s_seg_len, s_col_base, S);
-      __syncthreads();
-      global_path = s_fail != 0;
-      if (!global_path) {
-        const uint32_t ne = s_ne;
-        for (uint32_t j = t; j < ne; j += kRtThreads) rt_rank_extra(j, ne, S);
-        for (int c = t; c <= rt.ncols; c += kRtThreads) s_ecol_base[c] = rt_extra_col_base(c, ne, S);
-        for (uint32_t k = t; k < total; k += kRtThreads)
-          if (rt_own(rt, rt.sc_lo + S.scol[k], S.scy[k])) {
-            const uint32_t o = atomicAdd(&s_nown, 1u);
-            if (o < (uint32_t)caps.own) S.own[o] = (uint16_t)k;
-          }
-        for (uint32_t j = t; j < ne; j += kRtThreads)
-          if (rt_own(rt, rt.sc_lo + S.ecol[j], S.ecy[j])) {
-            const uint32_t o = atomicAdd(&s_nown, 1u);
-            if (o < (uint32_t)caps.own) S.own[o] = (uint16_t)(0x8000u | j);
-          }
-        __syncthreads();
-        nown = s_nown;
-        global_path = nown > (uint32_t)caps.own;
-        if (!global_path) {
-#pragma unroll
-          for (int jj = 0; jj < kRtOwnPerThread; ++jj) {
-            const uint32_t o = t + jj * kRtThreads;
-            if (o < nown)
-              rt_agent_lds(rp, rt, S.own[o], s_col_base, s_ecol_base, S, &nx[jj], &ny[jj],
-                           &nhx[jj], &nhy[jj], &nid[jj]);
-          }
-        }
-      }
-    }
-  } else {
-    // load: the tile's agents from the bin-grouped read buffer, unchanged
-    const uint32_t tile = rt.gt - rt.tbase;
-    const uint32_t b0 = rp.bin_base + (tile << (rp.g.tx_log2 + rp.g.tr_log2));
-    const uint32_t lo = offset_a[b0], hi = offset_a[b0 + (uint32_t)nb_tile];
-    nown = hi - lo;
-    global_path = nown > (uint32_t)caps.own || nb_tile > caps.nb;
-    if (global_path) {
-      for (uint32_t i = lo + t; i < hi; i += kRtThreads) {
-        int32_t cx, cy;
-        discretize(rp.g, Ra.x[i], Ra.y[i], &cx, &cy);
-        rt_push(rp, rt, D, rt.tbase + tile_of(rp.g, cx, cy), Ra.x[i], Ra.y[i], Ra.hx[i], Ra.hy[i],
-                Ra.id[i]);
-      }
-    } else {
-#pragma unroll
-      for (int jj = 0; jj < kRtOwnPerThread; ++jj) {
-        const uint32_t o = t + jj * kRtThreads;
-        if (o < nown) {
-          nx[jj] = Ra.x[lo + o];
-          ny[jj] = Ra.y[lo + o];
-          nhx[jj] = Ra.hx[lo + o];
-          nhy[jj] = Ra.hy[lo + o];
-          nid[jj] = Ra.id[lo + o];
-        }
-      }
-    }
+  const bool lds_ok = rp.lds_ok != 0 && rt_fits_tables(rt, sg, caps) && nb_tile <= caps.nb;
+  if (lds_ok)
+    for (int b = t; b <= nb_tile; b += kRtThreads) S.hist[b] = 0u;
+  uint32_t len = 0;
+  // waves 0-2: the staged columns' runs; wave 3: the extras' sources
+  if (lds_ok && t < rt.ncols) len = rt_segments(rp, rt, D, t, s_seg_start, s_seg_len);
+  const int ks = t - (kRtThreads - kWave);
+  if (ks >= 0 && ks < 10) {
+    uint32_t b;
+    const uint32_t n = rt_extra_count(rp, rt, D, ks, &b);
+    s_src_base[ks] = b;
+    s_src_pre[ks + 1] = n;
+  }
+  uint32_t total;
+  const uint32_t ex = block_exclusive_scan<kRtThreads>(len, s_warp, &total);
+  if (lds_ok && t < rt.ncols) s_col_base[t] = ex;
+  if (t == 0) {
+    s_col_base[rt.ncols] = total;
+    s_src_pre[0] = 0;
+    for (int k = 0; k < 10; ++k) s_src_pre[k + 1] += s_src_pre[k];
   }
   __syncthreads();
-
-  if (global_path) {
-    if (!load) {
-      if (t == 0) atomicOr(D.flags, kRtFlagGlobal);
-      for (uint32_t o = t;; o += kRtThreads) {
-        float x, y, hx, hy;
-        uint32_t id;
-        bool skip;
-        if (!rt_own_global(rp, rt, D, o, &x, &y, &hx, &hy, &id, &skip)) break;
-        if (skip) continue;
-        float ax, ay, ahx, ahy;
-        rt_agent_global(rp, rt, D, x, y, hx, hy, id, &ax, &ay, &ahx, &ahy);
-        int32_t cx, cy;
-        discretize(rp.g, ax, ay, &cx, &cy);
-        if (!cell_valid(rp.g, cx, cy)) {
-          atomicOr(D.flags, kRtFlagRange);
-          continue;
-        }
-        rt_push(rp, rt, D, rt.tbase + tile_of(rp.g, cx, cy), ax, ay, ahx, ahy, id);
+  bool global_path = !lds_ok || total > (uint32_t)caps.runs;
+  uint32_t nown = 0, ne = 0;
+  if (!global_path) {
+    const uint32_t next = s_src_pre[10];
+    if (!rt_fetch(rp, rt, D, t, t + kRtThreads, total, t, next, s_seg_start, s_seg_len, s_col_base,
+                  s_src_base, s_src_pre, caps, &s_ne, &s_nown, S))
+      s_fail = 1u;
+    for (uint32_t k = t + 2 * kRtThreads; k < total; k += kRtThreads)
+      rt_stage(rp, rt, D, k, s_seg_start, s_seg_len, s_col_base, caps.own, &s_nown, S);
+    for (uint32_t j = t + kRtThreads; j < next; j += kRtThreads)
+      if (!rt_gather_extra(rp, rt, D, j, s_src_base, s_src_pre, caps.extras, &s_ne, caps.own,
+                           &s_nown, S))
+        s_fail = 1u;
+    const uint32_t nbt = (uint32_t)(rt.ncols * sg.nbc);
+    for (uint32_t q = t; q < nbt; q += kRtThreads) rt_bintab(rp, rt, D, sg, q, s_seg_len, S);
+    __syncthreads();
+    nown = s_nown;
+    ne = s_ne;
+    global_path = s_fail != 0 || nown > (uint32_t)caps.own;
+  }
+  if (global_path && cls == 0 && lds_ok) {
+    // outgrew the normal capacities: hand the tile to the dense launch
+    // (nothing of it has been written yet)
+    if (t == 0) {
+      const uint32_t slot = atomicAdd(&D.ovf[par][0], 1u);
+      if (slot < kRtOvfMax) {
+        D.ovf[par][1 + slot] = rt.gt;
+        s_defer = 1u;
       }
-    }
-    for (int b = t; b <= nb_tile; b += kRtThreads) D.boff[pout][D.tboff[rt.gt] + b] = 0u;
-    if (t == 0) D.cnt[pout][rt.gt] = 0u;
-  } else {
-#pragma unroll
-    for (int jj = 0; jj < kRtOwnPerThread; ++jj) {
-      const uint32_t o = t + jj * kRtThreads;
-      if (o < nown)
-        rt_classify(rp, rt, D, false, nx[jj], ny[jj], nhx[jj], nhy[jj], nid[jj], &s_nstay, S);
+      atom_or(D.flags, kRtFlagDefer);
     }
     __syncthreads();
-    const uint32_t nstay = s_nstay;
-    const uint32_t scap = D.scap[rt.gt];
-    {
-      for (int b = t; b <= nb_tile; b += kRtThreads) S.hist[b] = 0u;
-      __syncthreads();
-      for (uint32_t s = t; s < nstay; s += kRtThreads) atomicAdd(&S.hist[S.obin[s]], 1u);
-      __syncthreads();
-      // exclusive scan of the nb_tile bin counts, 4 per thread
-      uint32_t v[4], sum = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int b = 4 * t + k;
-        v[k] = b < nb_tile ? S.hist[b] : 0u;
-        sum += v[k];
-      }
-      uint32_t tot;
-      uint32_t run = block_exclusive_scan<kRtThreads>(sum, s_warp, &tot);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int b = 4 * t + k;
-        if (b < nb_tile) {
-          S.bbase[b] = run;
-          S.bcur[b] = run;
-        }
-        run += v[k];
-      }
-      if (t == 0) S.bbase[nb_tile] = nstay;
-      __syncthreads();
-      for (uint32_t s = t; s < nstay; s += kRtThreads) {
-        const uint32_t pos = atomicAdd(&S.bcur[S.obin[s]], 1u);
-        S.operm[pos] = (uint16_t)s;
-      }
-      __syncthreads();
-      for (uint32_t s = t; s < nstay; s += kRtThreads) rt_write_stayer(rp, rt, D, s, S);
-      for (int b = t; b <= nb_tile; b += kRtThreads)
-        D.boff[pout][D.tboff[rt.gt] + b] = S.bbase[b] < scap ? S.bbase[b] : scap;
-      if (t == 0) D.cnt[pout][rt.gt] = nstay < scap ? nstay : scap;
-    }
+    if (s_defer) return;
+  }
+  if (!global_path) {
+    // extras sorted by (column, row, id) and their column starts
+    for (uint32_t j = t; j < ne; j += kRtThreads) rt_rank_extra(j, ne, S);
+    for (int c = t; c <= rt.ncols; c += kRtThreads) rt_ecbase(c, ne, S);
+    __syncthreads();
+    // queries and Bird::step, then classification of the thread's own
+    // results (movers are pushed at once); no barrier between the two
+    for (uint32_t o = t; o < nown; o += kRtThreads) rt_agent_lds(rp, rt, sg, o, s_col_base, S);
+    for (uint32_t o = t; o < nown; o += kRtThreads) rt_classify(rp, rt, D, o, &s_nstay, S);
+    __syncthreads();
+    rt_output(rp, rt, D, me, bo_own, nb_tile, s_nstay, S, s_warp);
+  } else {
+    if (t == 0) atom_or(D.flags, kRtFlagGlobal);
+    rt_tile_global(rp, rt, D, (uint32_t)t, kRtThreads);
+    for (int b = t; b <= nb_tile; b += kRtThreads) D.boff[pout][bo_own + b] = 0u;
+    if (t == 0) D.cnt[pout][rt.gt] = 0u;
   }
 
-  if (!load && t == 0) {
+  if (t == 0) {
+    // counters of step g + 2: nothing reads or writes them during step g
     D.icnt[rt.ph_reset][rt.gt] = 0u;
     if (rt.gt == 0) *D.scnt[rt.ph_reset] = 0u;
-    // relaxed: the count carries no data; the next launch reads gstep
-    // after this kernel has completed
-    const uint32_t prev = __hip_atomic_fetch_add(D.done, 1u, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-    if (prev == gridDim.x - 1) {
-      *D.gstep = g + 1;
-      __hip_atomic_store(D.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
   }
 }
 
-hipError_t launch_rt_step(const WorkList& wl, const RtDev& D, const RtCaps& caps, int load,
-                          Buf Ra, const uint32_t* offset_a, uint64_t load_g, hipStream_t s) {
-  if (wl.ntiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rt_step, dim3(wl.ntiles), dim3(kRtThreads), rt_lds_bytes(caps), s, wl.reps,
-                     wl.nreps, wl.tile_prefix, D, caps, load, Ra, offset_a, load_g);
+// Entry into the tile-resident layout: each tile of class `cls` takes its
+// agents from the bin-grouped read buffer Ra (schedule step load_g's output)
+// and writes them as the stayers of parity (load_g + 1) & 1.
+__global__ __launch_bounds__(kRtThreads) void k_rt_load(
+    const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, RtDev D,
+    RtCaps caps, int cls, const uint32_t* __restrict__ list, Buf Ra,
+    const uint32_t* __restrict__ offset_a, uint64_t load_g) {
+  extern __shared__ __align__(16) uint8_t smem_raw[];
+  __shared__ uint32_t s_warp[kRtThreads / kWave];
+  __shared__ uint32_t s_nstay;
+  const int t = threadIdx.x;
+  const uint32_t item = cls ? list[blockIdx.x] : xcd_remap(blockIdx.x, gridDim.x);
+  const RtTile rt = rt_tile(reps, nreps, tile_prefix, item, load_g);
+  const RtMeta me = D.meta[rt.gt];
+  if ((int)me.cls != cls) return;
+  const RepDev& rp = reps[rt.r];
+  const RtSmem S = rt_carve(smem_raw, caps);
+  const int nb_tile = rp.g.tx * rp.g.tr;
+  const int pout = rt.pin ^ 1;
+  const uint32_t bo_own = rt_boff_at(rp, rt, rt.gt);
+  const uint32_t tile = rt.gt - rt.tbase;
+  const uint32_t b0 = rp.bin_base + (tile << (rp.g.tx_log2 + rp.g.tr_log2));
+  const uint32_t lo = offset_a[b0], hi = offset_a[b0 + (uint32_t)nb_tile];
+  const uint32_t nown = hi - lo;
+  if (t == 0) s_nstay = 0;
+  if (nown > (uint32_t)caps.own || nb_tile > caps.nb) {
+    for (uint32_t i = lo + t; i < hi; i += kRtThreads) {
+      int32_t cx, cy;
+      discretize(rp.g, Ra.x[i], Ra.y[i], &cx, &cy);
+      rt_push(rp, rt, D, rt.tbase + tile_of(rp.g, cx, cy), Ra.x[i], Ra.y[i], Ra.hx[i], Ra.hy[i],
+              Ra.id[i]);
+    }
+    for (int b = t; b <= nb_tile; b += kRtThreads) D.boff[pout][bo_own + b] = 0u;
+    if (t == 0) D.cnt[pout][rt.gt] = 0u;
+    return;
+  }
+  for (int b = t; b <= nb_tile; b += kRtThreads) S.hist[b] = 0u;
+  __syncthreads();
+  for (uint32_t o = t; o < nown; o += kRtThreads) {
+    S.rx[o] = Ra.x[lo + o];
+    S.ry[o] = Ra.y[lo + o];
+    S.rhx[o] = Ra.hx[lo + o];
+    S.rhy[o] = Ra.hy[lo + o];
+    S.rid[o] = Ra.id[lo + o];
+    rt_classify(rp, rt, D, o, &s_nstay, S);
+  }
+  __syncthreads();
+  rt_output(rp, rt, D, me, bo_own, nb_tile, s_nstay, S, s_warp);
+}
+
+hipError_t launch_rt_step(const WorkList& wl, const RtDev& D, const RtCaps& caps, uint32_t koff,
+                          int cls, const uint32_t* list, uint32_t nlist, hipStream_t s) {
+  const uint32_t grid = cls ? nlist + kRtOvfMax : wl.ntiles;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rt_step, dim3(grid), dim3(kRtThreads), rt_lds_bytes(caps), s, wl.reps,
+                     wl.nreps, wl.tile_prefix, D, caps, koff, cls, list, nlist);
+  return hipGetLastError();
+}
+
+hipError_t launch_rt_load(const WorkList& wl, const RtDev& D, const RtCaps& caps, int cls,
+                          const uint32_t* list, uint32_t nlist, Buf Ra, const uint32_t* offset_a,
+                          uint64_t load_g, hipStream_t s) {
+  const uint32_t grid = cls ? nlist : wl.ntiles;
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rt_load, dim3(grid), dim3(kRtThreads), rt_lds_bytes(caps), s, wl.reps,
+                     wl.nreps, wl.tile_prefix, D, caps, cls, list, Ra, offset_a, load_g);
   return hipGetLastError();
 }
 
@@ -738,7 +768,7 @@ __global__ void k_rt_unload(const RepDev* __restrict__ reps, int nreps,
     uint32_t gi;
     if (o < ns) {
       B = &D.stay[rt.pin];
-      gi = D.sbase[rt.gt] + o;
+      gi = D.meta[rt.gt].sbase + o;
     } else if (o < ns + ni) {
       B = &D.inbox[rt.pin];
       gi = ib + (o - ns);
@@ -763,6 +793,21 @@ hipError_t launch_rt_unload(const WorkList& wl, const RtDev& D, Buf A, uint32_t*
   return hipGetLastError();
 }
 
+// count[t] = agents of tile t in the bin-grouped layout (tiles are runs of
+// nbt consecutive bins).
+__global__ void k_tile_counts(const uint32_t* __restrict__ offset, uint32_t nbt, uint32_t nt,
+                              uint32_t* __restrict__ count) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nt) count[t] = offset[(size_t)(t + 1) * nbt] - offset[(size_t)t * nbt];
+}
+
+hipError_t launch_tile_counts(const uint32_t* offset, uint32_t nbt, uint32_t nt, uint32_t* count,
+                              hipStream_t s) {
+  if (nt == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tile_counts, dim3((nt + 255) / 256), dim3(256), 0, s, offset, nbt, nt, count);
+  return hipGetLastError();
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -784,6 +829,15 @@ __global__ void k_set_u64(uint64_t* p, uint64_t v) {
 
 hipError_t launch_set_u64(uint64_t* p, uint64_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, p, v);
+  return hipGetLastError();
+}
+
+__global__ void k_add_u64(uint64_t* p, uint64_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p += v;
+}
+
+hipError_t launch_add_u64(uint64_t* p, uint64_t v, hipStream_t s) {
+  hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, s, p, v);
   return hipGetLastError();
 }
 

@@ -123,12 +123,30 @@ KG_HD int32_t t_transform(int32_t n, int32_t size) {
   return (n % size) + size;
 }
 
-// src/engine/fields/field_2d.rs:1004-1014 (Rust `%` on f32 is fmod)
-KG_HD float toroidal_transform(float val, float dim) {
-  if (val >= 0.0f && val < dim) return val;
+// General case of toroidal_transform, kept out of line: fmodf expands to a
+// long reduction loop and almost never runs.
+#ifdef __HIPCC__
+__host__ __device__ inline __attribute__((noinline))
+#else
+inline
+#endif
+float toroidal_transform_slow(float val, float dim) {
   float v = fmodf(val, dim);
   if (v < 0.0f) v += dim;
   return v;
+}
+
+// src/engine/fields/field_2d.rs:1004-1014 (Rust `%` on f32 is fmod).  For
+// dim <= val < 2 dim, fmod(val, dim) = val - dim and the subtraction is exact
+// (Sterbenz); for -dim < val < 0, fmod(val, dim) = val.  Both shortcuts give
+// the bits of the general formula; other values take the fmodf path.
+KG_HD float toroidal_transform(float val, float dim) {
+  if (val >= 0.0f && val < dim) return val;
+  if (dim > 0.0f) {
+    if (val >= dim && val < 2.0f * dim) return val - dim;
+    if (val < 0.0f && val > -dim) return val + dim;
+  }
+  return toroidal_transform_slow(val, dim);
 }
 
 // src/engine/fields/field_2d.rs:988-1002

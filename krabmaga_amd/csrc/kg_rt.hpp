@@ -1,6 +1,6 @@
 // kg_rt.hpp -- tile-resident agent step: the whole Schedule::step of the
 // flockers model (neighbour query, Bird::step, rebinning, buffer swap) in one
-// kernel launch.
+// kernel launch per tile class.
 //
 // Layout.  Every tile of a replica owns a region of the stayer arrays
 // (capacity scap, base sbase) holding its agents sorted by (bin, cell y, id)
@@ -17,13 +17,21 @@
 // Per step and tile (one work-group): stage the tile's own stayers plus the
 // halo rows/columns its agents' clamped windows reach (already sorted) into
 // LDS, gather the inbox/spill entries ("extras") of the 3 x 3 tile block that
-// fall in the staged rectangle and sort them per column, run every own
-// agent's query as a merge of the two sorted sequences (so neighbours are
-// visited in bag-scan order, field_2d.rs:418-435), compute Bird::step
-// (bird.rs:38-154), and write the new states: stayers sorted into the other
-// parity's region, movers into their destination tile's inbox.  Tiles whose
-// working set does not fit the LDS capacities run the same step from global
-// memory and send all their agents through the inboxes.
+// fall in the staged rectangle and sort them per column, index every staged
+// column by cell row, run every own agent's query as a merge of the two
+// sorted sequences (so neighbours are visited in bag-scan order,
+// field_2d.rs:418-435), compute Bird::step (bird.rs:38-154) into an LDS
+// result slot, then write the new states: stayers sorted into the other
+// parity's region, movers into their destination tile's inbox.
+//
+// Tile classes.  Tiles are planned as normal or dense.  The normal launch
+// covers every tile with LDS capacities sized for normal tiles and returns
+// at once for dense ones; a second launch runs the dense tiles (a short
+// list) with larger capacities, plus the normal tiles that outgrew their
+// capacities since the plan (they defer themselves before writing anything).
+// A tile whose working set still exceeds its launch's capacities runs the
+// same step from global memory and sends all its agents through the inboxes.
+// Deferral and the global path raise flags that make the host re-plan.
 #pragma once
 
 #include "kg_phases.hpp"
@@ -31,21 +39,31 @@
 namespace kg {
 
 constexpr int kRtThreads = 256;
-constexpr int kRtOwnPerThread = 4;
-constexpr int kRtMaxOwn = kRtThreads * kRtOwnPerThread;
 constexpr int kRtMaxBins = 1024;  // tx * tr of a tile on the LDS path
 
 constexpr uint32_t kRtFlagSpill = 1u;    // an inbox overflowed into the spill list
 constexpr uint32_t kRtFlagGlobal = 2u;   // a tile took the global path
 constexpr uint32_t kRtFlagRange = 4u;    // a location left the bag grid
 constexpr uint32_t kRtFlagStayCap = 8u;  // a tile's stayers exceeded its capacity
+constexpr uint32_t kRtFlagDefer = 16u;   // a normal tile outgrew its class and was deferred
+
+// Normal tiles that outgrow their launch's capacities defer to the dense
+// launch of the same step through a per-parity list ([0] count, then ids);
+// the dense launch has kRtOvfMax work-groups beyond its static tiles.
+constexpr uint32_t kRtOvfMax = 32;
+
+// Per-tile plan entry, two 16-byte loads.
+struct alignas(16) RtMeta {
+  uint32_t sbase;  // first stayer slot
+  uint32_t scap;   // stayer capacity
+  uint32_t ibase;  // first inbox slot
+  uint32_t icap;   // inbox capacity
+  uint32_t cls;    // 0 normal, 1 dense
+  uint32_t pad[3];
+};
 
 struct RtDev {
-  const uint32_t* sbase;  // per global tile
-  const uint32_t* scap;
-  const uint32_t* ibase;
-  const uint32_t* icap;
-  const uint32_t* tboff;  // start of the tile's boff table
+  const RtMeta* meta;  // per global tile
   uint32_t* cnt[2];
   uint32_t* boff[2];
   uint32_t* icnt[3];
@@ -56,16 +74,18 @@ struct RtDev {
   uint32_t* spill_rep[2];  // replica of each spill entry
   uint32_t spill_cap;
   uint32_t* flags;
-  uint32_t* done;
-  uint64_t* gstep;
+  uint64_t* gstep;  // schedule step of launch k = step_base + *gstep + k
+  uint32_t* ovf[2];  // deferred normal tiles, by step parity (kRtOvfMax + 1 each)
 };
 
 // LDS capacities of one launch.
 struct RtCaps {
-  int runs;   // staged stayers
-  int extras; // staged inbox/spill entries
-  int own;    // own agents (<= kRtMaxOwn)
-  int nb;     // max tx * tr over the replicas (<= kRtMaxBins)
+  int runs;    // staged stayers
+  int extras;  // staged inbox/spill entries
+  int own;     // own agents
+  int nb;      // max tx * tr over the replicas (<= kRtMaxBins)
+  int nbc;     // bin-table entries per staged column (staged bins + 1)
+  int cols;    // staged columns (<= kMaxStageCols)
 };
 
 struct RtTile {
@@ -115,33 +135,95 @@ KG_HD int64_t rt_tile_at(const Geom& g, const RtTile& rt, int32_t ttx, int32_t t
   return (int64_t)rt.tbase + (int64_t)tty * g.ntx + ttx;
 }
 
+// Start of tile tt's bin-offset table (tx * tr + 1 entries) in D.boff[p]:
+// the tiles of a replica have equal bin counts and consecutive tables.
+KG_HD uint32_t rt_boff_at(const RepDev& rp, const RtTile& rt, uint32_t tt) {
+  return rp.boff_base + (tt - rt.tbase) * (uint32_t)(rp.g.tx * rp.g.tr + 1);
+}
+
+// Rows of a staged column: up to three segments, the bins of tile rows
+// tcy-1, tcy, tcy+1 that cover [ry_lo, ry_hi].  The same for every column.
+struct RtSegs {
+  int32_t blo[3];   // first staged bin (within its tile column)
+  int32_t nbin[3];  // staged bins of the segment (0 if none)
+  int32_t pre[3];   // bin-table index of the segment's first bin
+  int32_t nbc;      // bin-table entries per column (staged bins + 1)
+};
+
+KG_HD RtSegs rt_segs(const Geom& g, const RtTile& rt) {
+  RtSegs sg;
+  int32_t pre = 0;
+  for (int k = 0; k < 3; ++k) {
+    const int32_t tty = rt.tcy - 1 + k;
+    const int32_t by0 = tty * g.ty;
+    const int32_t lo_row = imax(rt.ry_lo, by0);
+    const int32_t hi_row = imin(rt.ry_hi, by0 + g.ty - 1);
+    const bool on = tty >= 0 && tty < g.nty && lo_row <= hi_row;
+    sg.blo[k] = on ? (lo_row - by0) >> g.hb_log2 : 0;
+    sg.nbin[k] = on ? ((hi_row - by0) >> g.hb_log2) + 1 - sg.blo[k] : 0;
+    sg.pre[k] = pre;
+    pre += sg.nbin[k];
+  }
+  sg.nbc = pre + 1;
+  return sg;
+}
+
+// Element k (0..2) of a three-entry table without indexing memory (a
+// per-lane index into a local array would put the array in scratch).
+KG_HD int32_t rt_pick3(const int32_t* a, int32_t k) {
+  return k == 0 ? a[0] : (k == 1 ? a[1] : a[2]);
+}
+
+// Bin-table index of staged row j (ry_lo <= j <= ry_hi).
+KG_HD int32_t rt_seg_bin(const Geom& g, const RtTile& rt, const RtSegs& sg, int32_t j) {
+  const int32_t k = (j >> (g.tr_log2 + g.hb_log2)) - (rt.tcy - 1);
+  const int32_t b = (j & (g.ty - 1)) >> g.hb_log2;
+  return rt_pick3(sg.pre, k) + b - rt_pick3(sg.blo, k);
+}
+
+// Whether the tile's staged rectangle fits the launch's tables.
+KG_HD bool rt_fits_tables(const RtTile& rt, const RtSegs& sg, const RtCaps& caps) {
+  return rt.ncols <= caps.cols && rt.ncols <= kMaxStageCols && sg.nbc <= caps.nbc;
+}
+
 // Staged column t: up to three sorted stayer runs (tile rows tcy-1, tcy,
-// tcy+1) covering rows [ry_lo, ry_hi].  Returns the column's length.
+// tcy+1) covering rows [ry_lo, ry_hi].  Returns the column's length.  The
+// table loads of all three rows are issued before any is used.
 KG_HD uint32_t rt_segments(const RepDev& rp, const RtTile& rt, const RtDev& D, int t,
                            uint32_t* seg_start, uint32_t* seg_len) {
   const Geom& g = rp.g;
   const int32_t col = rt.sc_lo + t;
   const int32_t ttx = col >> g.tx_log2;
   const int32_t lc = col & (g.tx - 1);
-  uint32_t len = 0;
+  bool on[3];
+  uint32_t tt[3], blo[3], bhi[3];
+#pragma unroll
   for (int k = 0; k < 3; ++k) {
-    uint32_t st = 0, ln = 0;
     const int32_t tty = rt.tcy - 1 + k;
-    const int64_t tt = rt_tile_at(g, rt, ttx, tty);
-    if (tt >= 0) {
-      const int32_t by0 = tty * g.ty;
-      const int32_t lo_row = imax(rt.ry_lo, by0);
-      const int32_t hi_row = imin(rt.ry_hi, by0 + g.ty - 1);
-      if (lo_row <= hi_row) {
-        const uint32_t* bo = D.boff[rt.pin] + D.tboff[tt];
-        const uint32_t b_lo = ((uint32_t)lc << g.tr_log2) + (uint32_t)((lo_row - by0) >> g.hb_log2);
-        const uint32_t b_hi =
-            ((uint32_t)lc << g.tr_log2) + (uint32_t)((hi_row - by0) >> g.hb_log2) + 1u;
-        st = D.sbase[tt] + bo[b_lo];
-        ln = bo[b_hi] - bo[b_lo];
-      }
-    }
-    seg_start[3 * t + k] = st;
+    const int64_t t64 = rt_tile_at(g, rt, ttx, tty);
+    const int32_t by0 = tty * g.ty;
+    const int32_t lo_row = imax(rt.ry_lo, by0);
+    const int32_t hi_row = imin(rt.ry_hi, by0 + g.ty - 1);
+    on[k] = t64 >= 0 && lo_row <= hi_row;
+    tt[k] = on[k] ? (uint32_t)t64 : rt.gt;
+    blo[k] = on[k] ? ((uint32_t)lc << g.tr_log2) + (uint32_t)((lo_row - by0) >> g.hb_log2) : 0u;
+    bhi[k] = on[k] ? ((uint32_t)lc << g.tr_log2) + (uint32_t)((hi_row - by0) >> g.hb_log2) + 1u
+                   : 0u;
+  }
+  const uint32_t* bo = D.boff[rt.pin];
+  uint32_t sb[3], lo[3], hi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t base = rt_boff_at(rp, rt, tt[k]);
+    sb[k] = D.meta[tt[k]].sbase;
+    lo[k] = bo[base + blo[k]];
+    hi[k] = bo[base + bhi[k]];
+  }
+  uint32_t len = 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t ln = on[k] ? hi[k] - lo[k] : 0u;
+    seg_start[3 * t + k] = on[k] ? sb[k] + lo[k] : 0u;
     seg_len[3 * t + k] = ln;
     len += ln;
   }
@@ -164,8 +246,9 @@ KG_HD uint32_t rt_extra_count(const RepDev& rp, const RtTile& rt, const RtDev& D
     return 0;
   }
   uint32_t n = D.icnt[rt.ph_in][tt];
-  if (n > D.icap[tt]) n = D.icap[tt];
-  *base = D.ibase[tt];
+  const uint32_t icap = D.meta[tt].icap, ibase = D.meta[tt].ibase;
+  if (n > icap) n = icap;
+  *base = ibase;
   return n;
 }
 
@@ -180,45 +263,64 @@ KG_HD bool rt_own(const RtTile& rt, int32_t cx, int32_t cy) {
   return cx >= rt.c0 && cx < rt.c1 && cy >= rt.y0 && cy < rt.y1;
 }
 
-// LDS of the tile path.  Phase A (query) and phase B (output) overlay.
+// LDS of the tile path: results and the output lists live through the
+// whole step; the staging arrays are dead once every query has run.
 struct RtSmem {
-  // phase A
-  float *sx, *sy, *shx, *shy;
-  uint32_t* sid;
-  int32_t* scy;
-  uint16_t* scol;
-  float *ex, *ey, *ehx, *ehy;
-  uint32_t* eid;
-  int32_t* ecy;
-  uint16_t* ecol;
-  uint16_t* eperm;
-  uint16_t* own;  // own agents: staged index, or 0x8000 | extra index
-  // phase B
-  float *ox, *oy, *ohx, *ohy;
-  uint32_t* oid;
+  // results, one per own agent
+  float *rx, *ry, *rhx, *rhy;
+  uint32_t* rid;
+  // output: stayers in classification order, bin histogram and starts
   int32_t* ocy;
-  uint16_t* obin;
-  uint16_t* operm;
+  uint32_t* oid;
   uint32_t* hist;   // nb + 1
   uint32_t* bbase;  // nb + 1
   uint32_t* bcur;   // nb
+  uint16_t* obin;
+  uint16_t* osrc;   // result slot of each stayer
+  uint16_t* operm;
+  // staging
+  float *sx, *sy, *shx, *shy;
+  uint32_t* sid;
+  int32_t* scy;
+  float *ex, *ey, *ehx, *ehy;
+  uint32_t* eid;
+  int32_t* ecy;
+  uint32_t* own;     // own agents: (column << 16) | staged index, or | 0x8000 | extra slot
+  uint16_t* ecol;
+  uint16_t* eperm;   // sorted position -> extra slot
+  uint16_t* bintab;  // per staged column: start of each staged bin, relative to the column
+  uint16_t* ecbase;  // per staged column + 1: first sorted extra of the column
 };
 
-KG_HD size_t rt_smem_a(const RtCaps& c) {
-  return (size_t)c.runs * 26 + (size_t)c.extras * 30 + (size_t)c.own * 2;
-}
-KG_HD size_t rt_smem_b(const RtCaps& c) {
-  return (size_t)c.own * 30 + (size_t)(3 * c.nb + 2) * 4;
+KG_HD size_t rt_smem_res(const RtCaps& c) {
+  return (size_t)c.own * 34 + (size_t)(3 * c.nb + 2) * 4;
 }
 KG_HD size_t rt_smem_bytes(const RtCaps& c) {
-  const size_t a = rt_smem_a(c), b = rt_smem_b(c);
-  return ((a > b ? a : b) + 15) & ~(size_t)15;
+  size_t n = rt_smem_res(c);
+  n = (n + 15) & ~(size_t)15;
+  n += (size_t)c.runs * 24 + (size_t)c.extras * 28 + (size_t)c.own * 4 +
+       (size_t)c.cols * (size_t)c.nbc * 2 + (size_t)(c.cols + 2) * 2;
+  return (n + 15) & ~(size_t)15;
 }
 
 KG_HD RtSmem rt_carve(uint8_t* p, const RtCaps& c) {
   RtSmem s;
-  // phase A: 4-byte arrays first, then 2-byte arrays
   float* f = (float*)p;
+  s.rx = f; f += c.own;
+  s.ry = f; f += c.own;
+  s.rhx = f; f += c.own;
+  s.rhy = f; f += c.own;
+  s.rid = (uint32_t*)f; f += c.own;
+  s.ocy = (int32_t*)f; f += c.own;
+  s.oid = (uint32_t*)f; f += c.own;
+  s.hist = (uint32_t*)f; f += c.nb + 1;
+  s.bbase = (uint32_t*)f; f += c.nb + 1;
+  s.bcur = (uint32_t*)f; f += c.nb;
+  uint16_t* h = (uint16_t*)f;
+  s.obin = h; h += c.own;
+  s.osrc = h; h += c.own;
+  s.operm = h; h += c.own;
+  f = (float*)(p + ((rt_smem_res(c) + 15) & ~(size_t)15));
   s.sx = f; f += c.runs;
   s.sy = f; f += c.runs;
   s.shx = f; f += c.runs;
@@ -231,32 +333,20 @@ KG_HD RtSmem rt_carve(uint8_t* p, const RtCaps& c) {
   s.ehy = f; f += c.extras;
   s.eid = (uint32_t*)f; f += c.extras;
   s.ecy = (int32_t*)f; f += c.extras;
-  uint16_t* h = (uint16_t*)f;
-  s.scol = h; h += c.runs;
+  s.own = (uint32_t*)f; f += c.own;
+  h = (uint16_t*)f;
   s.ecol = h; h += c.extras;
   s.eperm = h; h += c.extras;
-  s.own = h;
-  // phase B
-  f = (float*)p;
-  s.ox = f; f += c.own;
-  s.oy = f; f += c.own;
-  s.ohx = f; f += c.own;
-  s.ohy = f; f += c.own;
-  s.oid = (uint32_t*)f; f += c.own;
-  s.ocy = (int32_t*)f; f += c.own;
-  s.hist = (uint32_t*)f; f += c.nb + 1;
-  s.bbase = (uint32_t*)f; f += c.nb + 1;
-  s.bcur = (uint32_t*)f; f += c.nb;
-  h = (uint16_t*)f;
-  s.obin = h; h += c.own;
-  s.operm = h;
+  s.bintab = h; h += (size_t)c.cols * (size_t)c.nbc;
+  s.ecbase = h;
   return s;
 }
 
-// Stages run item k into LDS.
-KG_HD void rt_stage(const RepDev& rp, const RtTile& rt, const RtDev& D, uint32_t k,
-                    const uint32_t* seg_start, const uint32_t* seg_len, const uint32_t* col_base,
-                    RtSmem S) {
+// Global slot, staged column and segment (0 above, 1 own tile, 2 below) of
+// run item k.
+KG_HD uint32_t rt_stage_src(const RtTile& rt, uint32_t k, const uint32_t* seg_start,
+                            const uint32_t* seg_len, const uint32_t* col_base, int* col,
+                            int* seg_out) {
   int lo = 0, hi = rt.ncols;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -269,30 +359,54 @@ KG_HD void rt_stage(const RepDev& rp, const RtTile& rt, const RtDev& D, uint32_t
     off -= seg_len[3 * lo + seg];
     ++seg;
   }
-  const Buf& B = D.stay[rt.pin];
-  const uint32_t gi = seg_start[3 * lo + seg] + off;
-  const float y = B.y[gi];
-  S.sx[k] = B.x[gi];
-  S.sy[k] = y;
-  S.shx[k] = B.hx[gi];
-  S.shy[k] = B.hy[gi];
-  S.sid[k] = B.id[gi];
-  S.scy[k] = f2i(floorf(y / rp.g.d));
-  S.scol[k] = (uint16_t)lo;
+  *col = lo;
+  *seg_out = seg;
+  return seg_start[3 * lo + seg] + off;
 }
 
-// Reads extra number j (flattened over the 10 sources) and, if it is in the
-// staged rectangle, appends it to the LDS extras.  Returns false if the
-// extras capacity is exceeded.
-KG_HD bool rt_gather_extra(const RepDev& rp, const RtTile& rt, const RtDev& D, uint32_t j,
-                           const uint32_t* src_base, const uint32_t* src_pre, int cap_e,
-                           uint32_t* ne, RtSmem S) {
-  int k = 0;
-  while (j >= src_pre[k + 1]) ++k;
-  const uint32_t gi = src_base[k] + (j - src_pre[k]);
-  const Buf& B = (k == 9) ? D.spill[rt.pin] : D.inbox[rt.pin];
-  if (k == 9 && D.spill_rep[rt.pin][gi] != (uint32_t)rt.r) return true;
-  const float x = B.x[gi], y = B.y[gi];
+// Writes staged item k; own-tile items of own columns join the own list.
+KG_HD void rt_stage_put(const RepDev& rp, const RtTile& rt, uint32_t k, int col, int seg, float x,
+                        float y, float hx, float hy, uint32_t id, int cap_own, uint32_t* nown,
+                        RtSmem S) {
+  S.sx[k] = x;
+  S.sy[k] = y;
+  S.shx[k] = hx;
+  S.shy[k] = hy;
+  S.sid[k] = id;
+  S.scy[k] = f2i(floorf(y / rp.g.d));
+  const int32_t cx = rt.sc_lo + col;
+  if (seg == 1 && cx >= rt.c0 && cx < rt.c1) {
+    const uint32_t o = atom_add(nown, 1u);
+    if (o < (uint32_t)cap_own) S.own[o] = ((uint32_t)col << 16) | k;
+  }
+}
+
+// Stages run item k into LDS.
+KG_HD void rt_stage(const RepDev& rp, const RtTile& rt, const RtDev& D, uint32_t k,
+                    const uint32_t* seg_start, const uint32_t* seg_len, const uint32_t* col_base,
+                    int cap_own, uint32_t* nown, RtSmem S) {
+  int col, seg;
+  const uint32_t gi = rt_stage_src(rt, k, seg_start, seg_len, col_base, &col, &seg);
+  const Buf& B = D.stay[rt.pin];
+  rt_stage_put(rp, rt, k, col, seg, B.x[gi], B.y[gi], B.hx[gi], B.hy[gi], B.id[gi], cap_own, nown,
+               S);
+}
+
+// Source k (0..9) and global slot of extra number j (flattened over the 10
+// sources).
+KG_HD uint32_t rt_extra_src(uint32_t j, const uint32_t* src_base, const uint32_t* src_pre, int* k) {
+  int q = 0;
+  while (j >= src_pre[q + 1]) ++q;
+  *k = q;
+  return src_base[q] + (j - src_pre[q]);
+}
+
+// Appends an extra to the LDS extras if it is in the staged rectangle (and
+// to the own list if in an own cell).  Returns false if the extras capacity
+// is exceeded.
+KG_HD bool rt_extra_put(const RepDev& rp, const RtTile& rt, float x, float y, float hx, float hy,
+                        uint32_t id, int cap_e, uint32_t* ne, int cap_own, uint32_t* nown,
+                        RtSmem S) {
   int32_t cx, cy;
   discretize(rp.g, x, y, &cx, &cy);
   if (!rt_keep(rt, cx, cy)) return true;
@@ -300,12 +414,96 @@ KG_HD bool rt_gather_extra(const RepDev& rp, const RtTile& rt, const RtDev& D, u
   if (slot >= (uint32_t)cap_e) return false;
   S.ex[slot] = x;
   S.ey[slot] = y;
-  S.ehx[slot] = B.hx[gi];
-  S.ehy[slot] = B.hy[gi];
-  S.eid[slot] = B.id[gi];
+  S.ehx[slot] = hx;
+  S.ehy[slot] = hy;
+  S.eid[slot] = id;
   S.ecy[slot] = cy;
   S.ecol[slot] = (uint16_t)(cx - rt.sc_lo);
+  if (rt_own(rt, cx, cy)) {
+    const uint32_t o = atom_add(nown, 1u);
+    if (o < (uint32_t)cap_own) S.own[o] = ((uint32_t)(cx - rt.sc_lo) << 16) | 0x8000u | slot;
+  }
   return true;
+}
+
+// Reads extra number j and, if it is in the staged rectangle, appends it to
+// the LDS extras.  Returns false if the extras capacity is exceeded.
+KG_HD bool rt_gather_extra(const RepDev& rp, const RtTile& rt, const RtDev& D, uint32_t j,
+                           const uint32_t* src_base, const uint32_t* src_pre, int cap_e,
+                           uint32_t* ne, int cap_own, uint32_t* nown, RtSmem S) {
+  int k;
+  const uint32_t gi = rt_extra_src(j, src_base, src_pre, &k);
+  const Buf& B = (k == 9) ? D.spill[rt.pin] : D.inbox[rt.pin];
+  if (k == 9 && D.spill_rep[rt.pin][gi] != (uint32_t)rt.r) return true;
+  return rt_extra_put(rp, rt, B.x[gi], B.y[gi], B.hx[gi], B.hy[gi], B.id[gi], cap_e, ne, cap_own,
+                      nown, S);
+}
+
+// Run items k0 and k1 and extra j (each skipped when past its end) with all
+// global loads issued before any LDS write.  Returns false if the extras
+// capacity is exceeded.
+KG_HD bool rt_fetch(const RepDev& rp, const RtTile& rt, const RtDev& D, uint32_t k0, uint32_t k1,
+                    uint32_t total, uint32_t j, uint32_t next, const uint32_t* seg_start,
+                    const uint32_t* seg_len, const uint32_t* col_base, const uint32_t* src_base,
+                    const uint32_t* src_pre, const RtCaps& caps, uint32_t* ne, uint32_t* nown,
+                    RtSmem S) {
+  const bool h0 = k0 < total, h1 = k1 < total, he = j < next;
+  int c0 = 0, c1 = 0, s0 = 0, s1 = 0, ek = 0;
+  uint32_t g0 = 0, g1 = 0, ge = 0;
+  if (h0) g0 = rt_stage_src(rt, k0, seg_start, seg_len, col_base, &c0, &s0);
+  if (h1) g1 = rt_stage_src(rt, k1, seg_start, seg_len, col_base, &c1, &s1);
+  if (he) ge = rt_extra_src(j, src_base, src_pre, &ek);
+  const Buf& B = D.stay[rt.pin];
+  const Buf& E = (ek == 9) ? D.spill[rt.pin] : D.inbox[rt.pin];
+  float x0 = 0, y0 = 0, hx0 = 0, hy0 = 0, x1 = 0, y1 = 0, hx1 = 0, hy1 = 0;
+  float xe = 0, ye = 0, hxe = 0, hye = 0;
+  uint32_t id0 = 0, id1 = 0, ide = 0, repe = (uint32_t)rt.r;
+  if (h0) {
+    x0 = B.x[g0]; y0 = B.y[g0]; hx0 = B.hx[g0]; hy0 = B.hy[g0]; id0 = B.id[g0];
+  }
+  if (h1) {
+    x1 = B.x[g1]; y1 = B.y[g1]; hx1 = B.hx[g1]; hy1 = B.hy[g1]; id1 = B.id[g1];
+  }
+  if (he) {
+    xe = E.x[ge]; ye = E.y[ge]; hxe = E.hx[ge]; hye = E.hy[ge]; ide = E.id[ge];
+    if (ek == 9) repe = D.spill_rep[rt.pin][ge];
+  }
+  if (h0) rt_stage_put(rp, rt, k0, c0, s0, x0, y0, hx0, hy0, id0, caps.own, nown, S);
+  if (h1) rt_stage_put(rp, rt, k1, c1, s1, x1, y1, hx1, hy1, id1, caps.own, nown, S);
+  if (he && repe == (uint32_t)rt.r)
+    return rt_extra_put(rp, rt, xe, ye, hxe, hye, ide, caps.extras, ne, caps.own, nown, S);
+  return true;
+}
+
+// Bin table entry p (column p / nbc, entry p % nbc): the start of staged
+// bin x of the column relative to the column's first staged item; entry
+// nbc - 1 is the column's length.  From the bin offsets of the segment's tile
+// and the lengths of the segments before it.
+KG_HD void rt_bintab(const RepDev& rp, const RtTile& rt, const RtDev& D, const RtSegs& sg,
+                     uint32_t p, const uint32_t* seg_len, RtSmem S) {
+  const Geom& g = rp.g;
+  const int c = (int)(p / (uint32_t)sg.nbc);
+  const int32_t x = (int32_t)(p % (uint32_t)sg.nbc);
+  const uint32_t* len = seg_len + 3 * c;
+  if (x == sg.nbc - 1) {
+    S.bintab[p] = (uint16_t)(len[0] + len[1] + len[2]);
+    return;
+  }
+  // x < the staged bins, so k <= 2
+  const int k = x < sg.pre[0] + sg.nbin[0] ? 0 : (x < sg.pre[1] + sg.nbin[1] ? 1 : 2);
+  const uint32_t v0 = k > 0 ? len[0] : 0u, v1 = k > 1 ? len[1] : 0u;
+  uint32_t v = v0 + v1;
+  const int32_t b = x - rt_pick3(sg.pre, k);
+  if (b > 0) {
+    const int32_t col = rt.sc_lo + c;
+    const int32_t ttx = col >> g.tx_log2;
+    const int32_t lc = col & (g.tx - 1);
+    const uint32_t tt = rt.tbase + (uint32_t)((rt.tcy - 1 + k) * g.ntx + ttx);
+    const uint32_t* bo = D.boff[rt.pin] + rt_boff_at(rp, rt, tt);
+    const uint32_t b0 = ((uint32_t)lc << g.tr_log2) + (uint32_t)rt_pick3(sg.blo, k);
+    v += bo[b0 + (uint32_t)b] - bo[b0];
+  }
+  S.bintab[p] = (uint16_t)v;
 }
 
 // Sorts extra j among all extras by (column, cell y, id, slot).
@@ -323,79 +521,115 @@ KG_HD void rt_rank_extra(uint32_t j, uint32_t ne, RtSmem S) {
   S.eperm[rank] = (uint16_t)j;
 }
 
-// Start of column c in the sorted extras.
-KG_HD uint32_t rt_extra_col_base(int c, uint32_t ne, RtSmem S) {
+// Start of column c among the sorted extras: the extras of lower columns.
+KG_HD void rt_ecbase(int c, uint32_t ne, RtSmem S) {
   uint32_t n = 0;
   for (uint32_t q = 0; q < ne; ++q) n += S.ecol[q] < (uint32_t)c;
-  return n;
+  S.ecbase[c] = (uint16_t)n;
 }
 
-// Bird::step for one own agent with neighbours merged from the staged runs
-// and the sorted extras.
-KG_HD void rt_agent_lds(const RepDev& rp, const RtTile& rt, uint16_t ref, const uint32_t* col_base,
-                        const uint32_t* ecol_base, RtSmem S, float* nx, float* ny, float* nhx,
-                        float* nhy, uint32_t* nid) {
+// Squared toroidal distance of one corner coordinate to a query coordinate,
+// the per-axis term of field_2d.rs:974-986 (dist2 forms dx*dx + dy*dy).
+KG_HD float rt_corner_sq(float corner, float q, float dim) {
+  const float d = toroidal_distance(corner, q, dim);
+  return d * d;
+}
+
+// check_circle (field_2d.rs:934-972) from the four per-axis squared corner
+// distances; the sums are the ones dist2 forms.
+KG_HD int rt_check_from_sq(float sx_nw, float sx_sw, float sy_nw, float sy_ne, float T) {
+  const float s_nw = sx_nw + sy_nw, s_ne = sx_nw + sy_ne;
+  const float s_sw = sx_sw + sy_nw, s_se = sx_sw + sy_ne;
+  if (s_nw <= T && s_ne <= T && s_sw <= T && s_se <= T) return 1;
+  if (s_nw > T && s_ne > T && s_sw > T && s_se > T) return -1;
+  return 0;
+}
+
+// Bird::step for own agent number o: neighbours come from the staged runs
+// (the row table gives the window rows' range of each column) merged with
+// the column's sorted extras; the new state goes to result slot o.  For
+// windows of up to three rows the corner terms of check_circle are computed
+// once per agent (rows) and once per column instead of once per cell.
+KG_HD void rt_agent_lds(const RepDev& rp, const RtTile& rt, const RtSegs& sg, uint32_t o,
+                        const uint32_t* col_base, RtSmem S) {
   const Geom& g = rp.g;
   const float T = rp.sq_thresh;
   const int32_t dd = rp.disc_dist;
+  const uint32_t ref = S.own[o];
+  const uint32_t j0 = ref & 0x7FFFu;
   float qx, qy, qhx, qhy;
   uint32_t qid;
-  int32_t cx, cy;
+  int32_t cy;
+  const int32_t cx = rt.sc_lo + (int32_t)(ref >> 16);
   if (ref & 0x8000u) {
-    const uint32_t j = ref & 0x7FFFu;
-    qx = S.ex[j]; qy = S.ey[j]; qhx = S.ehx[j]; qhy = S.ehy[j]; qid = S.eid[j];
-    cx = rt.sc_lo + S.ecol[j];
-    cy = S.ecy[j];
+    qx = S.ex[j0]; qy = S.ey[j0]; qhx = S.ehx[j0]; qhy = S.ehy[j0]; qid = S.eid[j0];
+    cy = S.ecy[j0];
   } else {
-    qx = S.sx[ref]; qy = S.sy[ref]; qhx = S.shx[ref]; qhy = S.shy[ref]; qid = S.sid[ref];
-    cx = rt.sc_lo + S.scol[ref];
-    cy = S.scy[ref];
+    qx = S.sx[j0]; qy = S.sy[j0]; qhx = S.shx[j0]; qhy = S.shy[j0]; qid = S.sid[j0];
+    cy = S.scy[j0];
   }
   const int32_t min_i = imax(0, cx - dd), max_i = imin(cx + dd, g.max_x - 1);
   const int32_t min_j = imax(0, cy - dd), max_j = imin(cy + dd, g.max_y - 1);
+  const int32_t x_lo = rt_seg_bin(g, rt, sg, min_j), x_hi = rt_seg_bin(g, rt, sg, max_j) + 1;
+  // corner rows of the window's cells: nw_y = j * d, ne_y = min(nw_y + d, h)
+  const bool pre = max_j - min_j < 3;
+  float syn[3], sye[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float nwy = (float)(min_j + k) * g.d;
+    const float ney = fminf(nwy + g.d, g.h);
+    syn[k] = pre ? rt_corner_sq(nwy, qy, g.h) : 0.0f;
+    sye[k] = pre ? rt_corner_sq(ney, qy, g.h) : 0.0f;
+  }
   FlockAcc acc;
   for (int32_t i = min_i; i <= max_i; ++i) {
     const int c = i - rt.sc_lo;
-    // runs: first slot with cell y >= min_j
-    uint32_t a = col_base[c], a_end = col_base[c + 1];
-    {
-      uint32_t lo = a, hi = a_end;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (S.scy[mid] < min_j) lo = mid + 1;
-        else hi = mid;
-      }
-      a = lo;
-    }
-    uint32_t b = ecol_base[c], b_end = ecol_base[c + 1];
+    const uint16_t* tab = S.bintab + (size_t)c * (size_t)sg.nbc;
+    const uint32_t base = col_base[c];
+    uint32_t a = base + tab[x_lo], a_end = base + tab[x_hi];
+    while (a < a_end && S.scy[a] < min_j) ++a;  // rows of min_j's bin below the window
+    uint32_t b = S.ecbase[c], b_end = S.ecbase[c + 1];
     while (b < b_end && S.ecy[S.eperm[b]] < min_j) ++b;
+    // corner columns: nw_x = i * d, sw_x = min(nw_x + d, w)
+    const float nwx = (float)i * g.d;
+    const float swx = fminf(nwx + g.d, g.w);
+    const float sxn = rt_corner_sq(nwx, qx, g.w), sxs = rt_corner_sq(swx, qx, g.w);
     int32_t cur = -1;
     int chk = 0;
     for (;;) {
       const bool ha = a < a_end && S.scy[a] <= max_j;
-      const bool hb = b < b_end && S.ecy[S.eperm[b]] <= max_j;
+      uint32_t e = 0;
+      bool hb = false;
+      if (b < b_end) {
+        e = S.eperm[b];
+        hb = S.ecy[e] <= max_j;
+      }
       if (!ha && !hb) break;
       bool take_a = ha;
-      if (ha && hb) {
-        const uint32_t e = S.eperm[b];
+      if (ha && hb)
         take_a = S.scy[a] < S.ecy[e] || (S.scy[a] == S.ecy[e] && S.sid[a] <= S.eid[e]);
-      }
-      float ex_, ey_, ehx_, ehy_;
-      uint32_t eid_;
-      int32_t ecy_;
-      if (take_a) {
-        ex_ = S.sx[a]; ey_ = S.sy[a]; ehx_ = S.shx[a]; ehy_ = S.shy[a]; eid_ = S.sid[a];
-        ecy_ = S.scy[a];
-        ++a;
-      } else {
-        const uint32_t e = S.eperm[b];
-        ex_ = S.ex[e]; ey_ = S.ey[e]; ehx_ = S.ehx[e]; ehy_ = S.ehy[e]; eid_ = S.eid[e];
-        ecy_ = S.ecy[e];
-        ++b;
-      }
+      const uint32_t src = take_a ? a : e;
+      const float* px = take_a ? S.sx : S.ex;
+      const float* py = take_a ? S.sy : S.ey;
+      const float* phx = take_a ? S.shx : S.ehx;
+      const float* phy = take_a ? S.shy : S.ehy;
+      const uint32_t* pid = take_a ? S.sid : S.eid;
+      const int32_t* pcy = take_a ? S.scy : S.ecy;
+      const float ex_ = px[src], ey_ = py[src], ehx_ = phx[src], ehy_ = phy[src];
+      const uint32_t eid_ = pid[src];
+      const int32_t ecy_ = pcy[src];
+      a += take_a ? 1u : 0u;
+      b += take_a ? 0u : 1u;
       if (ecy_ != cur) {
         cur = ecy_;
-        chk = check_circle_sq(g, i, ecy_, qx, qy, T);
+        if (pre) {
+          const int k = ecy_ - min_j;
+          const float yn = k == 0 ? syn[0] : (k == 1 ? syn[1] : syn[2]);
+          const float ye = k == 0 ? sye[0] : (k == 1 ? sye[1] : sye[2]);
+          chk = rt_check_from_sq(sxn, sxs, yn, ye, T);
+        } else {
+          chk = check_circle_sq(g, i, ecy_, qx, qy, T);
+        }
       }
       if (chk == -1) continue;
       const float dx = toroidal_distance(qx, ex_, g.w);
@@ -403,18 +637,24 @@ KG_HD void rt_agent_lds(const RepDev& rp, const RtTile& rt, uint16_t ref, const 
       if (chk == 1 || dx * dx + dy * dy <= T) acc.add(eid_ == qid, dx, dy, ehx_, ehy_);
     }
   }
-  flock_finish(acc, rp.p, qid, rt.step, qx, qy, qhx, qhy, g.w, nx, ny, nhx, nhy);
-  *nid = qid;
+  float nx, ny, nhx, nhy;
+  flock_finish(acc, rp.p, qid, rt.step, qx, qy, qhx, qhy, g.w, &nx, &ny, &nhx, &nhy);
+  S.rx[o] = nx;
+  S.ry[o] = ny;
+  S.rhx[o] = nhx;
+  S.rhy[o] = nhy;
+  S.rid[o] = qid;
 }
 
 // Sends one agent to its destination tile's inbox (or the spill list).
 KG_HD void rt_push(const RepDev& rp, const RtTile& rt, const RtDev& D, uint32_t dest, float x,
                    float y, float hx, float hy, uint32_t id) {
   const int pout = rt.pin ^ 1;
+  const uint32_t icap = D.meta[dest].icap, ibase = D.meta[dest].ibase;  // loaded with the atomic
   const uint32_t slot = atom_add(&D.icnt[rt.ph_out][dest], 1u);
-  if (slot < D.icap[dest]) {
+  if (slot < icap) {
     const Buf& B = D.inbox[pout];
-    const uint32_t gi = D.ibase[dest] + slot;
+    const uint32_t gi = ibase + slot;
     B.x[gi] = x; B.y[gi] = y; B.hx[gi] = hx; B.hy[gi] = hy; B.id[gi] = id;
     return;
   }
@@ -428,37 +668,39 @@ KG_HD void rt_push(const RepDev& rp, const RtTile& rt, const RtDev& D, uint32_t 
   (void)rp;
 }
 
-// Classifies a new state: a stayer goes to the LDS output list (returns its
-// slot), a mover is pushed at once (returns -1).  Out-of-grid agents are
-// dropped and flagged.
-KG_HD int64_t rt_classify(const RepDev& rp, const RtTile& rt, const RtDev& D, bool all_movers,
-                          float x, float y, float hx, float hy, uint32_t id, uint32_t* nstay,
-                          RtSmem S) {
+// Classifies result o: a stayer joins the LDS output list and the bin
+// histogram, a mover is pushed at once.  Out-of-grid agents are dropped and
+// flagged.
+KG_HD void rt_classify(const RepDev& rp, const RtTile& rt, const RtDev& D, uint32_t o,
+                       uint32_t* nstay, RtSmem S) {
   const Geom& g = rp.g;
+  const float x = S.rx[o], y = S.ry[o];
   int32_t cx, cy;
   discretize(g, x, y, &cx, &cy);
   if (!cell_valid(g, cx, cy)) {
     atom_or(D.flags, kRtFlagRange);
-    return -1;
+    return;
   }
   const uint32_t dest = rt.tbase + tile_of(g, cx, cy);
-  if (dest != rt.gt || all_movers) {
-    rt_push(rp, rt, D, dest, x, y, hx, hy, id);
-    return -1;
+  if (dest != rt.gt) {
+    rt_push(rp, rt, D, dest, x, y, S.rhx[o], S.rhy[o], S.rid[o]);
+    return;
   }
   const uint32_t slot = atom_add(nstay, 1u);
-  S.ox[slot] = x; S.oy[slot] = y; S.ohx[slot] = hx; S.ohy[slot] = hy; S.oid[slot] = id;
+  const uint32_t bin = ((uint32_t)(cx & (g.tx - 1)) << g.tr_log2) +
+                       (uint32_t)((cy & (g.ty - 1)) >> g.hb_log2);
   S.ocy[slot] = cy;
-  S.obin[slot] = (uint16_t)(((uint32_t)(cx & (g.tx - 1)) << g.tr_log2) +
-                            (uint32_t)((cy & (g.ty - 1)) >> g.hb_log2));
-  return slot;
+  S.oid[slot] = S.rid[o];
+  S.osrc[slot] = (uint16_t)o;
+  S.obin[slot] = (uint16_t)bin;
+  atom_add(&S.hist[bin], 1u);
 }
 
 // Places stayer s inside its bin by (cell y, id, slot) and writes it out.
 // Stayers sorted past the tile's capacity go to its own inbox instead (the
 // tile keeps the first scap of its sorted stayers; boff is clipped).
-KG_HD void rt_write_stayer(const RepDev& rp, const RtTile& rt, const RtDev& D, uint32_t s,
-                           RtSmem S) {
+KG_HD void rt_write_stayer(const RepDev& rp, const RtTile& rt, const RtDev& D, RtMeta me,
+                           uint32_t s, RtSmem S) {
   const uint32_t b = S.obin[s];
   const uint32_t lo = S.bbase[b], hi = S.bbase[b + 1];
   const int32_t cy = S.ocy[s];
@@ -471,14 +713,15 @@ KG_HD void rt_write_stayer(const RepDev& rp, const RtTile& rt, const RtDev& D, u
     rank += (qy < cy) || (qy == cy && (qid < id || (qid == id && o < s)));
   }
   const uint32_t pos = lo + rank;
-  if (pos >= D.scap[rt.gt]) {
+  const uint32_t r = S.osrc[s];
+  if (pos >= me.scap) {
     atom_or(D.flags, kRtFlagStayCap);
-    rt_push(rp, rt, D, rt.gt, S.ox[s], S.oy[s], S.ohx[s], S.ohy[s], id);
+    rt_push(rp, rt, D, rt.gt, S.rx[r], S.ry[r], S.rhx[r], S.rhy[r], id);
     return;
   }
   const Buf& B = D.stay[rt.pin ^ 1];
-  const uint32_t gi = D.sbase[rt.gt] + pos;
-  B.x[gi] = S.ox[s]; B.y[gi] = S.oy[s]; B.hx[gi] = S.ohx[s]; B.hy[gi] = S.ohy[s]; B.id[gi] = id;
+  const uint32_t gi = me.sbase + pos;
+  B.x[gi] = S.rx[r]; B.y[gi] = S.ry[r]; B.hx[gi] = S.rhx[r]; B.hy[gi] = S.rhy[r]; B.id[gi] = id;
 }
 
 // ---- global path (tiles beyond the LDS capacities) --------------------------
@@ -509,10 +752,11 @@ KG_HD void rt_query_global(const RepDev& rp, const RtTile& rt, const RtDev& D, f
       const int64_t tt = rt_tile_at(g, rt, ttx, tty);
       uint32_t a = 0, a_end = 0;
       if (tt >= 0) {
-        const uint32_t* bo = D.boff[rt.pin] + D.tboff[tt];
+        const uint32_t* bo = D.boff[rt.pin] + rt_boff_at(rp, rt, (uint32_t)tt);
         const uint32_t b = ((uint32_t)lc << g.tr_log2) + (uint32_t)((j - tty * g.ty) >> g.hb_log2);
-        a = D.sbase[tt] + bo[b];
-        a_end = D.sbase[tt] + bo[b + 1];
+        const uint32_t sb = D.meta[tt].sbase;
+        a = sb + bo[b];
+        a_end = sb + bo[b + 1];
         while (a < a_end && f2i(floorf(SB.y[a] / g.d)) < j) ++a;
       }
       // extras of cell (i, j) in ascending (id, source, index) by selection
@@ -570,7 +814,7 @@ KG_HD bool rt_own_global(const RepDev& rp, const RtTile& rt, const RtDev& D, uin
   const uint32_t ns = D.cnt[rt.pin][rt.gt];
   if (o < ns) {
     const Buf& B = D.stay[rt.pin];
-    const uint32_t gi = D.sbase[rt.gt] + o;
+    const uint32_t gi = D.meta[rt.gt].sbase + o;
     *x = B.x[gi]; *y = B.y[gi]; *hx = B.hx[gi]; *hy = B.hy[gi]; *id = B.id[gi];
     return true;
   }
@@ -605,6 +849,29 @@ KG_HD void rt_agent_global(const RepDev& rp, const RtTile& rt, const RtDev& D, f
     acc.add(eid == qid, dx, dy, ehx, ehy);
   });
   flock_finish(acc, rp.p, qid, rt.step, qx, qy, qhx, qhy, rp.g.w, nx, ny, nhx, nhy);
+}
+
+// Global path of a whole tile, thread t of nthreads: every own agent is
+// stepped from global memory and pushed to its destination's inbox (its own
+// tile's inbox for stayers).
+KG_HD void rt_tile_global(const RepDev& rp, const RtTile& rt, const RtDev& D, uint32_t t,
+                          uint32_t nthreads) {
+  for (uint32_t o = t;; o += nthreads) {
+    float x, y, hx, hy;
+    uint32_t id;
+    bool skip;
+    if (!rt_own_global(rp, rt, D, o, &x, &y, &hx, &hy, &id, &skip)) break;
+    if (skip) continue;
+    float ax, ay, ahx, ahy;
+    rt_agent_global(rp, rt, D, x, y, hx, hy, id, &ax, &ay, &ahx, &ahy);
+    int32_t cx, cy;
+    discretize(rp.g, ax, ay, &cx, &cy);
+    if (!cell_valid(rp.g, cx, cy)) {
+      atom_or(D.flags, kRtFlagRange);
+      continue;
+    }
+    rt_push(rp, rt, D, rt.tbase + tile_of(rp.g, cx, cy), ax, ay, ahx, ahy, id);
+  }
 }
 
 }  // namespace kg

@@ -5,11 +5,7 @@ namespace kg {
 
 RtDev RtState::dev() const {
   RtDev D;
-  D.sbase = sbase;
-  D.scap = scap;
-  D.ibase = ibase;
-  D.icap = icap;
-  D.tboff = tboff;
+  D.meta = meta;
   for (int p = 0; p < 2; ++p) {
     D.cnt[p] = cnt[p];
     D.boff[p] = boff[p];
@@ -24,8 +20,9 @@ RtDev RtState::dev() const {
   }
   D.spill_cap = (uint32_t)spill_alloc;
   D.flags = flags;
-  D.done = done;
   D.gstep = gstep;
+  D.ovf[0] = ovf;
+  D.ovf[1] = ovf + (kRtOvfMax + 1);
   return D;
 }
 
@@ -36,8 +33,10 @@ void RtState::drop_graphs() {
 
 void RtState::release() {
   drop_graphs();
-  for (uint32_t* p : {sbase, scap, ibase, icap, tboff, cnt[0], cnt[1], boff[0], boff[1], icnt[0],
-                      icnt[1], icnt[2], scnt, spill_rep[0], spill_rep[1], flags, done})
+  (void)hipFree(meta);
+  (void)hipFree(dense);
+  for (uint32_t* p : {cnt[0], cnt[1], boff[0], boff[1], icnt[0], icnt[1], icnt[2], scnt,
+                      spill_rep[0], spill_rep[1], flags, ovf})
     (void)hipFree(p);
   (void)hipFree(gstep);
   for (int p = 0; p < 2; ++p) {
@@ -48,7 +47,7 @@ void RtState::release() {
   *this = RtState();
 }
 
-int RtState::setup(const RtPlan& pl, uint64_t spill_cap, hipStream_t s) {
+int RtState::setup(const RtPlan& pl, const RtClassPlan& cls, uint64_t spill_cap, hipStream_t s) {
   const uint64_t nt = pl.sbase.size();
   drop_graphs();
   if (!allocated || pl.stay_total > stay_alloc || pl.inbox_total > inbox_alloc ||
@@ -64,11 +63,8 @@ int RtState::setup(const RtPlan& pl, uint64_t spill_cap, hipStream_t s) {
     auto ok = [&](hipError_t r) {
       if (e == hipSuccess) e = r;
     };
-    ok(dmalloc(&sbase, nt));
-    ok(dmalloc(&scap, nt));
-    ok(dmalloc(&ibase, nt));
-    ok(dmalloc(&icap, nt));
-    ok(dmalloc(&tboff, nt));
+    ok(dmalloc(&meta, nt));
+    ok(dmalloc(&dense, nt));
     for (int p = 0; p < 2; ++p) {
       ok(dmalloc(&cnt[p], nt));
       ok(dmalloc(&boff[p], boff_alloc));
@@ -80,7 +76,7 @@ int RtState::setup(const RtPlan& pl, uint64_t spill_cap, hipStream_t s) {
     for (int k = 0; k < 3; ++k) ok(dmalloc(&icnt[k], nt));
     ok(dmalloc(&scnt, 3));
     ok(dmalloc(&flags, 1));
-    ok(dmalloc(&done, 1));
+    ok(dmalloc(&ovf, 2 * (kRtOvfMax + 1)));
     ok(dmalloc(&gstep, 1));
     if (e != hipSuccess) {
       release();
@@ -91,12 +87,14 @@ int RtState::setup(const RtPlan& pl, uint64_t spill_cap, hipStream_t s) {
   }
   plan = pl;
   ntiles = (uint32_t)nt;
-  const size_t bytes = nt * sizeof(uint32_t);
-  KG_HIP(hipMemcpyAsync(sbase, plan.sbase.data(), bytes, hipMemcpyHostToDevice, s));
-  KG_HIP(hipMemcpyAsync(scap, plan.scap.data(), bytes, hipMemcpyHostToDevice, s));
-  KG_HIP(hipMemcpyAsync(ibase, plan.ibase.data(), bytes, hipMemcpyHostToDevice, s));
-  KG_HIP(hipMemcpyAsync(icap, plan.icap.data(), bytes, hipMemcpyHostToDevice, s));
-  KG_HIP(hipMemcpyAsync(tboff, plan.tboff.data(), bytes, hipMemcpyHostToDevice, s));
+  KG_HIP(hipMemcpyAsync(meta, plan.meta.data(), nt * sizeof(RtMeta), hipMemcpyHostToDevice, s));
+  dense_host = cls.dense;
+  ndense = (uint32_t)dense_host.size();
+  caps[0] = cls.caps[0];
+  caps[1] = cls.caps[1];
+  if (ndense)
+    KG_HIP(hipMemcpyAsync(dense, dense_host.data(), ndense * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
   // the plan vectors are members and stay unchanged until the copies ran
   KG_HIP(hipStreamSynchronize(s));
   return KG_OK;
@@ -106,7 +104,7 @@ int RtState::reset(uint64_t g, hipStream_t s) {
   for (int k = 0; k < 3; ++k) KG_HIP(launch_fill_u32(icnt[k], 0u, ntiles, s));
   KG_HIP(launch_fill_u32(scnt, 0u, 3, s));
   KG_HIP(launch_fill_u32(flags, 0u, 1, s));
-  KG_HIP(launch_fill_u32(done, 0u, 1, s));
+  KG_HIP(launch_fill_u32(ovf, 0u, 2 * (kRtOvfMax + 1), s));
   KG_HIP(launch_set_u64(gstep, g, s));
   G = g;
   return KG_OK;

@@ -30,24 +30,28 @@ struct DevBuf {
 struct RtState {
   bool allocated = false;
   RtPlan plan;
-  RtCaps caps{};
+  RtCaps caps[2]{};          // normal, dense
+  uint32_t* dense = nullptr;  // device list of dense tiles
+  uint32_t ndense = 0;
+  std::vector<uint32_t> dense_host;  // kept until the upload ran
   uint32_t ntiles = 0;
   uint64_t stay_alloc = 0, inbox_alloc = 0, boff_alloc = 0, spill_alloc = 0, tiles_alloc = 0;
-  uint32_t *sbase = nullptr, *scap = nullptr, *ibase = nullptr, *icap = nullptr, *tboff = nullptr;
+  RtMeta* meta = nullptr;  // per tile
   uint32_t *cnt[2] = {nullptr, nullptr}, *boff[2] = {nullptr, nullptr};
   uint32_t* icnt[3] = {nullptr, nullptr, nullptr};
   uint32_t* scnt = nullptr;  // 3 counters
   DevBuf stay[2], inbox[2], spill[2];
   uint32_t* spill_rep[2] = {nullptr, nullptr};
   uint32_t* flags = nullptr;
-  uint32_t* done = nullptr;
   uint64_t* gstep = nullptr;
+  uint32_t* ovf = nullptr;  // 2 x (kRtOvfMax + 1)
   uint64_t G = 0;  // value of *gstep once queued work has run
   std::vector<std::pair<int, hipGraphExec_t>> graphs;  // (steps, exec)
 
   RtDev dev() const;
-  // (Re)allocates for `plan` (growing only) and uploads the plan tables.
-  int setup(const RtPlan& plan, uint64_t spill_cap, hipStream_t s);
+  // (Re)allocates for `plan` (growing only) and uploads the plan tables and
+  // the class split.
+  int setup(const RtPlan& plan, const RtClassPlan& cls, uint64_t spill_cap, hipStream_t s);
   // Zeroes the counters and flags and sets the step counter to G.
   int reset(uint64_t G, hipStream_t s);
   void drop_graphs();
@@ -55,5 +59,6 @@ struct RtState {
 };
 
 hipError_t launch_set_u64(uint64_t* p, uint64_t v, hipStream_t s);
+hipError_t launch_add_u64(uint64_t* p, uint64_t v, hipStream_t s);
 
 }  // namespace kg

@@ -200,7 +200,7 @@ class Field2D:
 
     STAT_NAMES = ("rt_active", "rt_unsuitable", "rt_plans", "rt_flags", "tiles", "tile_cols",
                   "tile_rows", "bin_rows", "lds_runs", "lds_own", "lds_extras", "lds_bytes",
-                  "lds_limit", "max_tile_cap", "max_inbox_cap", "rt_steps")
+                  "lds_limit", "max_tile_cap", "dense_tiles", "rt_steps")
 
     def stats(self):
         """Layout statistics (see kg_field_stats)."""

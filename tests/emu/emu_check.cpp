@@ -28,6 +28,8 @@ int emu_rt_enter(emu_field* f, int caps_runs, int caps_own, int caps_extras, int
 int emu_rt_steps(emu_field* f, const kg_flock_params* p, uint64_t step0, uint32_t nsteps);
 uint32_t emu_rt_flags(emu_field* f);
 uint32_t emu_rt_read(emu_field* f, uint32_t* ids, float* x, float* y, float* hx, float* hy);
+void emu_rt_counts(emu_field* f, uint64_t* lds, uint64_t* glob, uint64_t* deferred,
+                   uint32_t* ndense);
 }
 
 struct RtScenario {
@@ -38,7 +40,7 @@ struct RtScenario {
   uint64_t seed;
   uint32_t tx, ty, hb;  // 0 = automatic
   int runs, own, extras; // LDS capacities, 0 = planned
-  int tight;            // capacities below the counts
+  int tight;            // bit 0: capacities below the counts; bit 1: class split
   uint64_t shuffle;
   int layout;
   float radius;
@@ -191,8 +193,13 @@ static int run_rt(const RtScenario& s) {
   std::vector<uint32_t> gid(s.n);
   std::vector<float> gx(s.n), gy(s.n), ghx(s.n), ghy(s.n);
   const uint32_t m = emu_rt_read(e, gid.data(), gx.data(), gy.data(), ghx.data(), ghy.data());
-  char extra[96];
-  std::snprintf(extra, sizeof(extra), "rc=%d flags=0x%x", rc, emu_rt_flags(e));
+  char extra[160];
+  uint64_t nl = 0, ng = 0, ndf = 0;
+  uint32_t nde = 0;
+  emu_rt_counts(e, &nl, &ng, &ndf, &nde);
+  std::snprintf(extra, sizeof(extra), "rc=%d flags=0x%x dense=%u lds=%llu global=%llu deferred=%llu",
+                rc, emu_rt_flags(e), nde, (unsigned long long)nl, (unsigned long long)ng,
+                (unsigned long long)ndf);
   const int bad = compare(s.name, s.n, gid, gx, gy, ghx, ghy, m, fl, extra);
   emu_destroy(e);
   or_flocker_free(fl);
@@ -239,6 +246,15 @@ int main() {
        0, 2, 2.0f, 0.7f},
       {"rt-radius-5x5", 2500, 160.0f, 4.0f, 8, 6, 0, 0, 0, 0, 0, 0, 0, 0, 0, 10.0f, 0.7f},
       {"rt-big-jumps", 2000, 300.0f, D, 10, 8, 0, 0, 0, 0, 0, 0, 0, 0, 0, 10.0f, 9.0f},
+      // tiny normal-class LDS budget: most tiles run in the dense class
+      {"rt-class-split", 1500, 200.0f, D, 30, 11, 0, 0, 0, 0, 0, 0, 2, 0, 0, 10.0f, 0.7f},
+      {"rt-class-split-clustered", 6000, 700.0f, D, 10, 12, 0, 0, 0, 0, 0, 0, 2, 9, 1, 10.0f,
+       0.7f},
+      // normal-class capacities far below the counts: tiles defer to the
+      // dense launch (and past 32 per step take the global path)
+      {"rt-deferred", 3000, 300.0f, D, 12, 13, 0, 0, 0, 64, 24, 8, 4, 0, 1, 10.0f, 0.7f},
+      {"rt-deferred-shuffled", 1500, 200.0f, D, 10, 14, 0, 0, 0, 96, 40, 16, 4, 21, 0, 10.0f,
+       0.7f},
   };
   for (const RtScenario& s : rs) fails += run_rt(s);
   std::printf("%d scenario(s) failed\n", fails);

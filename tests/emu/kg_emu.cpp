@@ -41,20 +41,18 @@ struct HostBuf {
 struct EmuRt {
   bool active = false;
   RtPlan plan;
-  RtCaps caps{};
+  RtCaps caps[2]{};
+  std::vector<uint32_t> dense;
   HostBuf stay[2], inbox[2], spill[2];
   std::vector<uint32_t> spill_rep[2];
   std::vector<uint32_t> cnt[2], boff[2], icnt[3];
   uint32_t scnt[3] = {0, 0, 0};
-  uint32_t flags = 0, done = 0;
+  uint32_t flags = 0;
   uint64_t gstep = 0;
+  std::vector<uint32_t> ovf[2];
   RtDev dev() {
     RtDev D;
-    D.sbase = plan.sbase.data();
-    D.scap = plan.scap.data();
-    D.ibase = plan.ibase.data();
-    D.icap = plan.icap.data();
-    D.tboff = plan.tboff.data();
+    D.meta = plan.meta.data();
     for (int p = 0; p < 2; ++p) {
       D.cnt[p] = cnt[p].data();
       D.boff[p] = boff[p].data();
@@ -69,8 +67,11 @@ struct EmuRt {
     }
     D.spill_cap = (uint32_t)spill[0].x.size();
     D.flags = &flags;
-    D.done = &done;
     D.gstep = &gstep;
+    for (int p = 0; p < 2; ++p) {
+      ovf[p].resize(kRtOvfMax + 1, 0u);
+      D.ovf[p] = ovf[p].data();
+    }
     return D;
   }
 };
@@ -88,7 +89,7 @@ struct emu_field {
   bool w_aligned = false;
   bool force_global = false;
   uint64_t shuffle_seed = 0;  // 0 = scatter in index order
-  uint64_t lds_tiles = 0, global_tiles = 0;
+  uint64_t lds_tiles = 0, global_tiles = 0, deferred_tiles = 0;
 };
 
 extern "C" {
@@ -142,140 +143,148 @@ void emu_destroy(emu_field* f) { delete f; }
 
 static std::vector<uint32_t> order(uint32_t lo, uint32_t hi, uint64_t seed);
 
-static void emu_rt_block(emu_field* f, uint32_t item, int load, uint64_t load_g,
-                         std::vector<uint8_t>& smem) {
+// Output phase of a tile, as rt_output in kg_kernels.hip: the stayers are
+// classified and counted per bin already.
+static void emu_rt_output(const RepDev& rp, const RtTile& rt, const RtDev& D, const RtMeta& me,
+                          uint32_t bo_own, int nb_tile, uint32_t nstay, RtSmem S) {
+  const int pout = rt.pin ^ 1;
+  uint32_t run = 0;
+  for (int b = 0; b < nb_tile; ++b) {
+    S.bbase[b] = run;
+    S.bcur[b] = run;
+    run += S.hist[b];
+  }
+  S.bbase[nb_tile] = nstay;
+  for (uint32_t s2 = 0; s2 < nstay; ++s2) S.operm[S.bcur[S.obin[s2]]++] = (uint16_t)s2;
+  const uint32_t scap = me.scap;
+  for (uint32_t s2 = 0; s2 < nstay; ++s2) rt_write_stayer(rp, rt, D, me, s2, S);
+  for (int b = 0; b <= nb_tile; ++b)
+    D.boff[pout][bo_own + b] = S.bbase[b] < scap ? S.bbase[b] : scap;
+  D.cnt[pout][rt.gt] = nstay < scap ? nstay : scap;
+}
+
+// One work-group of k_rt_step (load = 0) or k_rt_load (load = 1) for the
+// tile `item` in class `cls`, with the threads of each phase run in turn.
+static void emu_rt_block(emu_field* f, uint32_t item, int cls, int load, uint64_t load_g,
+                         std::vector<uint8_t>& smem, bool deferred = false) {
   EmuRt& E = f->rt;
   RtDev D = E.dev();
-  const RtCaps caps = E.caps;
+  const RtCaps caps = E.caps[cls];
   const uint32_t prefix[2] = {0u, f->g.ntiles};
   const uint64_t g = load ? load_g : E.gstep;
   const RtTile rt = rt_tile(&f->rep, 1, prefix, item, g);
+  const RtMeta me = D.meta[rt.gt];
+  if (!deferred && (int)me.cls != cls) return;
   const RepDev& rp = f->rep;
-  std::fill(smem.begin(), smem.end(), 0x5A);
+  smem.assign(rt_smem_bytes(caps), 0x5A);
   const RtSmem S = rt_carve(smem.data(), caps);
   const int nb_tile = rp.g.tx * rp.g.tr;
   const int pout = rt.pin ^ 1;
-  std::vector<float> nx(kRtMaxOwn), ny(kRtMaxOwn), nhx(kRtMaxOwn), nhy(kRtMaxOwn);
-  std::vector<uint32_t> nid(kRtMaxOwn);
-  uint32_t seg_start[3 * kMaxStageCols], seg_len[3 * kMaxStageCols];
-  uint32_t col_base[kMaxStageCols + 1], ecol_base[kMaxStageCols + 1];
-  uint32_t src_base[10], src_pre[11];
-  uint32_t ne = 0, nown_c = 0, nstay = 0, fail = 0, nown = 0;
-  bool global_path = false;
+  const uint32_t bo_own = rt_boff_at(rp, rt, rt.gt);
   Buf Ra = f->R.view();
-  if (!load) {
-    const bool lds_ok = rp.lds_ok != 0 && rt.ncols <= kMaxStageCols && nb_tile <= caps.nb;
-    uint32_t total = 0;
-    for (int t = 0; t < rt.ncols && lds_ok; ++t) {
-      col_base[t] = total;
-      total += rt_segments(rp, rt, D, t, seg_start, seg_len);
-    }
-    col_base[rt.ncols] = total;
-    src_pre[0] = 0;
-    for (int k = 0; k < 10; ++k) src_pre[k + 1] = src_pre[k] + rt_extra_count(rp, rt, D, k, &src_base[k]);
-    global_path = !lds_ok || total > (uint32_t)caps.runs;
-    if (!global_path) {
-      for (uint32_t j = 0; j < src_pre[10]; ++j)
-        if (!rt_gather_extra(rp, rt, D, j, src_base, src_pre, caps.extras, &ne, S)) fail = 1;
-      for (uint32_t k = 0; k < total; ++k)
-        rt_stage(rp, rt, D, k, seg_start, seg_len, col_base, S);
-      global_path = fail != 0;
-      if (!global_path) {
-        for (uint32_t j = 0; j < ne; ++j) rt_rank_extra(j, ne, S);
-        for (int c = 0; c <= rt.ncols; ++c) ecol_base[c] = rt_extra_col_base(c, ne, S);
-        for (uint32_t k = 0; k < total; ++k)
-          if (rt_own(rt, rt.sc_lo + S.scol[k], S.scy[k])) {
-            const uint32_t o = nown_c++;
-            if (o < (uint32_t)caps.own) S.own[o] = (uint16_t)k;
-          }
-        for (uint32_t j = 0; j < ne; ++j)
-          if (rt_own(rt, rt.sc_lo + S.ecol[j], S.ecy[j])) {
-            const uint32_t o = nown_c++;
-            if (o < (uint32_t)caps.own) S.own[o] = (uint16_t)(0x8000u | j);
-          }
-        nown = nown_c;
-        global_path = nown > (uint32_t)caps.own;
-        if (!global_path)
-          for (uint32_t o = 0; o < nown; ++o)
-            rt_agent_lds(rp, rt, S.own[o], col_base, ecol_base, S, &nx[o], &ny[o], &nhx[o],
-                         &nhy[o], &nid[o]);
-      }
-    }
-  } else {
+  if (load) {
     const uint32_t tile = rt.gt - rt.tbase;
     const uint32_t b0 = rp.bin_base + (tile << (rp.g.tx_log2 + rp.g.tr_log2));
     const uint32_t lo = f->offset[b0], hi = f->offset[b0 + (uint32_t)nb_tile];
-    nown = hi - lo;
-    global_path = nown > (uint32_t)caps.own || nb_tile > caps.nb;
-    if (global_path) {
+    const uint32_t nown = hi - lo;
+    if (nown > (uint32_t)caps.own || nb_tile > caps.nb) {
       for (uint32_t i = lo; i < hi; ++i) {
         int32_t cx, cy;
         discretize(rp.g, Ra.x[i], Ra.y[i], &cx, &cy);
-        rt_push(rp, rt, D, rt.tbase + tile_of(rp.g, cx, cy), Ra.x[i], Ra.y[i], Ra.hx[i], Ra.hy[i],
-                Ra.id[i]);
+        rt_push(rp, rt, D, rt.tbase + tile_of(rp.g, cx, cy), Ra.x[i], Ra.y[i], Ra.hx[i],
+                Ra.hy[i], Ra.id[i]);
       }
-    } else {
-      for (uint32_t o = 0; o < nown; ++o) {
-        nx[o] = Ra.x[lo + o];
-        ny[o] = Ra.y[lo + o];
-        nhx[o] = Ra.hx[lo + o];
-        nhy[o] = Ra.hy[lo + o];
-        nid[o] = Ra.id[lo + o];
-      }
+      for (int b = 0; b <= nb_tile; ++b) D.boff[pout][bo_own + b] = 0u;
+      D.cnt[pout][rt.gt] = 0u;
+      ++f->global_tiles;
+      return;
+    }
+    for (int b = 0; b <= nb_tile; ++b) S.hist[b] = 0u;
+    uint32_t nstay = 0;
+    for (uint32_t o = 0; o < nown; ++o) {
+      S.rx[o] = Ra.x[lo + o];
+      S.ry[o] = Ra.y[lo + o];
+      S.rhx[o] = Ra.hx[lo + o];
+      S.rhy[o] = Ra.hy[lo + o];
+      S.rid[o] = Ra.id[lo + o];
+      rt_classify(rp, rt, D, o, &nstay, S);
+    }
+    emu_rt_output(rp, rt, D, me, bo_own, nb_tile, nstay, S);
+    return;
+  }
+  uint32_t seg_start[3 * kMaxStageCols], seg_len[3 * kMaxStageCols];
+  uint32_t col_base[kMaxStageCols + 1];
+  uint32_t src_base[10], src_pre[11];
+  uint32_t ne = 0, nown = 0;
+  bool fail = false;
+  const RtSegs sg = rt_segs(rp.g, rt);
+  const bool lds_ok = rp.lds_ok != 0 && rt_fits_tables(rt, sg, caps) && nb_tile <= caps.nb;
+  if (lds_ok)
+    for (int b = 0; b <= nb_tile; ++b) S.hist[b] = 0u;
+  uint32_t total = 0;
+  for (int t = 0; t < rt.ncols && lds_ok; ++t) {
+    col_base[t] = total;
+    total += rt_segments(rp, rt, D, t, seg_start, seg_len);
+  }
+  col_base[rt.ncols] = total;
+  src_pre[0] = 0;
+  for (int k = 0; k < 10; ++k) src_pre[k + 1] = src_pre[k] + rt_extra_count(rp, rt, D, k, &src_base[k]);
+  bool global_path = !lds_ok || total > (uint32_t)caps.runs;
+  if (!global_path) {
+    const uint32_t next = src_pre[10];
+    for (uint32_t t = 0; t < (uint32_t)kRtThreads; ++t)
+      if (!rt_fetch(rp, rt, D, t, t + kRtThreads, total, t, next, seg_start, seg_len, col_base,
+                    src_base, src_pre, caps, &ne, &nown, S))
+        fail = true;
+    for (uint32_t k = 2 * kRtThreads; k < total; ++k)
+      rt_stage(rp, rt, D, k, seg_start, seg_len, col_base, caps.own, &nown, S);
+    for (uint32_t j = kRtThreads; j < next; ++j)
+      if (!rt_gather_extra(rp, rt, D, j, src_base, src_pre, caps.extras, &ne, caps.own, &nown, S))
+        fail = true;
+    for (uint32_t q = 0; q < (uint32_t)(rt.ncols * sg.nbc); ++q)
+      rt_bintab(rp, rt, D, sg, q, seg_len, S);
+    global_path = fail || nown > (uint32_t)caps.own;
+  }
+  if (global_path && cls == 0 && lds_ok) {
+    const int par = (int)(g & 1u);
+    const uint32_t slot = D.ovf[par][0]++;
+    *D.flags |= kRtFlagDefer;
+    if (slot < kRtOvfMax) {
+      D.ovf[par][1 + slot] = rt.gt;
+      ++f->deferred_tiles;
+      return;
     }
   }
-  std::fill(smem.begin(), smem.end(), 0x3C);  // phase B overwrites phase A
-  if (global_path) {
-    if (!load) {
-      *D.flags |= kRtFlagGlobal;
-      for (uint32_t o = 0;; ++o) {
-        float x, y, hx, hy;
-        uint32_t id;
-        bool skip;
-        if (!rt_own_global(rp, rt, D, o, &x, &y, &hx, &hy, &id, &skip)) break;
-        if (skip) continue;
-        float ax, ay, ahx, ahy;
-        rt_agent_global(rp, rt, D, x, y, hx, hy, id, &ax, &ay, &ahx, &ahy);
-        int32_t cx, cy;
-        discretize(rp.g, ax, ay, &cx, &cy);
-        if (!cell_valid(rp.g, cx, cy)) {
-          *D.flags |= kRtFlagRange;
-          continue;
-        }
-        rt_push(rp, rt, D, rt.tbase + tile_of(rp.g, cx, cy), ax, ay, ahx, ahy, id);
-      }
+  if (!global_path) {
+    ++f->lds_tiles;
+    for (uint32_t j = 0; j < ne; ++j) rt_rank_extra(j, ne, S);
+    for (int c = 0; c <= rt.ncols; ++c) rt_ecbase(c, ne, S);
+    uint32_t nstay = 0;
+    for (uint32_t o = 0; o < nown; ++o) {
+      rt_agent_lds(rp, rt, sg, o, col_base, S);
+      rt_classify(rp, rt, D, o, &nstay, S);
     }
-    for (int b = 0; b <= nb_tile; ++b) D.boff[pout][D.tboff[rt.gt] + b] = 0u;
-    D.cnt[pout][rt.gt] = 0u;
+    // the staging arrays are dead from here on
+    const size_t res = (rt_smem_res(caps) + 15) & ~(size_t)15;
+    std::fill(smem.begin() + (ptrdiff_t)res, smem.end(), 0x3C);
+    emu_rt_output(rp, rt, D, me, bo_own, nb_tile, nstay, S);
   } else {
-    for (uint32_t o = 0; o < nown; ++o)
-      rt_classify(rp, rt, D, false, nx[o], ny[o], nhx[o], nhy[o], nid[o], &nstay, S);
-    const uint32_t scap = D.scap[rt.gt];
-    {
-      for (int b = 0; b <= nb_tile; ++b) S.hist[b] = 0u;
-      for (uint32_t s2 = 0; s2 < nstay; ++s2) S.hist[S.obin[s2]]++;
-      uint32_t run = 0;
-      for (int b = 0; b < nb_tile; ++b) {
-        S.bbase[b] = run;
-        S.bcur[b] = run;
-        run += S.hist[b];
-      }
-      S.bbase[nb_tile] = nstay;
-      for (uint32_t s2 = 0; s2 < nstay; ++s2) S.operm[S.bcur[S.obin[s2]]++] = (uint16_t)s2;
-      for (uint32_t s2 = 0; s2 < nstay; ++s2) rt_write_stayer(rp, rt, D, s2, S);
-      for (int b = 0; b <= nb_tile; ++b)
-        D.boff[pout][D.tboff[rt.gt] + b] = S.bbase[b] < scap ? S.bbase[b] : scap;
-      D.cnt[pout][rt.gt] = nstay < scap ? nstay : scap;
-    }
+    ++f->global_tiles;
+    *D.flags |= kRtFlagGlobal;
+    rt_tile_global(rp, rt, D, 0, 1);
+    for (int b = 0; b <= nb_tile; ++b) D.boff[pout][bo_own + b] = 0u;
+    D.cnt[pout][rt.gt] = 0u;
   }
-  if (!load) {
-    D.icnt[rt.ph_reset][rt.gt] = 0u;
-    if (rt.gt == 0) *D.scnt[rt.ph_reset] = 0u;
-  }
+  D.icnt[rt.ph_reset][rt.gt] = 0u;
+  if (rt.gt == 0) *D.scnt[rt.ph_reset] = 0u;
 }
 
-// Enters the tile-resident layout from the current read buffer.
-int emu_rt_enter(emu_field* f, int caps_runs, int caps_own, int caps_extras, int tight) {
+// Enters the tile-resident layout from the current read buffer.  caps_* > 0
+// override the planned capacities of both classes; `mode` bit 0 makes the
+// per-tile capacities far smaller than the counts (stay overflow, inbox
+// overflow, spill), bit 1 gives the normal class a tiny LDS budget so that
+// most tiles run in the dense class.
+int emu_rt_enter(emu_field* f, int caps_runs, int caps_own, int caps_extras, int mode) {
   if (f->w_aligned || f->nA) return -6;
   EmuRt& E = f->rt;
   const uint32_t nt = f->g.ntiles;
@@ -287,8 +296,7 @@ int emu_rt_enter(emu_field* f, int caps_runs, int caps_own, int caps_extras, int
   for (uint32_t c : counts) total += c;
   rt_expected_counts(f->g, total, &expect);
   rt_plan(counts, nb, expect, &E.plan);
-  E.caps = rt_caps(E.plan, f->g, f->rep.disc_dist, 60000);
-  if (tight) {
+  if (mode & 1) {
     // capacities far below the counts: exercises stay-capacity overflow,
     // inbox overflow and the spill list
     uint64_t s2 = 0, ib = 0;
@@ -297,15 +305,24 @@ int emu_rt_enter(emu_field* f, int caps_runs, int caps_own, int caps_extras, int
       E.plan.icap[t] = 4u;
       E.plan.sbase[t] = (uint32_t)s2;
       E.plan.ibase[t] = (uint32_t)ib;
+      E.plan.meta[t] = RtMeta{E.plan.sbase[t], E.plan.scap[t], E.plan.ibase[t], E.plan.icap[t], 0u,
+                              {0u, 0u, 0u}};
       s2 += E.plan.scap[t];
       ib += E.plan.icap[t];
     }
     E.plan.stay_total = s2;
     E.plan.inbox_total = ib;
   }
-  if (caps_runs > 0) E.caps.runs = caps_runs;
-  if (caps_own > 0) E.caps.own = caps_own;
-  if (caps_extras > 0) E.caps.extras = caps_extras;
+  RtClassPlan cls;
+  rt_classes(&E.plan, counts, f->g, f->rep.disc_dist, (mode & 2) ? 2048 : 32768, 157696, &cls);
+  E.caps[0] = cls.caps[0];
+  E.caps[1] = cls.caps[1];
+  E.dense = cls.dense;
+  for (int c = 0; c < ((mode & 4) ? 1 : 2); ++c) {
+    if (caps_runs > 0) E.caps[c].runs = caps_runs;
+    if (caps_own > 0) E.caps[c].own = caps_own;
+    if (caps_extras > 0) E.caps[c].extras = caps_extras;
+  }
   for (int p = 0; p < 2; ++p) {
     E.stay[p].resize(E.plan.stay_total);
     E.inbox[p].resize(E.plan.inbox_total);
@@ -319,10 +336,12 @@ int emu_rt_enter(emu_field* f, int caps_runs, int caps_own, int caps_extras, int
     E.scnt[k] = 0;
   }
   E.flags = 0;
-  E.done = 0;
   E.gstep = 1;
-  std::vector<uint8_t> smem(rt_smem_bytes(E.caps));
-  for (uint32_t item : order(0, nt, f->shuffle_seed)) emu_rt_block(f, item, 1, E.gstep - 1, smem);
+  for (int p = 0; p < 2; ++p) E.ovf[p].assign(kRtOvfMax + 1, 0u);
+  std::vector<uint8_t> smem;
+  for (int c = 0; c < 2; ++c)
+    for (uint32_t item : order(0, nt, f->shuffle_seed))
+      emu_rt_block(f, item, c, 1, E.gstep - 1, smem);
   E.active = true;
   return 0;
 }
@@ -332,16 +351,33 @@ int emu_rt_steps(emu_field* f, const kg_flock_params* p, uint64_t step0, uint32_
   if (!E.active) return -6;
   set_rep_params(&f->rep, *p);
   f->rep.step_base = step0 - E.gstep;
-  std::vector<uint8_t> smem(rt_smem_bytes(E.caps));
+  std::vector<uint8_t> smem;
   for (uint32_t s2 = 0; s2 < nsteps; ++s2) {
-    for (uint32_t item : order(0, f->g.ntiles, f->shuffle_seed ? f->shuffle_seed + s2 : 0))
-      emu_rt_block(f, item, 0, 0, smem);
+    // the dense-class launch runs after the normal one; tiles are
+    // independent within a step, so any order is valid
+    const uint64_t sh = f->shuffle_seed ? f->shuffle_seed + s2 : 0;
+    for (uint32_t item : order(0, f->g.ntiles, sh)) emu_rt_block(f, item, 0, 0, 0, smem);
+    for (uint32_t k : order(0, (uint32_t)E.dense.size(), sh))
+      emu_rt_block(f, E.dense[k], 1, 0, 0, smem);
+    const int par = (int)(E.gstep & 1u);
+    const uint32_t nd = std::min<uint32_t>(E.ovf[par][0], kRtOvfMax);
+    for (uint32_t i = 0; i < nd; ++i) emu_rt_block(f, E.ovf[par][1 + i], 1, 0, 0, smem, true);
+    E.ovf[par ^ 1][0] = 0;
     E.gstep += 1;
   }
   return (E.flags & kRtFlagRange) ? -3 : 0;
 }
 
 uint32_t emu_rt_flags(emu_field* f) { return f->rt.flags; }
+
+// Work-groups run on the LDS path, on the global path, and deferred.
+void emu_rt_counts(emu_field* f, uint64_t* lds, uint64_t* glob, uint64_t* deferred,
+                   uint32_t* ndense) {
+  *lds = f->lds_tiles;
+  *glob = f->global_tiles;
+  *deferred = f->deferred_tiles;
+  *ndense = (uint32_t)f->rt.dense.size();
+}
 
 // All agents of the tile-resident layout (any order).
 uint32_t emu_rt_read(emu_field* f, uint32_t* ids, float* x, float* y, float* hx, float* hy) {
@@ -359,7 +395,7 @@ uint32_t emu_rt_read(emu_field* f, uint32_t* ids, float* x, float* y, float* hx,
   };
   for (uint32_t item = 0; item < f->g.ntiles; ++item) {
     const RtTile rt = rt_tile(&f->rep, 1, prefix, item, E.gstep);
-    for (uint32_t o = 0; o < D.cnt[rt.pin][rt.gt]; ++o) put(D.stay[rt.pin], D.sbase[rt.gt] + o);
+    for (uint32_t o = 0; o < D.cnt[rt.pin][rt.gt]; ++o) put(D.stay[rt.pin], D.meta[rt.gt].sbase + o);
     uint32_t ib;
     const uint32_t ni = rt_extra_count(f->rep, rt, D, 4, &ib);
     for (uint32_t o = 0; o < ni; ++o) put(D.inbox[rt.pin], ib + o);
