@@ -190,6 +190,73 @@ int kg_field_profile_read(kg_field *f, int kernel, double *total_ms, uint64_t *l
  * [15] schedule steps run tile-resident. */
 int kg_field_stats(kg_field *f, uint64_t out[16]);
 
+/* ---- ensembles of runs (explore_parallel!, src/explore/model_exploration.rs:
+ * 354-432; the serving loop) --------------------------------------------
+ *
+ * A kg_ensemble advances many independent flockers runs -- each its own
+ * toroidal Field2D, parameters, Philox key and schedule step -- in one
+ * tile-resident layout, one launch per schedule step for all of them.  A
+ * run's results are bit-identical to the same run on its own kg_field
+ * (kg_schedule_run), whatever else shares the ensemble.  Runs can join and
+ * leave between kg_ensemble_run calls; the layout is rebuilt then. */
+typedef struct kg_ensemble kg_ensemble;
+
+/* One run: Field2D::new(width, height, discretization, toroidal = 1) with
+ * `agents` birds placed by Flocker::init (state.rs:41-56) from
+ * Philox(init_seed) (see kg_init_uniform), stepped with `params`. */
+typedef struct kg_run_desc {
+    float width;
+    float height;
+    float discretization;
+    int32_t toroidal;     /* must be 1 */
+    uint32_t agents;      /* >= 1 */
+    int32_t heading_mode; /* as kg_init_uniform */
+    uint64_t init_seed;
+    kg_flock_params params;
+} kg_run_desc;
+
+/* Layout choices an autotuner may set; 0 = automatic. */
+typedef struct kg_tuning {
+    uint32_t tile_cols;  /* tile width in cells (power of two) */
+    uint32_t tile_rows;  /* tile height in cells (power of two) */
+    uint32_t bin_rows;   /* bin height in cells (power of two) */
+    uint32_t lds_normal; /* LDS bytes per normal-class work-group */
+} kg_tuning;
+
+/* Device bytes a run of this description occupies in an ensemble. */
+int kg_run_footprint(const kg_run_desc *run, const kg_tuning *tuning, uint64_t *bytes);
+
+int kg_ensemble_create(const kg_tuning *tuning, kg_ensemble **out);
+void kg_ensemble_destroy(kg_ensemble *e);
+
+/* Adds runs (their schedule step 0 comes with the next kg_ensemble_run);
+ * ids[i] receives run i's id.  KG_E_INVALID if a run cannot use the
+ * tile-resident step (not toroidal, window wider than a tile). */
+int kg_ensemble_admit(kg_ensemble *e, const kg_run_desc *runs, uint32_t n, uint32_t *ids);
+
+/* Removes a run (its state is gone; read it first). */
+int kg_ensemble_retire(kg_ensemble *e, uint32_t id);
+
+/* Every run advances `nsteps` schedule steps (simulate!'s loop,
+ * src/lib.rs:1163-1173).  flags: KG_RUN_GRAPH.  Asynchronous. */
+int kg_ensemble_run(kg_ensemble *e, uint32_t nsteps, int flags);
+
+/* Schedule steps run so far by run `id`. */
+int kg_ensemble_steps(kg_ensemble *e, uint32_t id, uint64_t *steps);
+
+/* Copies run `id`'s birds to the host (any order); same contract as
+ * kg_read_agents.  Synchronises. */
+int kg_ensemble_read(kg_ensemble *e, uint32_t id, uint32_t *ids, float *x, float *y, float *hx,
+                     float *hy, size_t cap, size_t *n);
+
+int kg_ensemble_sync(kg_ensemble *e);
+void *kg_ensemble_stream(kg_ensemble *e);
+
+/* Statistics, 8 values: [0] runs, [1] agents, [2] tiles, [3] dense tiles,
+ * [4] layouts built, [5] OR of the tile-resident flags seen, [6] schedule
+ * steps run, [7] device bytes held. */
+int kg_ensemble_stats(kg_ensemble *e, uint64_t out[8]);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char *kg_last_error(void);
 

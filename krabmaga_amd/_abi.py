@@ -30,8 +30,14 @@ EXPORTS = [
     "kg_get_neighbors_within_relax_distance", "kg_get_objects", "kg_num_objects_at_location",
     "kg_num_empty_bags", "kg_nagents", "kg_count", "kg_neighbor_sets", "kg_flock_step",
     "kg_schedule_run", "kg_read_agents", "kg_field_sync", "kg_field_stream",
-    "kg_field_profile", "kg_field_profile_read", "kg_field_stats", "kg_last_error",
+    "kg_field_profile", "kg_field_profile_read", "kg_field_stats",
+    "kg_run_footprint", "kg_ensemble_create", "kg_ensemble_destroy", "kg_ensemble_admit",
+    "kg_ensemble_retire", "kg_ensemble_run", "kg_ensemble_steps", "kg_ensemble_read",
+    "kg_ensemble_sync", "kg_ensemble_stream", "kg_ensemble_stats", "kg_last_error",
 ]
+
+VOID_FUNCS = ("kg_field_destroy", "kg_field_stream", "kg_last_error", "kg_ensemble_destroy",
+              "kg_ensemble_stream")
 
 KERNELS = {"flock": 0, "scan": 1, "scatter": 2, "count_append": 3, "scatter_append": 4,
            "rt_step": 5, "rt_load": 6}
@@ -55,6 +61,18 @@ class FlockParams(ctypes.Structure):
                 ("momentum", ctypes.c_float), ("jump", ctypes.c_float),
                 ("radius", ctypes.c_float), ("reserved", ctypes.c_uint32),
                 ("seed", ctypes.c_uint64)]
+
+
+class RunDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_float), ("height", ctypes.c_float),
+                ("discretization", ctypes.c_float), ("toroidal", ctypes.c_int32),
+                ("agents", ctypes.c_uint32), ("heading_mode", ctypes.c_int32),
+                ("init_seed", ctypes.c_uint64), ("params", FlockParams)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("tile_cols", ctypes.c_uint32), ("tile_rows", ctypes.c_uint32),
+                ("bin_rows", ctypes.c_uint32), ("lds_normal", ctypes.c_uint32)]
 
 
 _lib = None
@@ -103,10 +121,25 @@ def lib():
     L.kg_field_profile_read.argtypes = [P, I, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_uint64)]
     L.kg_field_stats.argtypes = [P, ctypes.POINTER(ctypes.c_uint64)]
+    L.kg_run_footprint.argtypes = [ctypes.POINTER(RunDesc), ctypes.POINTER(Tuning),
+                                   ctypes.POINTER(ctypes.c_uint64)]
+    L.kg_ensemble_create.argtypes = [ctypes.POINTER(Tuning), ctypes.POINTER(P)]
+    L.kg_ensemble_destroy.argtypes = [P]
+    L.kg_ensemble_destroy.restype = None
+    L.kg_ensemble_admit.argtypes = [P, ctypes.POINTER(RunDesc), ctypes.c_uint32, _u32p]
+    L.kg_ensemble_retire.argtypes = [P, ctypes.c_uint32]
+    L.kg_ensemble_run.argtypes = [P, ctypes.c_uint32, I]
+    L.kg_ensemble_steps.argtypes = [P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64)]
+    L.kg_ensemble_read.argtypes = [P, ctypes.c_uint32, _u32p, _f32p, _f32p, _f32p, _f32p,
+                                   ctypes.c_size_t, _szp]
+    L.kg_ensemble_sync.argtypes = [P]
+    L.kg_ensemble_stream.argtypes = [P]
+    L.kg_ensemble_stream.restype = ctypes.c_void_p
+    L.kg_ensemble_stats.argtypes = [P, ctypes.POINTER(ctypes.c_uint64)]
     L.kg_last_error.argtypes = []
     L.kg_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
-        if name not in ("kg_field_destroy", "kg_field_stream", "kg_last_error"):
+        if name not in VOID_FUNCS:
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkrabgpu.so")
 
-SOURCES = ["kg_kernels.hip", "kg_common.cpp", "kg_geom.cpp", "kg_rt_state.cpp", "kg_field.cpp"]
+SOURCES = ["kg_kernels.hip", "kg_common.cpp", "kg_geom.cpp", "kg_rt_state.cpp", "kg_field.cpp",
+           "kg_ensemble.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -279,7 +279,7 @@ int rt_enter(kg_field* f) {
   const uint32_t nt = f->g.ntiles;
   const uint32_t nbt = (uint32_t)(f->g.tx * f->g.tr);
   if (!f->tcount) KG_HIP(dmalloc(&f->tcount, std::max<uint32_t>(nt, 1u)));
-  KG_HIP(launch_tile_counts(f->offset, nbt, nt, f->tcount, f->stream));
+  KG_HIP(launch_tile_counts(f->worklist(), f->offset, f->tcount, f->stream));
   std::vector<uint32_t> counts(nt), nb(nt, nbt);
   KG_HIP(hipMemcpyAsync(counts.data(), f->tcount, nt * sizeof(uint32_t), hipMemcpyDeviceToHost,
                         f->stream));
@@ -291,7 +291,8 @@ int rt_enter(kg_field* f) {
   rt_expected_counts(f->g, total, &expect);
   rt_plan(counts, nb, expect, &plan);
   RtClassPlan cls;
-  rt_classes(&plan, counts, f->g, f->h_rep.disc_dist, rt_lds_normal(), f->lds_limit, &cls);
+  rt_classes(&plan, counts, {RtRepInfo{f->g, f->h_rep.disc_dist, 0u}}, rt_lds_normal(),
+             f->lds_limit, &cls);
   ++f->rt_plans;
   if (!cls.fits) {
     // tiles too crowded for the LDS working set: stay bin-grouped
@@ -317,7 +318,8 @@ int ensure_atomic(kg_field* f) {
   if (!f->rt_mode) return KG_OK;
   if (int rc = rt_check_flags(f)) return rc;
   KG_HIP(launch_fill_u32(f->scratch, 0u, 1, f->stream));
-  KG_HIP(launch_rt_unload(f->worklist(), f->rt.dev(), f->A.b, f->scratch, f->stream));
+  KG_HIP(launch_rt_unload(f->worklist(), f->rt.dev(), f->A.b, f->scratch, 0, f->g.ntiles,
+                          f->stream));
   uint32_t n = 0;
   KG_HIP(hipMemcpyAsync(&n, f->scratch, sizeof(n), hipMemcpyDeviceToHost, f->stream));
   KG_HIP(hipStreamSynchronize(f->stream));

@@ -198,60 +198,71 @@ uint32_t round_up(double v, uint32_t q) {
 
 }  // namespace
 
-void rt_needs(const RtPlan& plan, const std::vector<uint32_t>& counts, const Geom& g, int32_t dd,
-              std::vector<uint32_t>* own, std::vector<uint32_t>* staged) {
+void rt_needs(const RtPlan& plan, const std::vector<uint32_t>& counts,
+              const std::vector<RtRepInfo>& reps, std::vector<uint32_t>* own,
+              std::vector<uint32_t>* staged) {
   const uint32_t nt = (uint32_t)plan.scap.size();
+  own->resize(nt);
+  staged->resize(nt);
   // LDS holds the agents actually present, with room for 64 steps of growth
-  // (a tile past it takes the global path once and the host re-plans); the
-  // HBM capacities of the plan are far roomier.
+  // (a tile past it defers to the dense launch once and the host re-plans);
+  // the HBM capacities of the plan are far roomier.
   std::vector<double> want(nt);
   for (uint32_t t = 0; t < nt; ++t)
     want[t] = std::min<double>(plan.scap[t], std::ceil(1.3 * counts[t]) + 32.0);
-  own->resize(nt);
-  staged->resize(nt);
-  // rows staged above / below a tile: whole bins covering dd rows
-  const double hrows = (double)g.hb * std::ceil((double)dd / (double)g.hb);
-  const double fx = std::min(1.0, (double)dd / (double)g.tx);
-  const double fy = std::min(1.0, hrows / (double)g.ty);
-  for (uint32_t t = 0; t < nt; ++t) {
-    const int32_t tcx = (int32_t)(t % (uint32_t)g.ntx), tcy = (int32_t)(t / (uint32_t)g.ntx);
-    double st = want[t];
-    for (int k = 0; k < 9; ++k) {
-      const int32_t ax = tcx - 1 + k % 3, ay = tcy - 1 + k / 3;
-      if (k == 4 || ax < 0 || ax >= g.ntx || ay < 0 || ay >= g.nty) continue;
-      const double w = (ax != tcx ? fx : 1.0) * (ay != tcy ? fy : 1.0);
-      st += w * want[(uint32_t)ay * (uint32_t)g.ntx + (uint32_t)ax];
+  for (const RtRepInfo& ri : reps) {
+    const Geom& g = ri.g;
+    // rows staged above / below a tile: whole bins covering dd rows
+    const double hrows = (double)g.hb * std::ceil((double)ri.dd / (double)g.hb);
+    const double fx = std::min(1.0, (double)ri.dd / (double)g.tx);
+    const double fy = std::min(1.0, hrows / (double)g.ty);
+    for (uint32_t lt = 0; lt < g.ntiles; ++lt) {
+      const uint32_t t = ri.tile0 + lt;
+      const int32_t tcx = (int32_t)(lt % (uint32_t)g.ntx), tcy = (int32_t)(lt / (uint32_t)g.ntx);
+      double st = want[t];
+      for (int k = 0; k < 9; ++k) {
+        const int32_t ax = tcx - 1 + k % 3, ay = tcy - 1 + k / 3;
+        if (k == 4 || ax < 0 || ax >= g.ntx || ay < 0 || ay >= g.nty) continue;
+        const double w = (ax != tcx ? fx : 1.0) * (ay != tcy ? fy : 1.0);
+        st += w * want[ri.tile0 + (uint32_t)ay * (uint32_t)g.ntx + (uint32_t)ax];
+      }
+      (*own)[t] = (uint32_t)want[t];
+      (*staged)[t] = (uint32_t)std::min(1.0e9, std::ceil(st * 1.1));
     }
-    (*own)[t] = (uint32_t)want[t];
-    (*staged)[t] = (uint32_t)std::min(1.0e9, std::ceil(st * 1.1));
   }
 }
 
-RtCaps rt_caps_for(const Geom& g, int32_t dd, uint32_t own, uint32_t staged, uint32_t icap) {
+RtCaps rt_caps_for(const std::vector<RtRepInfo>& reps, uint32_t own, uint32_t staged) {
   RtCaps c;
-  c.nb = g.tx * g.tr;
-  c.cols = std::min(kMaxStageCols, g.tx + 2 * dd);
-  // staged bins per column: the tile's tr plus whole bins covering dd rows
-  // above and below
-  c.nbc = g.tr + 2 * (int)((dd + g.hb - 1) / g.hb) + 1;
+  c.nb = 1;
+  c.cols = 1;
+  c.nbc = 2;
+  for (const RtRepInfo& ri : reps) {
+    const Geom& g = ri.g;
+    c.nb = std::max(c.nb, g.tx * g.tr);
+    c.cols = std::max(c.cols, std::min(kMaxStageCols, g.tx + 2 * ri.dd));
+    // staged bins per column: the tile's tr plus whole bins covering dd
+    // rows above and below
+    c.nbc = std::max(c.nbc, g.tr + 2 * (int)((ri.dd + g.hb - 1) / g.hb) + 1);
+  }
   c.own = (int)round_up(own, 64);
   c.runs = (int)round_up(staged, 64);
   // extras per step: agents that crossed into the staged rectangle, a few
   // percent of the tile's count; a quarter of the own capacity is ample
   c.extras = (int)std::min<uint32_t>(2048u, std::max<uint32_t>(64u, round_up(0.25 * own, 64)));
-  (void)icap;
   return c;
 }
 
-void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts, const Geom& g, int32_t dd,
-                size_t lds_normal, size_t lds_limit, RtClassPlan* out) {
+void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts,
+                const std::vector<RtRepInfo>& reps, size_t lds_normal, size_t lds_limit,
+                RtClassPlan* out) {
   const uint32_t nt = (uint32_t)plan->scap.size();
   std::vector<uint32_t> own, staged;
-  rt_needs(*plan, counts, g, dd, &own, &staged);
+  rt_needs(*plan, counts, reps, &own, &staged);
   out->dense.clear();
   out->fits = true;
   if (nt == 0) {
-    out->caps[0] = out->caps[1] = rt_caps_for(g, dd, 64, 64, 32);
+    out->caps[0] = out->caps[1] = rt_caps_for(reps, 64, 64);
     return;
   }
   // normal class: the largest own-count quantile whose capacities fit the
@@ -259,20 +270,15 @@ void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts, const Geom& g
   std::vector<uint32_t> so(own);
   std::sort(so.begin(), so.end());
   RtCaps cn{};
-  uint32_t own_n = 0, st_n = 0;
   for (double qtl : {1.0, 0.999, 0.995, 0.99, 0.98, 0.95, 0.9, 0.8, 0.5}) {
-    own_n = so[std::min<size_t>(nt - 1, (size_t)std::floor(qtl * (nt - 1)))];
-    st_n = 0;
-    uint32_t ic = 0;
+    const uint32_t own_n = so[std::min<size_t>(nt - 1, (size_t)std::floor(qtl * (nt - 1)))];
+    uint32_t st_n = 0;
     for (uint32_t t = 0; t < nt; ++t)
-      if (own[t] <= own_n) {
-        st_n = std::max(st_n, staged[t]);
-        ic = std::max(ic, plan->icap[t]);
-      }
-    cn = rt_caps_for(g, dd, own_n, st_n, ic);
+      if (own[t] <= own_n) st_n = std::max(st_n, staged[t]);
+    cn = rt_caps_for(reps, own_n, st_n);
     if (rt_smem_bytes(cn) <= lds_normal) break;
   }
-  uint32_t own_d = 0, st_d = 0, ic_d = 0;
+  uint32_t own_d = 0, st_d = 0;
   for (uint32_t t = 0; t < nt; ++t) {
     const bool dense = own[t] > (uint32_t)cn.own || staged[t] > (uint32_t)cn.runs;
     plan->meta[t].cls = dense ? 1u : 0u;
@@ -280,13 +286,12 @@ void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts, const Geom& g
       out->dense.push_back(t);
       own_d = std::max(own_d, own[t]);
       st_d = std::max(st_d, staged[t]);
-      ic_d = std::max(ic_d, plan->icap[t]);
     }
   }
   // the dense launch also takes normal tiles that outgrow their capacities
   // before the next plan, whose extras may be most of their agents
-  RtCaps cd = rt_caps_for(g, dd, std::max<uint32_t>(own_d, 2u * (uint32_t)cn.own),
-                          std::max<uint32_t>(st_d, 2u * (uint32_t)cn.runs), ic_d);
+  RtCaps cd = rt_caps_for(reps, std::max<uint32_t>(own_d, 2u * (uint32_t)cn.own),
+                          std::max<uint32_t>(st_d, 2u * (uint32_t)cn.runs));
   cd.extras = std::min(2048, std::max(cd.extras, cd.own));
   out->caps[0] = cn;
   out->caps[1] = cd;

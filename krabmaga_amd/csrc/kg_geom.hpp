@@ -43,14 +43,24 @@ void rt_plan(const std::vector<uint32_t>& counts, const std::vector<uint32_t>& n
 // Agents each tile of `g` would hold if `agents` were spread uniformly.
 void rt_expected_counts(const Geom& g, uint64_t agents, std::vector<double>* expect);
 
+// One replica of a tile-resident plan: its geometry, window reach and first
+// global tile.
+struct RtRepInfo {
+  Geom g;
+  int32_t dd;
+  uint32_t tile0;
+};
+
 // Per-tile LDS needs of a plan for the current per-tile counts: own agents
 // (the count with 30 % + 32 of headroom) and staged stayers (that plus the
 // neighbours' halo shares, with 10 % headroom).
-void rt_needs(const RtPlan& plan, const std::vector<uint32_t>& counts, const Geom& g, int32_t dd,
-              std::vector<uint32_t>* own, std::vector<uint32_t>* staged);
+void rt_needs(const RtPlan& plan, const std::vector<uint32_t>& counts,
+              const std::vector<RtRepInfo>& reps, std::vector<uint32_t>* own,
+              std::vector<uint32_t>* staged);
 
-// Capacities of one launch for the given needs.
-RtCaps rt_caps_for(const Geom& g, int32_t dd, uint32_t own, uint32_t staged, uint32_t icap);
+// Capacities of one launch for the given needs, covering every replica's
+// geometry.
+RtCaps rt_caps_for(const std::vector<RtRepInfo>& reps, uint32_t own, uint32_t staged);
 
 // Splits the tiles into a normal class, whose capacities fit `lds_normal`
 // bytes of LDS per work-group, and a dense class (sets plan->meta[].cls).
@@ -60,8 +70,9 @@ struct RtClassPlan {
   std::vector<uint32_t> dense;  // tiles of the dense class
   bool fits = false;
 };
-void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts, const Geom& g, int32_t dd,
-                size_t lds_normal, size_t lds_limit, RtClassPlan* out);
+void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts,
+                const std::vector<RtRepInfo>& reps, size_t lds_normal, size_t lds_limit,
+                RtClassPlan* out);
 
 // Geometry for the tile-resident layout: tiles of about `own_target` agents.
 bool make_geom_rt(float w, float h, float d, int tor, uint64_t expected_agents, uint32_t tx,

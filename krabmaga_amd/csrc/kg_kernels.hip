@@ -749,62 +749,79 @@ hipError_t launch_rt_load(const WorkList& wl, const RtDev& D, const RtCaps& caps
   return hipGetLastError();
 }
 
-// Writes every agent of the tile-resident layout (stayers, inbox and spill
-// entries of the current parity) into the append buffer A; *n_out counts.
-__global__ void k_rt_unload(const RepDev* __restrict__ reps, int nreps,
-                            const uint32_t* __restrict__ tile_prefix, RtDev D, Buf A,
-                            uint32_t* n_out) {
-  const uint32_t item = blockIdx.x;
+// Writes the agents of tiles [tile0, tile0 + gridDim.x) of the tile-resident
+// layout (stayers, inbox entries, and -- for a replica's first tile -- its
+// spill entries, all of the current parity) into the append buffer A: replica
+// r's agents land in [app_base, app_base + n) at rep_cursor[r] (one atomic per
+// work-group, so every replica's range is contiguous).
+__global__ __launch_bounds__(256) void k_rt_unload(
+    const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, RtDev D,
+    Buf A, uint32_t* __restrict__ rep_cursor, uint32_t tile0) {
+  __shared__ uint32_t s_nsp, s_base, s_slot;
+  const uint32_t item = tile0 + blockIdx.x;
   const uint64_t g = *D.gstep;
   const RtTile rt = rt_tile(reps, nreps, tile_prefix, item, g);
   const RepDev& rp = reps[rt.r];
+  const uint32_t t = threadIdx.x;
   const uint32_t ns = D.cnt[rt.pin][rt.gt];
   uint32_t ib;
   const uint32_t ni = rt_extra_count(rp, rt, D, 4, &ib);
   uint32_t sb;
-  const uint32_t nsp = (rt.gt == 0) ? rt_extra_count(rp, rt, D, 9, &sb) : 0u;
-  for (uint32_t o = threadIdx.x; o < ns + ni + nsp; o += blockDim.x) {
-    const Buf* B;
-    uint32_t gi;
-    if (o < ns) {
-      B = &D.stay[rt.pin];
-      gi = D.meta[rt.gt].sbase + o;
-    } else if (o < ns + ni) {
-      B = &D.inbox[rt.pin];
-      gi = ib + (o - ns);
-    } else {
-      B = &D.spill[rt.pin];
-      gi = o - ns - ni;
-    }
-    const uint32_t k = atomicAdd(n_out, 1u);
-    A.x[k] = B->x[gi];
-    A.y[k] = B->y[gi];
-    A.hx[k] = B->hx[gi];
-    A.hy[k] = B->hy[gi];
-    A.id[k] = B->id[gi];
+  const bool first = rt.gt == rt.tbase;
+  const uint32_t nsl = first ? rt_extra_count(rp, rt, D, 9, &sb) : 0u;
+  if (t == 0) {
+    s_nsp = 0;
+    s_slot = 0;
   }
+  __syncthreads();
+  for (uint32_t q = t; q < nsl; q += blockDim.x)
+    if (D.spill_rep[rt.pin][q] == (uint32_t)rt.r) atomicAdd(&s_nsp, 1u);
+  __syncthreads();
+  const uint32_t n = ns + ni + s_nsp;
+  if (t == 0) s_base = rp.app_base + atomicAdd(&rep_cursor[rt.r], n);
+  __syncthreads();
+  const uint32_t base = s_base;
+  auto put = [&](const Buf& B, uint32_t gi, uint32_t k) {
+    A.x[base + k] = B.x[gi];
+    A.y[base + k] = B.y[gi];
+    A.hx[base + k] = B.hx[gi];
+    A.hy[base + k] = B.hy[gi];
+    A.id[base + k] = B.id[gi];
+  };
+  const uint32_t sbase = D.meta[rt.gt].sbase;
+  for (uint32_t o = t; o < ns; o += blockDim.x) put(D.stay[rt.pin], sbase + o, o);
+  for (uint32_t o = t; o < ni; o += blockDim.x) put(D.inbox[rt.pin], ib + o, ns + o);
+  for (uint32_t q = t; q < nsl; q += blockDim.x)
+    if (D.spill_rep[rt.pin][q] == (uint32_t)rt.r) put(D.spill[rt.pin], q, ns + ni + atomicAdd(&s_slot, 1u));
 }
 
-hipError_t launch_rt_unload(const WorkList& wl, const RtDev& D, Buf A, uint32_t* n_out,
-                            hipStream_t s) {
-  if (wl.ntiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rt_unload, dim3(wl.ntiles), dim3(256), 0, s, wl.reps, wl.nreps,
-                     wl.tile_prefix, D, A, n_out);
+hipError_t launch_rt_unload(const WorkList& wl, const RtDev& D, Buf A, uint32_t* rep_cursor,
+                            uint32_t tile0, uint32_t ntiles, hipStream_t s) {
+  if (ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rt_unload, dim3(ntiles), dim3(256), 0, s, wl.reps, wl.nreps,
+                     wl.tile_prefix, D, A, rep_cursor, tile0);
   return hipGetLastError();
 }
 
-// count[t] = agents of tile t in the bin-grouped layout (tiles are runs of
-// nbt consecutive bins).
-__global__ void k_tile_counts(const uint32_t* __restrict__ offset, uint32_t nbt, uint32_t nt,
-                              uint32_t* __restrict__ count) {
+// count[t] = agents of tile t in the bin-grouped layout (a replica's tiles
+// are runs of tx * tr consecutive bins from its bin_base).
+__global__ void k_tile_counts(const RepDev* __restrict__ reps, int nreps,
+                              const uint32_t* __restrict__ tile_prefix, uint32_t nt,
+                              const uint32_t* __restrict__ offset, uint32_t* __restrict__ count) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nt) count[t] = offset[(size_t)(t + 1) * nbt] - offset[(size_t)t * nbt];
+  if (t >= nt) return;
+  const int r = find_rep(tile_prefix, nreps, t);
+  const Geom& g = reps[r].g;
+  const uint32_t nbt = (uint32_t)(g.tx * g.tr);
+  const uint32_t b0 = reps[r].bin_base + (t - tile_prefix[r]) * nbt;
+  count[t] = offset[b0 + nbt] - offset[b0];
 }
 
-hipError_t launch_tile_counts(const uint32_t* offset, uint32_t nbt, uint32_t nt, uint32_t* count,
+hipError_t launch_tile_counts(const WorkList& wl, const uint32_t* offset, uint32_t* count,
                               hipStream_t s) {
-  if (nt == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tile_counts, dim3((nt + 255) / 256), dim3(256), 0, s, offset, nbt, nt, count);
+  if (wl.ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tile_counts, dim3((wl.ntiles + 255) / 256), dim3(256), 0, s, wl.reps,
+                     wl.nreps, wl.tile_prefix, wl.ntiles, offset, count);
   return hipGetLastError();
 }
 

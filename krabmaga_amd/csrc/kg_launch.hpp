@@ -32,13 +32,13 @@ hipError_t launch_materialize(Buf R, Buf W, uint32_t n, Buf A, hipStream_t s);
 size_t rt_lds_bytes(const RtCaps& c);
 hipError_t launch_rt_step(const WorkList& wl, const RtDev& D, const RtCaps& caps, uint32_t koff,
                           int cls, const uint32_t* list, uint32_t nlist, hipStream_t s);
-hipError_t launch_tile_counts(const uint32_t* offset, uint32_t nbt, uint32_t nt, uint32_t* count,
+hipError_t launch_tile_counts(const WorkList& wl, const uint32_t* offset, uint32_t* count,
                               hipStream_t s);
 hipError_t launch_rt_load(const WorkList& wl, const RtDev& D, const RtCaps& caps, int cls,
                           const uint32_t* list, uint32_t nlist, Buf Ra, const uint32_t* offset_a,
                           uint64_t load_g, hipStream_t s);
-hipError_t launch_rt_unload(const WorkList& wl, const RtDev& D, Buf A, uint32_t* n_out,
-                            hipStream_t s);
+hipError_t launch_rt_unload(const WorkList& wl, const RtDev& D, Buf A, uint32_t* rep_cursor,
+                            uint32_t tile0, uint32_t ntiles, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint32_t n, hipStream_t s);
 
 // Single-replica queries (API and parity use; not on the hot path).

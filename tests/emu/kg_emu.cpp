@@ -314,7 +314,8 @@ int emu_rt_enter(emu_field* f, int caps_runs, int caps_own, int caps_extras, int
     E.plan.inbox_total = ib;
   }
   RtClassPlan cls;
-  rt_classes(&E.plan, counts, f->g, f->rep.disc_dist, (mode & 2) ? 2048 : 32768, 157696, &cls);
+  rt_classes(&E.plan, counts, {RtRepInfo{f->g, f->rep.disc_dist, 0u}}, (mode & 2) ? 2048 : 32768,
+             157696, &cls);
   E.caps[0] = cls.caps[0];
   E.caps[1] = cls.caps[1];
   E.dense = cls.dense;
