@@ -9,6 +9,11 @@ flockers model: neighbour query + Bird::step for every agent and the
 rebinning of the field (lazy_update).  State is resident in HBM before the
 timed region starts.
 
+Beside it, in the same JSON line: the batched ensemble (512 replicas of
+uneven size, one launch per step for all of them, layout chosen by the
+autotuner) against the same runs launched one by one, and the serving loop
+(Poisson arrivals admitted by HBM footprint) with p50/p99 latency.
+
 Prints one JSON line.  `python bench.py --gpus N` with N > 1 exits: this
 build measures one GPU.
 """
@@ -143,6 +148,136 @@ def cpu_baseline(args):
     }
 
 
+def _timed(hip, stream, fn):
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    fn()
+    hip.record(e1, stream)
+    return hip.elapsed_ms(e0, e1)
+
+
+def _one_by_one(hip, jobs, warmup, whole_run=False):
+    """The same runs launched one by one with the plain kernels, a field per
+    run.  whole_run=False: each run warmed up, then its steps timed with HIP
+    events (state resident, as the batched figure); True: the wall time of
+    each run's whole life -- field, Flocker::init, every step, read-back --
+    as a served request has it.  Returns the summed milliseconds."""
+    from krabmaga_amd import Field2D
+    total_ms = 0.0
+    for spec, steps in jobs:
+        if whole_run:
+            t0 = time.perf_counter()
+            f = Field2D(spec.width, spec.height, spec.discretization, True, spec.agents)
+            f.init_uniform(spec.agents, spec.init_seed, spec.heading_mode)
+            f.schedule_run(spec.params, 0, steps)
+            f.read_agents(0)
+            total_ms += 1e3 * (time.perf_counter() - t0)
+            f.close()
+            continue
+        f = Field2D(spec.width, spec.height, spec.discretization, True, spec.agents)
+        f.init_uniform(spec.agents, spec.init_seed, spec.heading_mode)
+        f.schedule_run(spec.params, 0, warmup)
+        f.sync()
+        total_ms += _timed(hip, f.stream, lambda: f.schedule_run(spec.params, warmup, steps))
+        f.close()
+    return total_ms
+
+
+def run_batched(args, hip):
+    """512 replicas of uneven size in one tile-resident layout."""
+    from krabmaga_amd import Ensemble, autotune, workloads
+    specs = workloads.batched(args.replicas)
+    agents = sum(s.agents for s in specs)
+    steps = args.batched_steps
+    log(f"batched: {len(specs)} replicas, {agents} agents")
+    table = []
+    tuning = None
+    if not args.no_tune:
+        tuning, table = autotune.tune(specs, steps=16, warmup=8, log=log)
+    ens = Ensemble(tuning)
+    ens.admit(specs)
+    ens.run(args.warmup)
+    ens.sync()
+    ms = _timed(hip, ens.stream, lambda: ens.run(steps))
+    st = ens.stats()
+    ens.close()
+    value = agents * steps / (ms * 1e-3)
+    log(f"batched: {ms / steps:.3f} ms/step, {value:.3e} agent-updates/s, layout {st}")
+    # one by one with the plain kernels, on a bounded sample of the runs
+    k = min(len(specs), args.one_by_one_runs)
+    sample = specs[:: max(1, len(specs) // k)][:k]
+    sample_agents = sum(s.agents for s in sample)
+    obo_ms = _one_by_one(hip, [(s, steps) for s in sample], args.warmup)
+    obo_value = sample_agents * steps / (obo_ms * 1e-3)
+    log(f"one by one: {len(sample)} runs, {obo_value:.3e} agent-updates/s")
+    out = {
+        "workload": f"{len(specs)} flockers replicas, agents log-uniform in [1000, 262144], "
+                    f"constant density 0.025, weights U[0.5, 2], jump U[0.3, 1], {steps} steps",
+        "replicas": len(specs), "agents": agents, "steps": steps,
+        "value": value, "unit": "agent-updates/s", "ms_per_step": ms / steps,
+        "one_by_one": {"value": obo_value, "runs": len(sample), "agents": sample_agents,
+                       "sample": "every %d-th replica, each on its own field" %
+                                 max(1, len(specs) // k)},
+        "vs_one_by_one": value / obo_value,
+        "tuning": None if tuning is None else tuning.__dict__,
+        "autotune": [{"tuning": t.__dict__, "ms_per_step": m} for t, m in table],
+        "layout": st,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_ensemble(specs, args.cpu_threads)
+    return out
+
+
+def cpu_ensemble(specs, threads):
+    """explore_parallel!'s rayon path: the oracle on a thread pool, one run
+    per thread at a time, bounded to one schedule step of every replica."""
+    from oracle import oracle as O
+    nthreads = threads or min(16, os.cpu_count() or 1)
+    jobs = [{"n": s.agents, "side": s.width, "discretization": s.discretization, "steps": 1,
+             "params": O.params(cohesion=s.params.cohesion, avoidance=s.params.avoidance,
+                                consistency=s.params.consistency,
+                                randomness=s.params.randomness, momentum=s.params.momentum,
+                                jump=s.params.jump, radius=s.params.radius, seed=s.params.seed),
+             "heading_mode": s.heading_mode} for s in specs]
+    wall, _, _ = O.run_ensemble(jobs, nthreads)
+    agents = sum(s.agents for s in specs)
+    return {"value": agents / wall, "unit": "agent-updates/s", "cores": nthreads,
+            "kind": "port",
+            "sample": f"1 schedule step of each of the {len(specs)} replicas "
+                      f"({agents} agent-updates, {wall:.1f} s)"}
+
+
+def run_serving(args, hip):
+    """Poisson arrivals of exploration runs, admitted by HBM footprint."""
+    from krabmaga_amd import serve, workloads
+    reqs = workloads.serving(rate=args.serve_rate, duration=args.serve_seconds)
+    offered = sum(r.spec.agents * r.steps for r in reqs) / args.serve_seconds
+    log(f"serving: {len(reqs)} requests over {args.serve_seconds} s")
+    srv = serve.Server(args.serve_budget_gb * 2**30, keep_state=True)
+    done = srv.serve(reqs)
+    sm = serve.summary(done, srv.wall)
+    log(f"serving: {sm}")
+    k = min(len(reqs), args.one_by_one_runs)
+    sample = reqs[:: max(1, len(reqs) // k)][:k]
+    obo_ms = _one_by_one(hip, [(r.spec, r.steps) for r in sample], args.warmup, whole_run=True)
+    obo_value = sum(r.spec.agents * r.steps for r in sample) / (obo_ms * 1e-3)
+    log(f"serving one by one: {len(sample)} runs, {obo_value:.3e} agent-updates/s")
+    sm.update({
+        "workload": f"Poisson arrivals at {args.serve_rate:g} runs/s for {args.serve_seconds:g} s, "
+                    "agents log-uniform in [1000, 1e6], steps uniform in [50, 2000], weights "
+                    "U[0.5, 2], jump U[0.3, 1]",
+        "offered_agent_updates_per_s": offered,
+        "hbm_budget_bytes": srv.budget, "peak_admitted_bytes": srv.peak_bytes,
+        "ticks": srv.ticks,
+        "one_by_one": {"value": obo_value, "runs": len(sample),
+                       "sample": "every %d-th request run alone on its own field: creation, "
+                                 "init, all steps and read-back timed, as a served request"
+                                 % max(1, len(reqs) // k)},
+        "vs_one_by_one": sm["agent_updates_per_s"] / obo_value,
+    })
+    return sm
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +287,16 @@ def main():
     ap.add_argument("--binned-steps", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--batched-steps", type=int, default=300)
+    ap.add_argument("--one-by-one-runs", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-seconds", type=float, default=0.5)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -215,6 +360,10 @@ def main():
     if not args.no_cpu:
         log("CPU baseline (oracle, 1 thread) ...")
         out["cpu_baseline"] = cpu_baseline(args)
+    if not args.no_batched:
+        out["batched"] = run_batched(args, hip)
+    if not args.no_serving:
+        out["serving"] = run_serving(args, hip)
     print(json.dumps(out))
 
 

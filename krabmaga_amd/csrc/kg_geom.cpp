@@ -290,9 +290,18 @@ void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts,
   }
   // the dense launch also takes normal tiles that outgrow their capacities
   // before the next plan, whose extras may be most of their agents
-  RtCaps cd = rt_caps_for(reps, std::max<uint32_t>(own_d, 2u * (uint32_t)cn.own),
-                          std::max<uint32_t>(st_d, 2u * (uint32_t)cn.runs));
-  cd.extras = std::min(2048, std::max(cd.extras, cd.own));
+  // (twice the normal capacities, or less where LDS cannot hold that)
+  uint32_t own_x = std::max<uint32_t>(own_d, 2u * (uint32_t)cn.own);
+  uint32_t st_x = std::max<uint32_t>(st_d, 2u * (uint32_t)cn.runs);
+  RtCaps cd;
+  for (;;) {
+    cd = rt_caps_for(reps, own_x, st_x);
+    cd.extras = std::min(2048, std::max(cd.extras, cd.own));
+    if (rt_smem_bytes(cd) <= lds_limit || (own_x <= own_d && st_x <= st_d)) break;
+    const uint32_t so = std::max<uint32_t>(64u, own_x / 8u), ss = std::max<uint32_t>(64u, st_x / 8u);
+    own_x = own_x > own_d + so ? own_x - so : own_d;
+    st_x = st_x > st_d + ss ? st_x - ss : st_d;
+  }
   out->caps[0] = cn;
   out->caps[1] = cd;
   out->fits = rt_smem_bytes(cn) <= lds_limit && rt_smem_bytes(cd) <= lds_limit &&
