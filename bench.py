@@ -254,7 +254,7 @@ def run_serving(args, hip):
     offered = sum(r.spec.agents * r.steps for r in reqs) / args.serve_seconds
     log(f"serving: {len(reqs)} requests over {args.serve_seconds} s")
     srv = serve.Server(args.serve_budget_gb * 2**30, keep_state=True)
-    done = srv.serve(reqs)
+    done = srv.serve(reqs, progress=log)
     sm = serve.summary(done, srv.wall)
     log(f"serving: {sm}")
     k = min(len(reqs), args.one_by_one_runs)
@@ -268,7 +268,7 @@ def run_serving(args, hip):
                     "U[0.5, 2], jump U[0.3, 1]",
         "offered_agent_updates_per_s": offered,
         "hbm_budget_bytes": srv.budget, "peak_admitted_bytes": srv.peak_bytes,
-        "ticks": srv.ticks,
+        "ticks": srv.ticks, "admissions": srv.admissions, "layout": srv.layout,
         "one_by_one": {"value": obo_value, "runs": len(sample),
                        "sample": "every %d-th request run alone on its own field: creation, "
                                  "init, all steps and read-back timed, as a served request"

@@ -32,6 +32,7 @@ struct EnsRun {
   Geom g{};
   uint64_t steps_done = 0;  // schedule steps completed
   bool fresh = true;        // initial state not generated yet
+  bool dead = false;        // retired; dropped at the next build
   uint32_t app_base = 0;    // slot in the append buffer of the current layout
   uint32_t tile0 = 0, ntiles = 0;
 };
@@ -215,6 +216,11 @@ hipError_t copy_buf(Buf dst, uint32_t dbase, Buf src, uint32_t sbase, uint32_t n
 int rebuild(kg_ensemble* e) {
   KG_HIP(hipStreamSynchronize(e->stream));
   const uint64_t G = e->rt.G;
+  // retired runs leave; the survivors keep their slots in the old layout
+  // for the unload below
+  e->runs.erase(std::remove_if(e->runs.begin(), e->runs.end(),
+                               [](const EnsRun& r) { return r.dead; }),
+                e->runs.end());
   // new slots; survivors of a built layout are unloaded first
   std::vector<uint32_t> old_base(e->runs.size());
   uint64_t agents = 0, bins = 0, tiles = 0;
@@ -385,18 +391,19 @@ int steps(kg_ensemble* e, uint32_t n, bool use_graph) {
 
 int check_flags(kg_ensemble* e) {
   KG_HIP(hipStreamSynchronize(e->stream));
-  uint32_t fl = 0;
-  KG_HIP(hipMemcpy(&fl, e->rt.flags, sizeof(fl), hipMemcpyDeviceToHost));
+  uint32_t fl2[2] = {0, 0};
+  KG_HIP(hipMemcpy(fl2, e->rt.flags, sizeof(fl2), hipMemcpyDeviceToHost));
+  const uint32_t fl = fl2[0];
   e->flags_seen |= fl;
   if (fl & kRtFlagRange)
     return set_error(KG_E_RANGE, "an agent left its run's bag grid");
-  if (fl & (kRtFlagSpill | kRtFlagGlobal | kRtFlagDefer)) e->replan = true;
+  if ((fl & (kRtFlagSpill | kRtFlagGlobal)) || fl2[1] >= kRtReplanDeferred) e->replan = true;
   return KG_OK;
 }
 
 EnsRun* find_run(kg_ensemble* e, uint32_t id, size_t* idx = nullptr) {
   for (size_t i = 0; i < e->runs.size(); ++i)
-    if (e->runs[i].id == id) {
+    if (e->runs[i].id == id && !e->runs[i].dead) {
       if (idx) *idx = i;
       return &e->runs[i];
     }
@@ -481,9 +488,16 @@ int kg_ensemble_retire(kg_ensemble* e, uint32_t id) {
   clear_error();
   if (!e) return set_error(KG_E_INVALID, "ensemble is NULL");
   size_t idx = 0;
-  if (!find_run(e, id, &idx)) return set_error(KG_E_INVALID, "no run %u", id);
-  // the survivors are unloaded from the current layout at the next build
-  // using their old slots, so only the list entry goes
+  EnsRun* r = find_run(e, id, &idx);
+  if (!r) return set_error(KG_E_INVALID, "no run %u", id);
+  if (e->built && !e->dirty && idx < e->nreps) {
+    // the run stays in the layout as a dead replica (its tiles exit at
+    // once) until the next build, so retiring costs no rebuild
+    r->dead = true;
+    e->h_reps[idx].dead = 1u;
+    KG_HIP(launch_set_rep(e->d_reps + idx, e->h_reps[idx], e->stream));
+    return KG_OK;
+  }
   e->runs.erase(e->runs.begin() + (ptrdiff_t)idx);
   e->dirty = true;
   return KG_OK;
@@ -555,9 +569,13 @@ void* kg_ensemble_stream(kg_ensemble* e) { return e ? (void*)e->stream : nullptr
 int kg_ensemble_stats(kg_ensemble* e, uint64_t out[8]) {
   clear_error();
   if (!e || !out) return set_error(KG_E_INVALID, "NULL argument");
-  uint64_t agents = 0;
-  for (const EnsRun& r : e->runs) agents += r.desc.agents;
-  const uint64_t v[8] = {e->runs.size(), agents,         e->ntiles,     e->rt.ndense,
+  uint64_t agents = 0, live = 0;
+  for (const EnsRun& r : e->runs)
+    if (!r.dead) {
+      agents += r.desc.agents;
+      ++live;
+    }
+  const uint64_t v[8] = {live,      agents,         e->ntiles,     e->rt.ndense,
                          e->builds,      e->flags_seen, e->steps_run, e->bytes_held()};
   std::memcpy(out, v, sizeof(v));
   return KG_OK;

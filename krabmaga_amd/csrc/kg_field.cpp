@@ -252,13 +252,16 @@ int set_params(kg_field* f, const kg_flock_params* p, uint64_t step_base) {
 int rt_check_flags(kg_field* f) {
   if (!f->rt_mode) return KG_OK;
   KG_HIP(hipStreamSynchronize(f->stream));
-  uint32_t fl = 0;
-  KG_HIP(hipMemcpy(&fl, f->rt.flags, sizeof(fl), hipMemcpyDeviceToHost));
+  uint32_t fl2[2] = {0, 0};
+  KG_HIP(hipMemcpy(fl2, f->rt.flags, sizeof(fl2), hipMemcpyDeviceToHost));
+  const uint32_t fl = fl2[0];
   f->rt_flags_seen |= fl;
   if (fl & kRtFlagRange) f->poisoned = true;
-  // stay overflow is absorbed by the inboxes; spilling or falling back to the
-  // global path costs every tile, so those ask for a new plan
-  if (fl & (kRtFlagSpill | kRtFlagGlobal | kRtFlagDefer)) f->rt_replan = true;
+  // stay overflow is absorbed by the inboxes and a few deferred tiles by
+  // the dense launch; many deferrals mean the plan no longer fits the
+  // density (left alone they grow into global-path tiles), and spilling or
+  // the global path costs every tile: those ask for a new plan
+  if ((fl & (kRtFlagSpill | kRtFlagGlobal)) || fl2[1] >= kRtReplanDeferred) f->rt_replan = true;
   return check_poison(f);
 }
 

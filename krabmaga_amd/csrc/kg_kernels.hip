@@ -575,9 +575,12 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_step(
   } else {
     // deferred normal tiles of this step; block nlist clears the list that
     // the next step's normal launch fills
-    if (blockIdx.x == nlist && t == 0) D.ovf[par ^ 1][0] = 0u;
-    const uint32_t i = blockIdx.x - nlist;
     const uint32_t n = D.ovf[par][0];
+    if (blockIdx.x == nlist && t == 0) {
+      D.ovf[par ^ 1][0] = 0u;
+      atomicMax(&D.flags[1], n);
+    }
+    const uint32_t i = blockIdx.x - nlist;
     if (i >= n || i >= kRtOvfMax) return;
     item = D.ovf[par][1 + i];
     deferred = true;
@@ -586,6 +589,12 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_step(
   const RtMeta me = D.meta[rt.gt];
   if (!deferred && (int)me.cls != cls) return;
   const RepDev& rp = reps[rt.r];
+  if (rp.dead) {
+    // a retired run: no work, but the engine-wide spill counter of step
+    // g + 2 is still tile 0's to clear
+    if (t == 0 && rt.gt == 0) *D.scnt[rt.ph_reset] = 0u;
+    return;
+  }
   const RtSmem S = rt_carve(smem_raw, caps);
   const int nb_tile = rp.g.tx * rp.g.tr;
   const int pout = rt.pin ^ 1;
@@ -762,6 +771,7 @@ __global__ __launch_bounds__(256) void k_rt_unload(
   const uint64_t g = *D.gstep;
   const RtTile rt = rt_tile(reps, nreps, tile_prefix, item, g);
   const RepDev& rp = reps[rt.r];
+  if (rp.dead) return;
   const uint32_t t = threadIdx.x;
   const uint32_t ns = D.cnt[rt.pin][rt.gt];
   uint32_t ib;

@@ -50,7 +50,9 @@ constexpr uint32_t kRtFlagDefer = 16u;   // a normal tile outgrew its class and 
 // Normal tiles that outgrow their launch's capacities defer to the dense
 // launch of the same step through a per-parity list ([0] count, then ids);
 // the dense launch has kRtOvfMax work-groups beyond its static tiles.
-constexpr uint32_t kRtOvfMax = 32;
+constexpr uint32_t kRtOvfMax = 256;
+// Deferrals per step past which the host re-plans.
+constexpr uint32_t kRtReplanDeferred = kRtOvfMax / 2;
 
 // Per-tile plan entry, two 16-byte loads.
 struct alignas(16) RtMeta {
@@ -73,7 +75,7 @@ struct RtDev {
   Buf spill[2];
   uint32_t* spill_rep[2];  // replica of each spill entry
   uint32_t spill_cap;
-  uint32_t* flags;
+  uint32_t* flags;  // [0] kRtFlag* bits, [1] most tiles deferred in one step
   uint64_t* gstep;  // schedule step of launch k = step_base + *gstep + k
   uint32_t* ovf[2];  // deferred normal tiles, by step parity (kRtOvfMax + 1 each)
 };

@@ -75,7 +75,7 @@ int RtState::setup(const RtPlan& pl, const RtClassPlan& cls, uint64_t spill_cap,
     }
     for (int k = 0; k < 3; ++k) ok(dmalloc(&icnt[k], nt));
     ok(dmalloc(&scnt, 3));
-    ok(dmalloc(&flags, 1));
+    ok(dmalloc(&flags, 2));
     ok(dmalloc(&ovf, 2 * (kRtOvfMax + 1)));
     ok(dmalloc(&gstep, 1));
     if (e != hipSuccess) {
@@ -103,7 +103,7 @@ int RtState::setup(const RtPlan& pl, const RtClassPlan& cls, uint64_t spill_cap,
 int RtState::reset(uint64_t g, hipStream_t s) {
   for (int k = 0; k < 3; ++k) KG_HIP(launch_fill_u32(icnt[k], 0u, ntiles, s));
   KG_HIP(launch_fill_u32(scnt, 0u, 3, s));
-  KG_HIP(launch_fill_u32(flags, 0u, 1, s));
+  KG_HIP(launch_fill_u32(flags, 0u, 2, s));
   KG_HIP(launch_fill_u32(ovf, 0u, 2 * (kRtOvfMax + 1), s));
   KG_HIP(launch_set_u64(gstep, g, s));
   G = g;

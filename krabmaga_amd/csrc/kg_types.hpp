@@ -21,6 +21,7 @@ struct RepDev {
   uint32_t app_base;   // first append-buffer entry of this replica
   uint32_t app_n;      // append-buffer entries of this replica
   uint32_t boff_base;  // tile-resident layout: start of tile 0's bin-offset table
+  uint32_t dead;       // retired replica: its tiles do no work (ensembles)
   uint64_t step_base;  // schedule step of this replica = step_base + *gstep
 };
 

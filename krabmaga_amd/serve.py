@@ -35,7 +35,7 @@ class Completed:
 
 class Server:
     def __init__(self, hbm_budget, tuning=None, max_tick=64, keep_state=False, clock=None,
-                 ensemble=None, footprint_fn=None):
+                 ensemble=None, footprint_fn=None, admit_every=1):
         self.budget = int(hbm_budget)
         self.tuning = tuning
         self.max_tick = int(max_tick)
@@ -43,13 +43,17 @@ class Server:
         self.clock = clock or time.perf_counter
         self.make_ensemble = ensemble or (lambda: Ensemble(self.tuning))
         self.footprint = footprint_fn or (lambda spec: footprint(spec, self.tuning))
+        # while runs are active, new ones join at most every `admit_every`
+        # ticks: each admission rebuilds the layout
+        self.admit_every = max(1, int(admit_every))
         self.ticks = 0
         self.admissions = 0
         self.peak_bytes = 0
 
-    def serve(self, requests):
+    def serve(self, requests, progress=None):
         """Runs the request stream to completion against the wall clock;
-        returns the Completed records in completion order."""
+        returns the Completed records in completion order.  progress(msg),
+        if given, is called every 16 ticks."""
         reqs = sorted(enumerate(requests), key=lambda r: r[1].arrival)
         fp = [self.footprint(r.spec) for _, r in reqs]
         for (i, r), b in zip(reqs, fp):
@@ -61,11 +65,13 @@ class Server:
         head = 0
         ens = self.make_ensemble()
         t0 = self.clock()
+        last_admit = -self.admit_every
         try:
             while head < len(reqs) or active:
                 now = self.clock() - t0
                 batch = []
-                while head < len(reqs) and reqs[head][1].arrival <= now and \
+                may_admit = not active or self.ticks - last_admit >= self.admit_every
+                while may_admit and head < len(reqs) and reqs[head][1].arrival <= now and \
                         used + fp[head] <= self.budget:
                     batch.append((reqs[head], fp[head]))
                     used += fp[head]
@@ -74,6 +80,7 @@ class Server:
                 if batch:
                     ids = ens.admit([r.spec for (_, r), _ in batch])
                     self.admissions += 1
+                    last_admit = self.ticks
                     for rid, ((i, r), b) in zip(ids, batch):
                         active[rid] = [i, r, r.steps, now, b]
                 if not active:
@@ -86,6 +93,9 @@ class Server:
                 ens.run(tick)
                 ens.sync()
                 self.ticks += 1
+                if progress and self.ticks % 16 == 0:
+                    progress(f"tick {self.ticks}: {len(active)} active, {len(done)} done, "
+                             f"{len(reqs) - head} queued, {self.clock() - t0:.1f} s")
                 now = self.clock() - t0
                 for rid in [k for k, a in active.items() if a[2] == tick]:
                     i, r, _, adm, b = active.pop(rid)
@@ -95,9 +105,10 @@ class Server:
                     done.append(Completed(i, r.arrival, adm, now, r.spec.agents, r.steps, state))
                 for a in active.values():
                     a[2] -= tick
+            self.wall = self.clock() - t0
+            self.layout = ens.stats() if hasattr(ens, "stats") else {}
         finally:
             ens.close()
-        self.wall = self.clock() - t0
         return done
 
 

@@ -251,7 +251,7 @@ int main() {
       {"rt-class-split-clustered", 6000, 700.0f, D, 10, 12, 0, 0, 0, 0, 0, 0, 2, 9, 1, 10.0f,
        0.7f},
       // normal-class capacities far below the counts: tiles defer to the
-      // dense launch (and past 32 per step take the global path)
+      // dense launch (past the deferral list's capacity they take the global path)
       {"rt-deferred", 3000, 300.0f, D, 12, 13, 0, 0, 0, 64, 24, 8, 4, 0, 1, 10.0f, 0.7f},
       {"rt-deferred-shuffled", 1500, 200.0f, D, 10, 14, 0, 0, 0, 96, 40, 16, 4, 21, 0, 10.0f,
        0.7f},

@@ -94,6 +94,13 @@ def test_ensemble_runs_join_and_leave():
         assert_same(ens, c, s_c, 25, "c")
         assert_same(ens, d, s_d, 10, "d")
         assert ens.stats()["builds"] >= 3
+        # retiring without an admission keeps the run as a dead replica
+        builds = ens.stats()["builds"]
+        ens.retire(c)
+        ens.run(9)
+        assert ens.stats()["builds"] == builds and ens.stats()["runs"] == 2
+        assert_same(ens, b, s_b, 54, "b after c left")
+        assert_same(ens, d, s_d, 19, "d after c left")
 
 
 def test_ensemble_tuning_does_not_change_results():
