@@ -62,6 +62,16 @@ class Hip:
     def record(self, e, stream):
         self._ok(self.L.hipEventRecord(e, ctypes.c_void_p(stream)), "hipEventRecord")
 
+    def copy_calibration(self, nbytes):
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        self._ok(self.L.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)), "hipMalloc")
+        self._ok(self.L.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)), "hipMalloc")
+        self._ok(self.L.hipMemset(a, 1, ctypes.c_size_t(nbytes)), "hipMemset")
+        self._ok(self.L.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3), "hipMemcpy")
+        self.sync()
+        self.L.hipFree(a)
+        self.L.hipFree(b)
+
     def elapsed_ms(self, a, b):
         self._ok(self.L.hipEventSynchronize(b), "hipEventSynchronize")
         ms = ctypes.c_float()
@@ -87,6 +97,20 @@ def timed_run(hip, f, p, first, steps, binned):
     ms = hip.elapsed_ms(e0, e1)
     hip.sync()
     return ms, time.perf_counter() - t0
+
+
+def profiled_traffic(name="k_rt_step_flockers_1M.md"):
+    """HBM bytes per launch of the tile-resident step, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a 1 GiB
+    copy), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1]), "profiles/" + name
+    except OSError:
+        pass
+    return None, None
 
 
 def run_plain(args, hip):
@@ -297,6 +321,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also copy 1 GiB device to device (a known-size reference for "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -305,6 +332,8 @@ def main():
 
     hip = Hip()
     w, ms, wall, kern, binned_ms, layout = run_plain(args, hip)
+    if args.calibrate_copy:
+        hip.copy_calibration(1 << 30)
     n, steps = w["n"], args.steps
     value = n * steps / (ms * 1e-3)
     # dominant kernel of the timed path: the tile-resident step, or the
@@ -314,6 +343,7 @@ def main():
     else:
         dom, dom_bytes = "flock", BYTES_FLOCK
     dom_us = kern[dom]["avg_us"]
+    traffic, traffic_src = profiled_traffic()
     achieved = dom_bytes * n / (dom_us * 1e-6) if dom_us > 0 else 0.0
     out = {
         "metric": METRIC,
@@ -340,7 +370,8 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": traffic if dom == "rt_step" else None,
+            "traffic_source": traffic_src if dom == "rt_step" else None,
             "bytes_per_agent": dom_bytes,
         },
         "step_roofline": {
