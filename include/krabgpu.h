@@ -51,6 +51,8 @@ typedef struct kg_field_desc {
     int32_t toroidal;     /* Field2D::new(_, _, _, t) */
     uint32_t capacity;    /* maximum agents per buffer */
     uint32_t bin_rows;    /* device bin height in cells; 0 = automatic */
+    uint32_t tile_cols;   /* device tile width in cells (power of two); 0 = automatic */
+    uint32_t tile_rows;   /* device tile height in cells (power of two); 0 = automatic */
 } kg_field_desc;
 
 /* Weights of the flockers step (tests/model/flockers/bird.rs:12-17) and the

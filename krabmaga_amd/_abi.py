@@ -52,7 +52,8 @@ class KgError(RuntimeError):
 class FieldDesc(ctypes.Structure):
     _fields_ = [("width", ctypes.c_float), ("height", ctypes.c_float),
                 ("discretization", ctypes.c_float), ("toroidal", ctypes.c_int32),
-                ("capacity", ctypes.c_uint32), ("bin_rows", ctypes.c_uint32)]
+                ("capacity", ctypes.c_uint32), ("bin_rows", ctypes.c_uint32),
+                ("tile_cols", ctypes.c_uint32), ("tile_rows", ctypes.c_uint32)]
 
 
 class FlockParams(ctypes.Structure):

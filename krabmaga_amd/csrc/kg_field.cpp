@@ -404,8 +404,13 @@ int kg_field_create(const kg_field_desc* desc, kg_field** out) {
   f->desc = *desc;
   f->cap = desc->capacity;
   std::string why;
+  if ((desc->tile_cols & (desc->tile_cols - 1)) || (desc->tile_rows & (desc->tile_rows - 1))) {
+    delete f;
+    return set_error(KG_E_INVALID, "tile_cols and tile_rows must be 0 or powers of two");
+  }
   if (!make_geom_rt(desc->width, desc->height, desc->discretization, desc->toroidal,
-                    desc->capacity, 0, 0, desc->bin_rows, &f->g, &why)) {
+                    desc->capacity, desc->tile_cols, desc->tile_rows, desc->bin_rows, &f->g,
+                    &why)) {
     delete f;
     return set_error(KG_E_INVALID, "%s", why.c_str());
   }

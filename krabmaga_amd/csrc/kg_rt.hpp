@@ -547,6 +547,31 @@ KG_HD int rt_check_from_sq(float sx_nw, float sx_sw, float sy_nw, float sy_ne, f
   return 0;
 }
 
+// One neighbour candidate of query (qx, qy) in bag-scan order: the
+// check_circle of its cell (computed when the cell row changes, from the
+// corner terms when `pre`), then the distance test and the accumulation.
+KG_HD void rt_visit(const Geom& g, int32_t i, float T, bool pre, int32_t min_j, float sxn,
+                    float sxs, float syn0, float syn1, float syn2, float sye0, float sye1,
+                    float sye2, float qx, float qy, uint32_t qid, float ex_, float ey_, float ehx_,
+                    float ehy_, uint32_t eid_, int32_t ecy_, int32_t* cur, int* chk,
+                    FlockAcc* acc) {
+  if (ecy_ != *cur) {
+    *cur = ecy_;
+    if (pre) {
+      const int k = ecy_ - min_j;
+      const float yn = k == 0 ? syn0 : (k == 1 ? syn1 : syn2);
+      const float ye = k == 0 ? sye0 : (k == 1 ? sye1 : sye2);
+      *chk = rt_check_from_sq(sxn, sxs, yn, ye, T);
+    } else {
+      *chk = check_circle_sq(g, i, ecy_, qx, qy, T);
+    }
+  }
+  if (*chk == -1) return;
+  const float dx = toroidal_distance(qx, ex_, g.w);
+  const float dy = toroidal_distance(qy, ey_, g.h);
+  if (*chk == 1 || dx * dx + dy * dy <= T) acc->add(eid_ == qid, dx, dy, ehx_, ehy_);
+}
+
 // Bird::step for own agent number o: neighbours come from the staged runs
 // (the row table gives the window rows' range of each column) merged with
 // the column's sorted extras; the new state goes to result slot o.  For
@@ -575,13 +600,18 @@ KG_HD void rt_agent_lds(const RepDev& rp, const RtTile& rt, const RtSegs& sg, ui
   const int32_t x_lo = rt_seg_bin(g, rt, sg, min_j), x_hi = rt_seg_bin(g, rt, sg, max_j) + 1;
   // corner rows of the window's cells: nw_y = j * d, ne_y = min(nw_y + d, h)
   const bool pre = max_j - min_j < 3;
-  float syn[3], sye[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const float nwy = (float)(min_j + k) * g.d;
-    const float ney = fminf(nwy + g.d, g.h);
-    syn[k] = pre ? rt_corner_sq(nwy, qy, g.h) : 0.0f;
-    sye[k] = pre ? rt_corner_sq(ney, qy, g.h) : 0.0f;
+  // scalars rather than a per-lane indexed array (which would live in
+  // scratch memory)
+  float syn0 = 0.0f, syn1 = 0.0f, syn2 = 0.0f, sye0 = 0.0f, sye1 = 0.0f, sye2 = 0.0f;
+  if (pre) {
+    const float nwy0 = (float)min_j * g.d, nwy1 = (float)(min_j + 1) * g.d;
+    const float nwy2 = (float)(min_j + 2) * g.d;
+    syn0 = rt_corner_sq(nwy0, qy, g.h);
+    syn1 = rt_corner_sq(nwy1, qy, g.h);
+    syn2 = rt_corner_sq(nwy2, qy, g.h);
+    sye0 = rt_corner_sq(fminf(nwy0 + g.d, g.h), qy, g.h);
+    sye1 = rt_corner_sq(fminf(nwy1 + g.d, g.h), qy, g.h);
+    sye2 = rt_corner_sq(fminf(nwy2 + g.d, g.h), qy, g.h);
   }
   FlockAcc acc;
   for (int32_t i = min_i; i <= max_i; ++i) {
@@ -598,6 +628,19 @@ KG_HD void rt_agent_lds(const RepDev& rp, const RtTile& rt, const RtSegs& sg, ui
     const float sxn = rt_corner_sq(nwx, qx, g.w), sxs = rt_corner_sq(swx, qx, g.w);
     int32_t cur = -1;
     int chk = 0;
+    if (!(b < b_end && S.ecy[S.eperm[b]] <= max_j)) {
+      // no extras in this column's window rows (the common case): the
+      // stayers alone, each item's fields loaded together
+      for (; a < a_end; ++a) {
+        const int32_t ecy_ = S.scy[a];
+        const float ex_ = S.sx[a], ey_ = S.sy[a], ehx_ = S.shx[a], ehy_ = S.shy[a];
+        const uint32_t eid_ = S.sid[a];
+        if (ecy_ > max_j) break;
+        rt_visit(g, i, T, pre, min_j, sxn, sxs, syn0, syn1, syn2, sye0, sye1, sye2, qx, qy, qid,
+                 ex_, ey_, ehx_, ehy_, eid_, ecy_, &cur, &chk, &acc);
+      }
+      continue;
+    }
     for (;;) {
       const bool ha = a < a_end && S.scy[a] <= max_j;
       uint32_t e = 0;
@@ -622,21 +665,8 @@ KG_HD void rt_agent_lds(const RepDev& rp, const RtTile& rt, const RtSegs& sg, ui
       const int32_t ecy_ = pcy[src];
       a += take_a ? 1u : 0u;
       b += take_a ? 0u : 1u;
-      if (ecy_ != cur) {
-        cur = ecy_;
-        if (pre) {
-          const int k = ecy_ - min_j;
-          const float yn = k == 0 ? syn[0] : (k == 1 ? syn[1] : syn[2]);
-          const float ye = k == 0 ? sye[0] : (k == 1 ? sye[1] : sye[2]);
-          chk = rt_check_from_sq(sxn, sxs, yn, ye, T);
-        } else {
-          chk = check_circle_sq(g, i, ecy_, qx, qy, T);
-        }
-      }
-      if (chk == -1) continue;
-      const float dx = toroidal_distance(qx, ex_, g.w);
-      const float dy = toroidal_distance(qy, ey_, g.h);
-      if (chk == 1 || dx * dx + dy * dy <= T) acc.add(eid_ == qid, dx, dy, ehx_, ehy_);
+      rt_visit(g, i, T, pre, min_j, sxn, sxs, syn0, syn1, syn2, sye0, sye1, sye2, qx, qy, qid,
+               ex_, ey_, ehx_, ehy_, eid_, ecy_, &cur, &chk, &acc);
     }
   }
   float nx, ny, nhx, nhy;

@@ -33,14 +33,16 @@ class FlockParams:
 class Field2D:
     """Field2D::new(w, h, d, toroidal) living in HBM (field_2d.rs:304-322)."""
 
-    def __init__(self, width, height, discretization, toroidal, capacity, bin_rows=0):
+    def __init__(self, width, height, discretization, toroidal, capacity, bin_rows=0,
+                 tile_cols=0, tile_rows=0):
         self._L = _abi.lib()
         self.width = float(width)
         self.height = float(height)
         self.discretization = float(discretization)
         self.toroidal = bool(toroidal)
         self.capacity = int(capacity)
-        desc = _abi.FieldDesc(width, height, discretization, int(bool(toroidal)), capacity, bin_rows)
+        desc = _abi.FieldDesc(width, height, discretization, int(bool(toroidal)), capacity, bin_rows,
+                              tile_cols, tile_rows)
         h = ctypes.c_void_p()
         check(self._L.kg_field_create(ctypes.byref(desc), ctypes.byref(h)))
         self._h = h

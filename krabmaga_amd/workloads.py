@@ -62,3 +62,28 @@ def serving(rate=200.0, duration=1.0, seed=4, n_min=1_000, n_max=1_000_000, s_mi
         out.append(Request(t, _run(rng, n, seed, i), steps))
         i += 1
     return out
+
+
+def clustered(n=8_388_608, side=40_960.0, blobs=64, sigma=200.0, uniform_frac=0.2, seed=2):
+    """Positions of the clustered-density workload: a fraction `uniform_frac`
+    uniform on the torus, the rest drawn from `blobs` Gaussian blobs (centres
+    uniform, standard deviation `sigma`) wrapped onto the torus with
+    toroidal_transform; headings unit vectors at uniform angles.  Returns
+    float32 arrays (x, y, hx, hy)."""
+    rng = np.random.default_rng(seed)
+    nu = int(round(n * uniform_frac))
+    nb = n - nu
+    centres = rng.uniform(0.0, side, size=(blobs, 2))
+    which = rng.integers(0, blobs, size=nb)
+    pts = centres[which] + rng.normal(0.0, sigma, size=(nb, 2))
+    pts = np.concatenate([pts, rng.uniform(0.0, side, size=(nu, 2))])
+    rng.shuffle(pts)
+    pts = np.mod(pts, side)
+    x = pts[:, 0].astype(np.float32)
+    y = pts[:, 1].astype(np.float32)
+    # float64 -> float32 rounding can land exactly on `side`; wrap it as
+    # toroidal_transform does (field_2d.rs:1004-1014)
+    x[x >= np.float32(side)] -= np.float32(side)
+    y[y >= np.float32(side)] -= np.float32(side)
+    a = rng.uniform(0.0, 2.0 * np.pi, size=n)
+    return x, y, np.cos(a).astype(np.float32), np.sin(a).astype(np.float32)
