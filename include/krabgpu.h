@@ -177,7 +177,8 @@ void *kg_field_stream(kg_field *f);
 #define KG_KERNEL_SCATTER_APPEND 4 /* binning of set_object_location agents */
 #define KG_KERNEL_RT_STEP 5        /* tile-resident schedule step */
 #define KG_KERNEL_RT_LOAD 6        /* entry into the tile-resident layout */
-#define KG_KERNEL_COUNT 7
+#define KG_KERNEL_SORT_BINS 7      /* (cell y, id) order inside every bin */
+#define KG_KERNEL_COUNT 8
 int kg_field_profile(kg_field *f, int enable);
 int kg_field_profile_read(kg_field *f, int kernel, double *total_ms, uint64_t *launches);
 

@@ -40,7 +40,7 @@ VOID_FUNCS = ("kg_field_destroy", "kg_field_stream", "kg_last_error", "kg_ensemb
               "kg_ensemble_stream")
 
 KERNELS = {"flock": 0, "scan": 1, "scatter": 2, "count_append": 3, "scatter_append": 4,
-           "rt_step": 5, "rt_load": 6}
+           "rt_step": 5, "rt_load": 6, "sort_bins": 7}
 
 
 class KgError(RuntimeError):

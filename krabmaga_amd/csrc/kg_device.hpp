@@ -190,33 +190,20 @@ KG_HD void query_global(const Geom& g, const Buf& R, const uint32_t* __restrict_
       const int32_t by = t_transform((int32_t)j, g.max_y);
       const int check = relax ? 1 : check_circle_sq(g, bx, by, x, y, T);
       if (check == -1) continue;
+      // a bin holds rows of one column sorted by (cell y, id): the bag is
+      // the run of cell y == by, in ascending id
       const uint32_t b = bin_of(g, bx, by);
-      const uint32_t s = offset[b], e = offset[b + 1];
-      // ascending-id emission of this bag's members
-      int64_t last = -1;
-      for (;;) {
-        uint32_t best = 0xFFFFFFFFu;
-        bool any = false;
-        for (uint32_t q = s; q < e; ++q) {
-          const uint32_t qid = R.id[q];
-          if ((int64_t)qid <= last) continue;
-          if (any && qid >= best) continue;
-          int32_t qx, qy;
-          discretize(g, R.x[q], R.y[q], &qx, &qy);
-          if (qx != bx || qy != by) continue;
-          best = qid;
-          any = true;
-        }
-        if (!any) break;
-        for (uint32_t q = s; q < e; ++q) {
-          if (R.id[q] != best) continue;
-          const float ex = R.x[q], ey = R.y[q];
-          int32_t qx, qy;
-          discretize(g, ex, ey, &qx, &qy);
-          if (qx != bx || qy != by) continue;
-          if (check == 1 || dist2(x, y, ex, ey, g.w, g.h, tor) <= T) visit(q);
-        }
-        last = best;
+      uint32_t lo = offset[b], hi = offset[b + 1];
+      const uint32_t e = hi;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (f2i(floorf(R.y[mid] / g.d)) < by) lo = mid + 1;
+        else hi = mid;
+      }
+      for (uint32_t q = lo; q < e; ++q) {
+        const float ex = R.x[q], ey = R.y[q];
+        if (f2i(floorf(ey / g.d)) != by) break;
+        if (check == 1 || dist2(x, y, ex, ey, g.w, g.h, tor) <= T) visit(q);
       }
     }
   }

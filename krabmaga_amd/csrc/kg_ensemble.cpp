@@ -333,19 +333,16 @@ int rebuild(kg_ensemble* e) {
   if (int rc = e->rt.setup(plan, cls, e->nagents, e->stream)) return rc;
   if (int rc = e->rt.reset(G, e->stream)) return rc;
   const RtDev D = e->rt.dev();
-  for (int c = 0; c < 2; ++c)
-    KG_HIP(launch_rt_load(w, D, e->rt.caps[c], c, e->rt.dense, e->rt.ndense, e->R.b, e->offset,
-                          G - 1, e->stream));
+  for (int c = 0; c < e->rt.ncls; ++c)
+    KG_HIP(launch_rt_load(w, D, e->rt.caps[c], c, e->rt.dense + e->rt.off[c],
+                          e->rt.off[c + 1] - e->rt.off[c], e->R.b, e->offset, G - 1, e->stream));
   for (EnsRun& r : e->runs) r.fresh = false;
   ++e->builds;
   return KG_OK;
 }
 
 hipError_t launch_step(kg_ensemble* e, const WorkList& w, const RtDev& D, uint32_t koff) {
-  hipError_t err = launch_rt_step(w, D, e->rt.caps[0], koff, 0, nullptr, 0, e->stream);
-  if (err == hipSuccess)
-    err = launch_rt_step(w, D, e->rt.caps[1], koff, 1, e->rt.dense, e->rt.ndense, e->stream);
-  return err;
+  return rt_launch_classes(w, D, e->rt, koff, e->stream);
 }
 
 // `n` schedule steps of every run (graph replays of 8 steps when allowed).

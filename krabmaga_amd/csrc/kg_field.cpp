@@ -213,6 +213,11 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
     ProfScope ps(f, KG_KERNEL_SCATTER_APPEND);
     KG_HIP(launch_scatter_append(wl, f->A.b, f->cursor, f->R2.b, f->stream));
   }
+  {
+    // the write buffer has been consumed: it is the sort's scratch
+    ProfScope ps(f, KG_KERNEL_SORT_BINS);
+    KG_HIP(launch_sort_bins(wl, f->R2.b, f->offset2, f->W.b, f->stream));
+  }
   std::swap(f->R, f->R2);
   std::swap(f->offset, f->offset2);
   f->parity ^= 1;
@@ -307,9 +312,10 @@ int rt_enter(kg_field* f) {
   {
     ProfScope ps(f, KG_KERNEL_RT_LOAD);
     const RtDev D = f->rt.dev();
-    for (int c = 0; c < 2; ++c)
-      KG_HIP(launch_rt_load(f->worklist(), D, f->rt.caps[c], c, f->rt.dense, f->rt.ndense,
-                            f->R.b, f->offset, f->rt.G - 1, f->stream));
+    for (int c = 0; c < f->rt.ncls; ++c)
+      KG_HIP(launch_rt_load(f->worklist(), D, f->rt.caps[c], c, f->rt.dense + f->rt.off[c],
+                            f->rt.off[c + 1] - f->rt.off[c], f->R.b, f->offset, f->rt.G - 1,
+                            f->stream));
   }
   f->rt_mode = true;
   f->rt_replan = false;
@@ -333,14 +339,9 @@ int ensure_atomic(kg_field* f) {
   return lazy_update_impl(f, nullptr);
 }
 
-// One tile-resident schedule step: the normal-class launch, then the
-// dense-class launch (the plan's dense tiles and any deferred normal tiles).
+// One tile-resident schedule step: one launch per tile class.
 hipError_t rt_launch_step(kg_field* f, const WorkList& wl, const RtDev& D, uint32_t koff) {
-  hipError_t e = launch_rt_step(wl, D, f->rt.caps[0], koff, 0, nullptr, 0, f->stream);
-  // the dense launch also runs the normal tiles deferred by the first
-  if (e == hipSuccess)
-    e = launch_rt_step(wl, D, f->rt.caps[1], koff, 1, f->rt.dense, f->rt.ndense, f->stream);
-  return e;
+  return rt_launch_classes(wl, D, f->rt, koff, f->stream);
 }
 
 // `nsteps` tile-resident schedule steps (graph replays of 8 steps when

@@ -250,6 +250,9 @@ RtCaps rt_caps_for(const std::vector<RtRepInfo>& reps, uint32_t own, uint32_t st
   // extras per step: agents that crossed into the staged rectangle, a few
   // percent of the tile's count; a quarter of the own capacity is ample
   c.extras = (int)std::min<uint32_t>(2048u, std::max<uint32_t>(64u, round_up(0.25 * own, 64)));
+  // one own bird per thread up to 1024: a launch lasts as long as its
+  // fullest tile, so larger classes get larger work-groups
+  c.threads = c.own <= 512 ? 256 : (c.own <= 1024 ? 512 : 1024);
   return c;
 }
 
@@ -260,13 +263,15 @@ void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts,
   std::vector<uint32_t> own, staged;
   rt_needs(*plan, counts, reps, &own, &staged);
   out->dense.clear();
+  std::fill(out->off, out->off + kRtMaxClasses + 1, 0u);
   out->fits = true;
   if (nt == 0) {
+    out->ncls = 2;
     out->caps[0] = out->caps[1] = rt_caps_for(reps, 64, 64);
     return;
   }
-  // normal class: the largest own-count quantile whose capacities fit the
-  // normal LDS budget; the tiles above it are dense
+  // class 0: the largest own-count quantile whose capacities fit the normal
+  // LDS budget
   std::vector<uint32_t> so(own);
   std::sort(so.begin(), so.end());
   RtCaps cn{};
@@ -278,34 +283,58 @@ void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts,
     cn = rt_caps_for(reps, own_n, st_n);
     if (rt_smem_bytes(cn) <= lds_normal) break;
   }
-  uint32_t own_d = 0, st_d = 0;
+  uint32_t own_max = 0, st_max = 0;
   for (uint32_t t = 0; t < nt; ++t) {
-    const bool dense = own[t] > (uint32_t)cn.own || staged[t] > (uint32_t)cn.runs;
-    plan->meta[t].cls = dense ? 1u : 0u;
-    if (dense) {
-      out->dense.push_back(t);
-      own_d = std::max(own_d, own[t]);
-      st_d = std::max(st_d, staged[t]);
-    }
+    own_max = std::max(own_max, own[t]);
+    st_max = std::max(st_max, staged[t]);
   }
-  // the dense launch also takes normal tiles that outgrow their capacities
-  // before the next plan, whose extras may be most of their agents
-  // (twice the normal capacities, or less where LDS cannot hold that)
-  uint32_t own_x = std::max<uint32_t>(own_d, 2u * (uint32_t)cn.own);
-  uint32_t st_x = std::max<uint32_t>(st_d, 2u * (uint32_t)cn.runs);
-  RtCaps cd;
-  for (;;) {
-    cd = rt_caps_for(reps, own_x, st_x);
-    cd.extras = std::min(2048, std::max(cd.extras, cd.own));
-    if (rt_smem_bytes(cd) <= lds_limit || (own_x <= own_d && st_x <= st_d)) break;
-    const uint32_t so = std::max<uint32_t>(64u, own_x / 8u), ss = std::max<uint32_t>(64u, st_x / 8u);
-    own_x = own_x > own_d + so ? own_x - so : own_d;
-    st_x = st_x > st_d + ss ? st_x - ss : st_d;
-  }
+  // middle classes: capacities 1.6x the previous class's, while they leave
+  // tiles uncovered and fit the LDS limit
+  int nc = 1;
   out->caps[0] = cn;
-  out->caps[1] = cd;
-  out->fits = rt_smem_bytes(cn) <= lds_limit && rt_smem_bytes(cd) <= lds_limit &&
-              cn.nb <= kRtMaxBins && cn.cols <= kMaxStageCols;
+  double f = 1.6;
+  while (nc < kRtMaxClasses - 1) {
+    const RtCaps& prev = out->caps[nc - 1];
+    if ((uint32_t)prev.own >= own_max && (uint32_t)prev.runs >= st_max) break;
+    const RtCaps c = rt_caps_for(reps, (uint32_t)(cn.own * f), (uint32_t)(cn.runs * f));
+    if (rt_smem_bytes(c) > lds_limit) break;
+    out->caps[nc++] = c;
+    f *= 1.6;
+  }
+  // the last class: the fullest tile, and room for tiles that outgrow class
+  // 0 (twice its capacities, or less where LDS cannot hold that), whose
+  // extras may be most of their agents
+  uint32_t own_x = std::max<uint32_t>(own_max, 2u * (uint32_t)cn.own);
+  uint32_t st_x = std::max<uint32_t>(st_max, 2u * (uint32_t)cn.runs);
+  RtCaps cl;
+  for (;;) {
+    cl = rt_caps_for(reps, own_x, st_x);
+    cl.extras = std::min(2048, std::max(cl.extras, cl.own));
+    if (rt_smem_bytes(cl) <= lds_limit || (own_x <= own_max && st_x <= st_max)) break;
+    const uint32_t so2 = std::max<uint32_t>(64u, own_x / 8u), ss2 = std::max<uint32_t>(64u, st_x / 8u);
+    own_x = own_x > own_max + so2 ? own_x - so2 : own_max;
+    st_x = st_x > st_max + ss2 ? st_x - ss2 : st_max;
+  }
+  out->caps[nc++] = cl;
+  out->ncls = nc;
+  // each tile in the smallest class that holds it
+  std::vector<std::vector<uint32_t>> lists(nc);
+  for (uint32_t t = 0; t < nt; ++t) {
+    int k = 0;
+    while (k < nc - 1 && (own[t] > (uint32_t)out->caps[k].own ||
+                          staged[t] > (uint32_t)out->caps[k].runs))
+      ++k;
+    plan->meta[t].cls = (uint32_t)k;
+    if (k > 0) lists[k].push_back(t);
+  }
+  for (int k = 1; k < nc; ++k) {
+    out->off[k] = (uint32_t)out->dense.size();
+    out->dense.insert(out->dense.end(), lists[k].begin(), lists[k].end());
+  }
+  out->off[nc] = (uint32_t)out->dense.size();
+  for (int k = 0; k < nc; ++k)
+    if (rt_smem_bytes(out->caps[k]) > lds_limit) out->fits = false;
+  out->fits = out->fits && cn.nb <= kRtMaxBins && cn.cols <= kMaxStageCols;
 }
 
 bool make_geom_rt(float w, float h, float d, int tor, uint64_t expected_agents, uint32_t tx,

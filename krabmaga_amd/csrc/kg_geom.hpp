@@ -62,12 +62,16 @@ void rt_needs(const RtPlan& plan, const std::vector<uint32_t>& counts,
 // geometry.
 RtCaps rt_caps_for(const std::vector<RtRepInfo>& reps, uint32_t own, uint32_t staged);
 
-// Splits the tiles into a normal class, whose capacities fit `lds_normal`
-// bytes of LDS per work-group, and a dense class (sets plan->meta[].cls).
-// fits = both classes fit `lds_limit`.
+// Splits the tiles into classes (sets plan->meta[].cls): class 0's
+// capacities fit `lds_normal` bytes of LDS per work-group, further classes
+// grow geometrically to the fullest tile, and the last one also has room for
+// tiles that outgrow class 0 before the next plan.  fits = every class fits
+// `lds_limit`.
 struct RtClassPlan {
-  RtCaps caps[2];
-  std::vector<uint32_t> dense;  // tiles of the dense class
+  int ncls = 2;
+  RtCaps caps[kRtMaxClasses];
+  std::vector<uint32_t> dense;        // tiles of classes 1.., grouped by class
+  uint32_t off[kRtMaxClasses + 1] = {};  // class k >= 1: dense[off[k] .. off[k + 1])
   bool fits = false;
 };
 void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts,

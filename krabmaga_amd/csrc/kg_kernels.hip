@@ -245,10 +245,10 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
   }
   if (use_lds) {
     const FlockSmem S = flock_carve(smem_raw, lds_cap);
-    for (uint32_t k = t; k < L; k += kFlockThreads)
+    for (uint32_t k = t; k < L; k += kFlockThreads) {
       flock_stage(rp, ft, k, R, s_seg_start, s_seg_len, s_col_base, S);
-    __syncthreads();
-    for (uint32_t k = t; k < L; k += kFlockThreads) flock_rank(ft, k, s_col_base, S, &s_nown);
+      flock_own(ft, k, S, &s_nown);
+    }
     __syncthreads();
     const uint32_t nown = s_nown;
     for (uint32_t o = t; o < nown; o += kFlockThreads)
@@ -268,6 +268,134 @@ hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W
   hipLaunchKernelGGL(k_flock, dim3(wl.ntiles), dim3(kFlockThreads), flock_lds_bytes(lds_cap), s,
                      wl.reps, wl.nreps, wl.tile_prefix, R, offset, W, count_next, gstep, err,
                      lds_cap);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// bin sort: every bin of a freshly scattered read buffer into (cell y, id)
+// order, ties by position; one block per (replica, tile).  A bin of at most
+// one wave's width is ranked by one wave in registers and rewritten in place
+// (every lane has loaded its entry before any lane stores); larger bins are
+// ranked by the whole block against keys staged in LDS, written to the
+// scratch buffer T at the same indices and copied back.
+
+constexpr int kSortThreads = 256;
+constexpr uint32_t kSortChunk = 2048;  // keys staged per pass for the large bins
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t lane) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_bins(
+    const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
+    const uint32_t* __restrict__ offset, Buf T) {
+  __shared__ uint64_t s_key[kSortChunk];
+  __shared__ uint32_t s_big[kSortThreads];
+  __shared__ uint32_t s_nbig;
+  const int t = threadIdx.x;
+  const uint32_t lane = (uint32_t)t & (kWave - 1);
+  const uint32_t wid = (uint32_t)t / kWave;
+  const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+  const int r = find_rep(tile_prefix, nreps, item);
+  const Geom g = reps[r].g;
+  const uint32_t nb = 1u << (g.tx_log2 + g.tr_log2);
+  const uint32_t b0 = reps[r].bin_base + ((item - tile_prefix[r]) << (g.tx_log2 + g.tr_log2));
+
+  // bins of up to 64 entries: one wave each
+  for (uint32_t k = wid; k < nb; k += kSortThreads / kWave) {
+    const uint32_t s = offset[b0 + k];
+    const uint32_t m = offset[b0 + k + 1] - s;
+    if (m <= 1u || m > (uint32_t)kWave) continue;
+    const bool act = lane < m;
+    uint32_t id = 0;
+    float x = 0.0f, y = 0.0f, hx = 0.0f, hy = 0.0f;
+    if (act) {
+      id = R.id[s + lane];
+      x = R.x[s + lane];
+      y = R.y[s + lane];
+      hx = R.hx[s + lane];
+      hy = R.hy[s + lane];
+    }
+    const uint64_t key = act ? bin_key(g, y, id) : ~0ull;
+    uint32_t rank = 0;
+    for (uint32_t j = 0; j < m; ++j) {
+      const uint64_t kj = readlane64(key, j);
+      rank += (kj < key) || (kj == key && j < lane);
+    }
+    if (act) {
+      R.id[s + rank] = id;
+      R.x[s + rank] = x;
+      R.y[s + rank] = y;
+      R.hx[s + rank] = hx;
+      R.hy[s + rank] = hy;
+    }
+  }
+
+  // larger bins: the whole block, a batch of up to 256 bins at a time
+  for (uint32_t k0 = 0; k0 < nb; k0 += kSortThreads) {
+    if (t == 0) s_nbig = 0u;
+    __syncthreads();
+    if (k0 + (uint32_t)t < nb) {
+      const uint32_t b = b0 + k0 + (uint32_t)t;
+      if (offset[b + 1] - offset[b] > (uint32_t)kWave) s_big[atomicAdd(&s_nbig, 1u)] = b;
+    }
+    __syncthreads();
+    const uint32_t nbig = s_nbig;
+    for (uint32_t i = 0; i < nbig; ++i) {
+      const uint32_t s = offset[s_big[i]];
+      const uint32_t m = offset[s_big[i] + 1] - s;
+      const bool staged_once = m <= kSortChunk;
+      if (staged_once) {
+        for (uint32_t j = t; j < m; j += kSortThreads) s_key[j] = bin_key(g, R.y[s + j], R.id[s + j]);
+        __syncthreads();
+      }
+      for (uint32_t e0 = 0; e0 < m; e0 += kSortThreads) {
+        const uint32_t e = e0 + (uint32_t)t;
+        const bool act = e < m;
+        const uint64_t key = act ? bin_key(g, R.y[s + e], R.id[s + e]) : 0ull;
+        uint32_t rank = 0;
+        for (uint32_t c0 = 0; c0 < m; c0 += kSortChunk) {
+          const uint32_t n = m - c0 < kSortChunk ? m - c0 : kSortChunk;
+          if (!staged_once) {
+            __syncthreads();
+            for (uint32_t j = t; j < n; j += kSortThreads)
+              s_key[j] = bin_key(g, R.y[s + c0 + j], R.id[s + c0 + j]);
+            __syncthreads();
+          }
+          if (act)
+            for (uint32_t j = 0; j < n; ++j) {
+              const uint64_t kj = s_key[j];
+              rank += (kj < key) || (kj == key && c0 + j < e);
+            }
+        }
+        if (act) {
+          T.id[s + rank] = R.id[s + e];
+          T.x[s + rank] = R.x[s + e];
+          T.y[s + rank] = R.y[s + e];
+          T.hx[s + rank] = R.hx[s + e];
+          T.hy[s + rank] = R.hy[s + e];
+        }
+      }
+      __syncthreads();
+      for (uint32_t j = t; j < m; j += kSortThreads) {
+        R.id[s + j] = T.id[s + j];
+        R.x[s + j] = T.x[s + j];
+        R.y[s + j] = T.y[s + j];
+        R.hx[s + j] = T.hx[s + j];
+        R.hy[s + j] = T.hy[s + j];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+hipError_t launch_sort_bins(const WorkList& wl, Buf R, const uint32_t* offset, Buf T,
+                            hipStream_t s) {
+  if (wl.ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_bins, dim3(wl.ntiles), dim3(kSortThreads), 0, s, wl.reps, wl.nreps,
+                     wl.tile_prefix, R, offset, T);
   return hipGetLastError();
 }
 
@@ -510,6 +638,7 @@ size_t rt_lds_bytes(const RtCaps& c) { return rt_smem_bytes(c); }
 // Output phase of a tile (after its stayers are classified and counted per
 // bin in LDS): sort the stayers by (bin, cell y, id), write them and the new
 // bin offsets.
+template <int kT>
 __device__ __forceinline__ void rt_output(const RepDev& rp, const RtTile& rt, const RtDev& D,
                                           const RtMeta& me, uint32_t bo_own, int nb_tile,
                                           uint32_t nstay, RtSmem S, uint32_t* s_warp) {
@@ -524,7 +653,7 @@ __device__ __forceinline__ void rt_output(const RepDev& rp, const RtTile& rt, co
     sum += v[k];
   }
   uint32_t tot;
-  uint32_t run = block_exclusive_scan<kRtThreads>(sum, s_warp, &tot);
+  uint32_t run = block_exclusive_scan<kT>(sum, s_warp, &tot);
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int b = 4 * t + k;
@@ -536,30 +665,34 @@ __device__ __forceinline__ void rt_output(const RepDev& rp, const RtTile& rt, co
   }
   if (t == 0) S.bbase[nb_tile] = nstay;
   __syncthreads();
-  for (uint32_t s = t; s < nstay; s += kRtThreads) {
+  for (uint32_t s = t; s < nstay; s += kT) {
     const uint32_t pos = atomicAdd(&S.bcur[S.obin[s]], 1u);
     S.operm[pos] = (uint16_t)s;
   }
   __syncthreads();
   const uint32_t scap = me.scap;
-  for (uint32_t s = t; s < nstay; s += kRtThreads) rt_write_stayer(rp, rt, D, me, s, S);
-  for (int b = t; b <= nb_tile; b += kRtThreads)
+  for (uint32_t s = t; s < nstay; s += kT) rt_write_stayer(rp, rt, D, me, s, S);
+  for (int b = t; b <= nb_tile; b += kT)
     D.boff[pout][bo_own + b] = S.bbase[b] < scap ? S.bbase[b] : scap;
   if (t == 0) D.cnt[pout][rt.gt] = nstay < scap ? nstay : scap;
 }
 
 // One schedule step of the tiles of class `cls`: cls 0 covers every tile
-// (dense tiles return at once), cls 1 the tiles in `list`.
-__global__ __launch_bounds__(kRtThreads) void k_rt_step(
+// (other classes' tiles return at once), further classes the tiles in
+// `list`.  `mode` is kRtModeClass (tiles that outgrow the capacities defer),
+// kRtModeFinal (they take the global path) or kRtModeDeferred (block i runs
+// the i-th tile deferred in this step, with `caps` of the last class).
+template <int kT>
+__global__ __launch_bounds__(kT) void k_rt_step(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, RtDev D,
-    RtCaps caps, uint32_t koff, int cls, const uint32_t* __restrict__ list, uint32_t nlist) {
+    RtCaps caps, uint32_t koff, int cls, const uint32_t* __restrict__ list, int mode) {
   extern __shared__ __align__(16) uint8_t smem_raw[];
   __shared__ uint32_t s_seg_start[3 * kMaxStageCols];
   __shared__ uint32_t s_seg_len[3 * kMaxStageCols];
   __shared__ uint32_t s_col_base[kMaxStageCols + 1];
   __shared__ uint32_t s_src_base[10];
   __shared__ uint32_t s_src_pre[11];
-  __shared__ uint32_t s_warp[kRtThreads / kWave];
+  __shared__ uint32_t s_warp[kT / kWave];
   __shared__ uint32_t s_ne, s_nown, s_nstay, s_fail, s_defer;
 
   const int t = threadIdx.x;
@@ -567,23 +700,18 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_step(
   const uint64_t g = *D.gstep + koff;
   const int par = (int)(g & 1u);
   uint32_t item;
-  bool deferred = false;
-  if (cls == 0) {
-    item = xcd_remap(blockIdx.x, gridDim.x);
-  } else if (blockIdx.x < nlist) {
-    item = list[blockIdx.x];
-  } else {
-    // deferred normal tiles of this step; block nlist clears the list that
-    // the next step's normal launch fills
+  const bool deferred = mode == kRtModeDeferred;
+  if (deferred) {
+    // block 0 clears the list that the next step's class launches fill
     const uint32_t n = D.ovf[par][0];
-    if (blockIdx.x == nlist && t == 0) {
+    if (blockIdx.x == 0 && t == 0) {
       D.ovf[par ^ 1][0] = 0u;
       atomicMax(&D.flags[1], n);
     }
-    const uint32_t i = blockIdx.x - nlist;
-    if (i >= n || i >= kRtOvfMax) return;
-    item = D.ovf[par][1 + i];
-    deferred = true;
+    if (blockIdx.x >= n || blockIdx.x >= kRtOvfMax) return;
+    item = D.ovf[par][1 + blockIdx.x];
+  } else {
+    item = cls ? list[blockIdx.x] : xcd_remap(blockIdx.x, gridDim.x);
   }
   const RtTile rt = rt_tile(reps, nreps, tile_prefix, item, g);
   const RtMeta me = D.meta[rt.gt];
@@ -610,11 +738,11 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_step(
 
   const bool lds_ok = rp.lds_ok != 0 && rt_fits_tables(rt, sg, caps) && nb_tile <= caps.nb;
   if (lds_ok)
-    for (int b = t; b <= nb_tile; b += kRtThreads) S.hist[b] = 0u;
+    for (int b = t; b <= nb_tile; b += kT) S.hist[b] = 0u;
   uint32_t len = 0;
   // waves 0-2: the staged columns' runs; wave 3: the extras' sources
   if (lds_ok && t < rt.ncols) len = rt_segments(rp, rt, D, t, s_seg_start, s_seg_len);
-  const int ks = t - (kRtThreads - kWave);
+  const int ks = t - (kT - kWave);
   if (ks >= 0 && ks < 10) {
     uint32_t b;
     const uint32_t n = rt_extra_count(rp, rt, D, ks, &b);
@@ -622,7 +750,7 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_step(
     s_src_pre[ks + 1] = n;
   }
   uint32_t total;
-  const uint32_t ex = block_exclusive_scan<kRtThreads>(len, s_warp, &total);
+  const uint32_t ex = block_exclusive_scan<kT>(len, s_warp, &total);
   if (lds_ok && t < rt.ncols) s_col_base[t] = ex;
   if (t == 0) {
     s_col_base[rt.ncols] = total;
@@ -634,25 +762,25 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_step(
   uint32_t nown = 0, ne = 0;
   if (!global_path) {
     const uint32_t next = s_src_pre[10];
-    if (!rt_fetch(rp, rt, D, t, t + kRtThreads, total, t, next, s_seg_start, s_seg_len, s_col_base,
+    if (!rt_fetch(rp, rt, D, t, t + kT, total, t, next, s_seg_start, s_seg_len, s_col_base,
                   s_src_base, s_src_pre, caps, &s_ne, &s_nown, S))
       s_fail = 1u;
-    for (uint32_t k = t + 2 * kRtThreads; k < total; k += kRtThreads)
+    for (uint32_t k = t + 2 * kT; k < total; k += kT)
       rt_stage(rp, rt, D, k, s_seg_start, s_seg_len, s_col_base, caps.own, &s_nown, S);
-    for (uint32_t j = t + kRtThreads; j < next; j += kRtThreads)
+    for (uint32_t j = t + kT; j < next; j += kT)
       if (!rt_gather_extra(rp, rt, D, j, s_src_base, s_src_pre, caps.extras, &s_ne, caps.own,
                            &s_nown, S))
         s_fail = 1u;
     const uint32_t nbt = (uint32_t)(rt.ncols * sg.nbc);
-    for (uint32_t q = t; q < nbt; q += kRtThreads) rt_bintab(rp, rt, D, sg, q, s_seg_len, S);
+    for (uint32_t q = t; q < nbt; q += kT) rt_bintab(rp, rt, D, sg, q, s_seg_len, S);
     __syncthreads();
     nown = s_nown;
     ne = s_ne;
     global_path = s_fail != 0 || nown > (uint32_t)caps.own;
   }
-  if (global_path && cls == 0 && lds_ok) {
-    // outgrew the normal capacities: hand the tile to the dense launch
-    // (nothing of it has been written yet)
+  if (global_path && mode == kRtModeClass && lds_ok) {
+    // outgrew its class's capacities: hand the tile to the step's launch
+    // for deferred tiles (nothing of it has been written yet)
     if (t == 0) {
       const uint32_t slot = atomicAdd(&D.ovf[par][0], 1u);
       if (slot < kRtOvfMax) {
@@ -666,19 +794,19 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_step(
   }
   if (!global_path) {
     // extras sorted by (column, row, id) and their column starts
-    for (uint32_t j = t; j < ne; j += kRtThreads) rt_rank_extra(j, ne, S);
-    for (int c = t; c <= rt.ncols; c += kRtThreads) rt_ecbase(c, ne, S);
+    for (uint32_t j = t; j < ne; j += kT) rt_rank_extra(j, ne, S);
+    for (int c = t; c <= rt.ncols; c += kT) rt_ecbase(c, ne, S);
     __syncthreads();
     // queries and Bird::step, then classification of the thread's own
     // results (movers are pushed at once); no barrier between the two
-    for (uint32_t o = t; o < nown; o += kRtThreads) rt_agent_lds(rp, rt, sg, o, s_col_base, S);
-    for (uint32_t o = t; o < nown; o += kRtThreads) rt_classify(rp, rt, D, o, &s_nstay, S);
+    for (uint32_t o = t; o < nown; o += kT) rt_agent_lds(rp, rt, sg, o, s_col_base, S);
+    for (uint32_t o = t; o < nown; o += kT) rt_classify(rp, rt, D, o, &s_nstay, S);
     __syncthreads();
-    rt_output(rp, rt, D, me, bo_own, nb_tile, s_nstay, S, s_warp);
+    rt_output<kT>(rp, rt, D, me, bo_own, nb_tile, s_nstay, S, s_warp);
   } else {
     if (t == 0) atom_or(D.flags, kRtFlagGlobal);
-    rt_tile_global(rp, rt, D, (uint32_t)t, kRtThreads);
-    for (int b = t; b <= nb_tile; b += kRtThreads) D.boff[pout][bo_own + b] = 0u;
+    rt_tile_global(rp, rt, D, (uint32_t)t, kT);
+    for (int b = t; b <= nb_tile; b += kT) D.boff[pout][bo_own + b] = 0u;
     if (t == 0) D.cnt[pout][rt.gt] = 0u;
   }
 
@@ -736,15 +864,24 @@ __global__ __launch_bounds__(kRtThreads) void k_rt_load(
     rt_classify(rp, rt, D, o, &s_nstay, S);
   }
   __syncthreads();
-  rt_output(rp, rt, D, me, bo_own, nb_tile, s_nstay, S, s_warp);
+  rt_output<kRtThreads>(rp, rt, D, me, bo_own, nb_tile, s_nstay, S, s_warp);
 }
 
 hipError_t launch_rt_step(const WorkList& wl, const RtDev& D, const RtCaps& caps, uint32_t koff,
-                          int cls, const uint32_t* list, uint32_t nlist, hipStream_t s) {
-  const uint32_t grid = cls ? nlist + kRtOvfMax : wl.ntiles;
+                          int cls, const uint32_t* list, uint32_t nlist, int mode, hipStream_t s) {
+  const uint32_t grid = mode == kRtModeDeferred ? kRtOvfMax : (cls ? nlist : wl.ntiles);
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rt_step, dim3(grid), dim3(kRtThreads), rt_lds_bytes(caps), s, wl.reps,
-                     wl.nreps, wl.tile_prefix, D, caps, koff, cls, list, nlist);
+  if (grid == 0) return hipSuccess;
+  const size_t lds = rt_lds_bytes(caps);
+  if (caps.threads == 1024)
+    hipLaunchKernelGGL(k_rt_step<1024>, dim3(grid), dim3(1024), lds, s, wl.reps, wl.nreps,
+                       wl.tile_prefix, D, caps, koff, cls, list, mode);
+  else if (caps.threads == 512)
+    hipLaunchKernelGGL(k_rt_step<512>, dim3(grid), dim3(512), lds, s, wl.reps, wl.nreps,
+                       wl.tile_prefix, D, caps, koff, cls, list, mode);
+  else
+    hipLaunchKernelGGL(k_rt_step<256>, dim3(grid), dim3(256), lds, s, wl.reps, wl.nreps,
+                       wl.tile_prefix, D, caps, koff, cls, list, mode);
   return hipGetLastError();
 }
 

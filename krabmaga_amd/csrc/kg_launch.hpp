@@ -21,6 +21,10 @@ hipError_t launch_scatter_aligned(const WorkList& wl, Buf R, Buf W, const uint32
                                   uint32_t* cursor, Buf R2, hipStream_t s);
 hipError_t launch_scatter_append(const WorkList& wl, Buf A, uint32_t* cursor, Buf R2,
                                  hipStream_t s);
+// Sorts every bin of R (offsets `offset`) by (cell y, id); T is scratch of
+// R's capacity whose contents are lost.
+hipError_t launch_sort_bins(const WorkList& wl, Buf R, const uint32_t* offset, Buf T,
+                            hipStream_t s);
 hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W,
                         uint32_t* count_next, const uint64_t* gstep, uint32_t* err, int lds_cap,
                         hipStream_t s);
@@ -31,7 +35,7 @@ hipError_t launch_materialize(Buf R, Buf W, uint32_t n, Buf A, hipStream_t s);
 // tile-resident layout that step `load_g + 1` reads.
 size_t rt_lds_bytes(const RtCaps& c);
 hipError_t launch_rt_step(const WorkList& wl, const RtDev& D, const RtCaps& caps, uint32_t koff,
-                          int cls, const uint32_t* list, uint32_t nlist, hipStream_t s);
+                          int cls, const uint32_t* list, uint32_t nlist, int mode, hipStream_t s);
 hipError_t launch_tile_counts(const WorkList& wl, const uint32_t* offset, uint32_t* count,
                               hipStream_t s);
 hipError_t launch_rt_load(const WorkList& wl, const RtDev& D, const RtCaps& caps, int cls,

@@ -112,13 +112,24 @@ KG_HD void scatter_aligned_one(const RepDev& rp, Buf R, Buf W, uint32_t i, uint3
   R2.id[pos] = R.id[i];
 }
 
+// ---- bin order -------------------------------------------------------------
+//
+// A bin holds hb rows of one column; the scatters leave its entries in
+// arbitrary order.  Every rebinning ends by sorting each bin by (cell y, id),
+// ties by position, so the entries of one bag are a contiguous run in
+// ascending id -- the canonical bag order -- and a query walks the run.
+
+KG_HD uint64_t bin_key(const Geom& g, float y, uint32_t id) {
+  return ((uint64_t)(uint32_t)f2i(floorf(y / g.d)) << 32) | id;
+}
+
 // ---- flock: fused neighbour query + Bird::step, one block per tile ---------
 //
 // LDS tile path (toroidal fields whose window reaches at most one tile
 // further): the tile's own bins plus the halo cells the own agents' windows
-// reach are staged into LDS, each staged column is sorted by (cell y, id) --
-// the reference's bag-scan order -- and every own agent scans its window from
-// LDS.  Tiles whose staged set exceeds the LDS capacity, and fields outside
+// reach are staged into LDS -- bins in row order, so each staged column is in
+// (cell y, id) order, the reference's bag-scan order -- and every own agent
+// scans its window from LDS.  Tiles whose staged set exceeds the LDS capacity, and fields outside
 // the LDS path's reach, take the global path (query_global per agent).
 
 struct FlockTile {
@@ -196,11 +207,11 @@ struct FlockSmem {
   float *x, *y, *hx, *hy;
   uint32_t *id, *gi;
   int32_t* cy;
-  uint16_t *perm, *col, *own;
+  uint16_t *col, *own;
 };
 
 KG_HD size_t flock_smem_bytes(int cap) {
-  return (size_t)cap * (7 * sizeof(uint32_t) + 3 * sizeof(uint16_t));
+  return (size_t)cap * (7 * sizeof(uint32_t) + 2 * sizeof(uint16_t));
 }
 
 KG_HD FlockSmem flock_carve(uint8_t* p, int cap) {
@@ -212,8 +223,7 @@ KG_HD FlockSmem flock_carve(uint8_t* p, int cap) {
   s.id = (uint32_t*)(s.hy + cap);
   s.gi = s.id + cap;
   s.cy = (int32_t*)(s.gi + cap);
-  s.perm = (uint16_t*)(s.cy + cap);
-  s.col = s.perm + cap;
+  s.col = (uint16_t*)(s.cy + cap);
   s.own = s.col + cap;
   return s;
 }
@@ -246,22 +256,10 @@ KG_HD void flock_stage(const RepDev& rp, const FlockTile& ft, uint32_t k, Buf R,
   S.col[k] = (uint16_t)lo;
 }
 
-// Ranks item k inside its staged column by (cell y, id, slot) and, if it is
-// one of the tile's own agents, appends it to the own list.
-KG_HD void flock_rank(const FlockTile& ft, uint32_t k, const uint32_t* col_base, FlockSmem S,
-                      uint32_t* nown) {
-  const int c = S.col[k];
-  const uint32_t lo = col_base[c], hi = col_base[c + 1];
+// Appends staged item k to the own list if it is one of the tile's agents.
+KG_HD void flock_own(const FlockTile& ft, uint32_t k, FlockSmem S, uint32_t* nown) {
   const int32_t ky = S.cy[k];
-  const uint32_t kid = S.id[k];
-  uint32_t rank = 0;
-  for (uint32_t j = lo; j < hi; ++j) {
-    const int32_t jy = S.cy[j];
-    const uint32_t jid = S.id[j];
-    rank += (jy < ky) || (jy == ky && (jid < kid || (jid == kid && j < k)));
-  }
-  S.perm[lo + rank] = (uint16_t)k;
-  const int32_t cx = ft.sc_lo + c;
+  const int32_t cx = ft.sc_lo + S.col[k];
   if (cx >= ft.c0 && cx < ft.c1 && ky >= ft.y0 && ky < ft.y1) {
     const uint32_t o = atom_add(nown, 1u);
     S.own[o] = (uint16_t)k;
@@ -301,14 +299,13 @@ KG_HD void flock_agent_lds(const RepDev& rp, const FlockTile& ft, uint32_t k,
     uint32_t lo = col_base[c], hi = col_base[c + 1];
     while (lo < hi) {  // first slot with cell y >= min_j
       const uint32_t mid = (lo + hi) >> 1;
-      if (S.cy[S.perm[mid]] < min_j) lo = mid + 1;
+      if (S.cy[mid] < min_j) lo = mid + 1;
       else hi = mid;
     }
     const uint32_t end = col_base[c + 1];
     int32_t cur = -1;
     int chk = 0;
-    for (uint32_t q = lo; q < end; ++q) {
-      const uint32_t e = S.perm[q];
+    for (uint32_t e = lo; e < end; ++e) {
       const int32_t ey_c = S.cy[e];
       if (ey_c > max_j) break;
       if (ey_c != cur) {

@@ -24,14 +24,17 @@
 // result slot, then write the new states: stayers sorted into the other
 // parity's region, movers into their destination tile's inbox.
 //
-// Tile classes.  Tiles are planned as normal or dense.  The normal launch
-// covers every tile with LDS capacities sized for normal tiles and returns
-// at once for dense ones; a second launch runs the dense tiles (a short
-// list) with larger capacities, plus the normal tiles that outgrew their
-// capacities since the plan (they defer themselves before writing anything).
-// A tile whose working set still exceeds its launch's capacities runs the
-// same step from global memory and sends all its agents through the inboxes.
-// Deferral and the global path raise flags that make the host re-plan.
+// Tile classes.  Each tile is planned into the smallest of up to four
+// classes whose LDS capacities hold it (capacities grow geometrically).  The
+// class-0 launch covers every tile and returns at once for the others; each
+// further class runs its tile list with larger capacities.  The class
+// launches of a step touch disjoint tiles and run side by side; a tile of a
+// class below the last that outgrew its capacities since the plan defers
+// itself before writing anything, and one more launch, after the class
+// launches, runs the deferred tiles with the last class's capacities.  A
+// tile whose working set still exceeds its launch's capacities runs the same
+// step from global memory and sends all its agents through the inboxes.
+// The global path, spilling and frequent deferral make the host re-plan.
 #pragma once
 
 #include "kg_phases.hpp"
@@ -39,6 +42,14 @@
 namespace kg {
 
 constexpr int kRtThreads = 256;
+// Tile classes of one plan: class 0 covers every tile in one launch, classes
+// 1.. run their tile lists.
+constexpr int kRtMaxClasses = 4;
+// Launch modes of k_rt_step: a class below the last (its tiles may defer),
+// the last class (no deferral), the deferred tiles of the step.
+constexpr int kRtModeClass = 0;
+constexpr int kRtModeFinal = 1;
+constexpr int kRtModeDeferred = 2;
 constexpr int kRtMaxBins = 1024;  // tx * tr of a tile on the LDS path
 
 constexpr uint32_t kRtFlagSpill = 1u;    // an inbox overflowed into the spill list
@@ -88,6 +99,7 @@ struct RtCaps {
   int nb;      // max tx * tr over the replicas (<= kRtMaxBins)
   int nbc;     // bin-table entries per staged column (staged bins + 1)
   int cols;    // staged columns (<= kMaxStageCols)
+  int threads; // work-group size of the launch: 256, 512 or 1024
 };
 
 struct RtTile {

@@ -33,6 +33,11 @@ void RtState::drop_graphs() {
 
 void RtState::release() {
   drop_graphs();
+  for (int k = 0; k < kRtMaxClasses; ++k) {
+    if (aux[k]) (void)hipStreamDestroy(aux[k]);
+    if (ev_done[k]) (void)hipEventDestroy(ev_done[k]);
+  }
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
   (void)hipFree(meta);
   (void)hipFree(dense);
   for (uint32_t* p : {cnt[0], cnt[1], boff[0], boff[1], icnt[0], icnt[1], icnt[2], scnt,
@@ -85,13 +90,21 @@ int RtState::setup(const RtPlan& pl, const RtClassPlan& cls, uint64_t spill_cap,
     }
     allocated = true;
   }
+  if (!ev_fork) {
+    KG_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    for (int k = 0; k < kRtMaxClasses; ++k) {
+      KG_HIP(hipStreamCreateWithFlags(&aux[k], hipStreamNonBlocking));
+      KG_HIP(hipEventCreateWithFlags(&ev_done[k], hipEventDisableTiming));
+    }
+  }
   plan = pl;
   ntiles = (uint32_t)nt;
   KG_HIP(hipMemcpyAsync(meta, plan.meta.data(), nt * sizeof(RtMeta), hipMemcpyHostToDevice, s));
   dense_host = cls.dense;
   ndense = (uint32_t)dense_host.size();
-  caps[0] = cls.caps[0];
-  caps[1] = cls.caps[1];
+  ncls = cls.ncls;
+  for (int k = 0; k < kRtMaxClasses; ++k) caps[k] = cls.caps[k];
+  for (int k = 0; k <= kRtMaxClasses; ++k) off[k] = cls.off[k];
   if (ndense)
     KG_HIP(hipMemcpyAsync(dense, dense_host.data(), ndense * sizeof(uint32_t),
                           hipMemcpyHostToDevice, s));
@@ -108,6 +121,33 @@ int RtState::reset(uint64_t g, hipStream_t s) {
   KG_HIP(launch_set_u64(gstep, g, s));
   G = g;
   return KG_OK;
+}
+
+hipError_t rt_launch_classes(const WorkList& wl, const RtDev& D, const RtState& rt, uint32_t koff,
+                             hipStream_t s) {
+  hipError_t e;
+  const int last = rt.ncls - 1;
+  if (last == 0) return launch_rt_step(wl, D, rt.caps[0], koff, 0, rt.dense, 0, kRtModeFinal, s);
+  // fork: classes 1.. on the auxiliary streams, class 0 on s
+  if ((e = hipEventRecord(rt.ev_fork, s)) != hipSuccess) return e;
+  for (int k = 1; k <= last; ++k) {
+    if ((e = hipStreamWaitEvent(rt.aux[k], rt.ev_fork, 0)) != hipSuccess) return e;
+    if ((e = launch_rt_step(wl, D, rt.caps[k], koff, k, rt.dense + rt.off[k],
+                            rt.off[k + 1] - rt.off[k], k == last ? kRtModeFinal : kRtModeClass,
+                            rt.aux[k])) != hipSuccess)
+      return e;
+    if ((e = hipEventRecord(rt.ev_done[k], rt.aux[k])) != hipSuccess) return e;
+  }
+  if ((e = launch_rt_step(wl, D, rt.caps[0], koff, 0, rt.dense, 0, kRtModeClass, s)) != hipSuccess)
+    return e;
+  // join the classes that may defer, run their deferred tiles with the last
+  // class's capacities, then join the last class
+  for (int k = 1; k < last; ++k)
+    if ((e = hipStreamWaitEvent(s, rt.ev_done[k], 0)) != hipSuccess) return e;
+  if ((e = launch_rt_step(wl, D, rt.caps[last], koff, last, rt.dense, 0, kRtModeDeferred, s)) !=
+      hipSuccess)
+    return e;
+  return hipStreamWaitEvent(s, rt.ev_done[last], 0);
 }
 
 }  // namespace kg

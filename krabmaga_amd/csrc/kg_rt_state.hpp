@@ -30,8 +30,10 @@ struct DevBuf {
 struct RtState {
   bool allocated = false;
   RtPlan plan;
-  RtCaps caps[2]{};          // normal, dense
-  uint32_t* dense = nullptr;  // device list of dense tiles
+  int ncls = 2;
+  RtCaps caps[kRtMaxClasses]{};
+  uint32_t* dense = nullptr;  // device tile lists of classes 1.., grouped by class
+  uint32_t off[kRtMaxClasses + 1] = {};
   uint32_t ndense = 0;
   std::vector<uint32_t> dense_host;  // kept until the upload ran
   uint32_t ntiles = 0;
@@ -47,6 +49,9 @@ struct RtState {
   uint32_t* ovf = nullptr;  // 2 x (kRtOvfMax + 1)
   uint64_t G = 0;  // value of *gstep once queued work has run
   std::vector<std::pair<int, hipGraphExec_t>> graphs;  // (steps, exec)
+  // the class launches of a step run side by side on these streams
+  hipStream_t aux[kRtMaxClasses] = {};
+  hipEvent_t ev_fork = nullptr, ev_done[kRtMaxClasses] = {};
 
   RtDev dev() const;
   // (Re)allocates for `plan` (growing only) and uploads the plan tables and
@@ -59,6 +64,12 @@ struct RtState {
 };
 
 hipError_t launch_set_u64(uint64_t* p, uint64_t v, hipStream_t s);
+
+// One tile-resident schedule step, launch koff of a batch: the class
+// launches of the plan in `rt` side by side (they touch disjoint tiles), then
+// one launch for the tiles they deferred.
+hipError_t rt_launch_classes(const WorkList& wl, const RtDev& D, const RtState& rt, uint32_t koff,
+                             hipStream_t s);
 hipError_t launch_add_u64(uint64_t* p, uint64_t v, hipStream_t s);
 
 }  // namespace kg

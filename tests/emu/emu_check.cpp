@@ -29,7 +29,7 @@ int emu_rt_steps(emu_field* f, const kg_flock_params* p, uint64_t step0, uint32_
 uint32_t emu_rt_flags(emu_field* f);
 uint32_t emu_rt_read(emu_field* f, uint32_t* ids, float* x, float* y, float* hx, float* hy);
 void emu_rt_counts(emu_field* f, uint64_t* lds, uint64_t* glob, uint64_t* deferred,
-                   uint32_t* ndense);
+                   uint32_t* ndense, int* ncls);
 }
 
 struct RtScenario {
@@ -196,9 +196,11 @@ static int run_rt(const RtScenario& s) {
   char extra[160];
   uint64_t nl = 0, ng = 0, ndf = 0;
   uint32_t nde = 0;
-  emu_rt_counts(e, &nl, &ng, &ndf, &nde);
-  std::snprintf(extra, sizeof(extra), "rc=%d flags=0x%x dense=%u lds=%llu global=%llu deferred=%llu",
-                rc, emu_rt_flags(e), nde, (unsigned long long)nl, (unsigned long long)ng,
+  int ncls = 0;
+  emu_rt_counts(e, &nl, &ng, &ndf, &nde, &ncls);
+  std::snprintf(extra, sizeof(extra),
+                "rc=%d flags=0x%x classes=%d dense=%u lds=%llu global=%llu deferred=%llu", rc,
+                emu_rt_flags(e), ncls, nde, (unsigned long long)nl, (unsigned long long)ng,
                 (unsigned long long)ndf);
   const int bad = compare(s.name, s.n, gid, gx, gy, ghx, ghy, m, fl, extra);
   emu_destroy(e);

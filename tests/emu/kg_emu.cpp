@@ -41,8 +41,10 @@ struct HostBuf {
 struct EmuRt {
   bool active = false;
   RtPlan plan;
-  RtCaps caps[2]{};
-  std::vector<uint32_t> dense;
+  int ncls = 2;
+  RtCaps caps[kRtMaxClasses]{};
+  std::vector<uint32_t> dense;  // class lists 1.., grouped by class
+  uint32_t off[kRtMaxClasses + 1] = {};
   HostBuf stay[2], inbox[2], spill[2];
   std::vector<uint32_t> spill_rep[2];
   std::vector<uint32_t> cnt[2], boff[2], icnt[3];
@@ -167,6 +169,7 @@ static void emu_rt_output(const RepDev& rp, const RtTile& rt, const RtDev& D, co
 // tile `item` in class `cls`, with the threads of each phase run in turn.
 static void emu_rt_block(emu_field* f, uint32_t item, int cls, int load, uint64_t load_g,
                          std::vector<uint8_t>& smem, bool deferred = false) {
+  const bool last = cls == f->rt.ncls - 1;
   EmuRt& E = f->rt;
   RtDev D = E.dev();
   const RtCaps caps = E.caps[cls];
@@ -232,20 +235,21 @@ static void emu_rt_block(emu_field* f, uint32_t item, int cls, int load, uint64_
   bool global_path = !lds_ok || total > (uint32_t)caps.runs;
   if (!global_path) {
     const uint32_t next = src_pre[10];
-    for (uint32_t t = 0; t < (uint32_t)kRtThreads; ++t)
-      if (!rt_fetch(rp, rt, D, t, t + kRtThreads, total, t, next, seg_start, seg_len, col_base,
+    const uint32_t kT = (uint32_t)caps.threads;
+    for (uint32_t t = 0; t < kT; ++t)
+      if (!rt_fetch(rp, rt, D, t, t + kT, total, t, next, seg_start, seg_len, col_base,
                     src_base, src_pre, caps, &ne, &nown, S))
         fail = true;
-    for (uint32_t k = 2 * kRtThreads; k < total; ++k)
+    for (uint32_t k = 2 * kT; k < total; ++k)
       rt_stage(rp, rt, D, k, seg_start, seg_len, col_base, caps.own, &nown, S);
-    for (uint32_t j = kRtThreads; j < next; ++j)
+    for (uint32_t j = kT; j < next; ++j)
       if (!rt_gather_extra(rp, rt, D, j, src_base, src_pre, caps.extras, &ne, caps.own, &nown, S))
         fail = true;
     for (uint32_t q = 0; q < (uint32_t)(rt.ncols * sg.nbc); ++q)
       rt_bintab(rp, rt, D, sg, q, seg_len, S);
     global_path = fail || nown > (uint32_t)caps.own;
   }
-  if (global_path && cls == 0 && lds_ok) {
+  if (global_path && !last && lds_ok) {
     const int par = (int)(g & 1u);
     const uint32_t slot = D.ovf[par][0]++;
     *D.flags |= kRtFlagDefer;
@@ -316,10 +320,11 @@ int emu_rt_enter(emu_field* f, int caps_runs, int caps_own, int caps_extras, int
   RtClassPlan cls;
   rt_classes(&E.plan, counts, {RtRepInfo{f->g, f->rep.disc_dist, 0u}}, (mode & 2) ? 2048 : 32768,
              157696, &cls);
-  E.caps[0] = cls.caps[0];
-  E.caps[1] = cls.caps[1];
+  E.ncls = cls.ncls;
+  for (int k = 0; k < kRtMaxClasses; ++k) E.caps[k] = cls.caps[k];
+  for (int k = 0; k <= kRtMaxClasses; ++k) E.off[k] = cls.off[k];
   E.dense = cls.dense;
-  for (int c = 0; c < ((mode & 4) ? 1 : 2); ++c) {
+  for (int c = 0; c < ((mode & 4) ? 1 : E.ncls); ++c) {
     if (caps_runs > 0) E.caps[c].runs = caps_runs;
     if (caps_own > 0) E.caps[c].own = caps_own;
     if (caps_extras > 0) E.caps[c].extras = caps_extras;
@@ -340,7 +345,7 @@ int emu_rt_enter(emu_field* f, int caps_runs, int caps_own, int caps_extras, int
   E.gstep = 1;
   for (int p = 0; p < 2; ++p) E.ovf[p].assign(kRtOvfMax + 1, 0u);
   std::vector<uint8_t> smem;
-  for (int c = 0; c < 2; ++c)
+  for (int c = 0; c < E.ncls; ++c)
     for (uint32_t item : order(0, nt, f->shuffle_seed))
       emu_rt_block(f, item, c, 1, E.gstep - 1, smem);
   E.active = true;
@@ -358,11 +363,12 @@ int emu_rt_steps(emu_field* f, const kg_flock_params* p, uint64_t step0, uint32_
     // independent within a step, so any order is valid
     const uint64_t sh = f->shuffle_seed ? f->shuffle_seed + s2 : 0;
     for (uint32_t item : order(0, f->g.ntiles, sh)) emu_rt_block(f, item, 0, 0, 0, smem);
-    for (uint32_t k : order(0, (uint32_t)E.dense.size(), sh))
-      emu_rt_block(f, E.dense[k], 1, 0, 0, smem);
+    for (int c = 1; c < E.ncls; ++c)
+      for (uint32_t k : order(E.off[c], E.off[c + 1], sh)) emu_rt_block(f, E.dense[k], c, 0, 0, smem);
     const int par = (int)(E.gstep & 1u);
     const uint32_t nd = std::min<uint32_t>(E.ovf[par][0], kRtOvfMax);
-    for (uint32_t i = 0; i < nd; ++i) emu_rt_block(f, E.ovf[par][1 + i], 1, 0, 0, smem, true);
+    for (uint32_t i = 0; i < nd; ++i)
+      emu_rt_block(f, E.ovf[par][1 + i], E.ncls - 1, 0, 0, smem, true);
     E.ovf[par ^ 1][0] = 0;
     E.gstep += 1;
   }
@@ -373,11 +379,12 @@ uint32_t emu_rt_flags(emu_field* f) { return f->rt.flags; }
 
 // Work-groups run on the LDS path, on the global path, and deferred.
 void emu_rt_counts(emu_field* f, uint64_t* lds, uint64_t* glob, uint64_t* deferred,
-                   uint32_t* ndense) {
+                   uint32_t* ndense, int* ncls) {
   *lds = f->lds_tiles;
   *glob = f->global_tiles;
   *deferred = f->deferred_tiles;
   *ndense = (uint32_t)f->rt.dense.size();
+  *ncls = f->rt.ncls;
 }
 
 // All agents of the tile-resident layout (any order).
@@ -477,6 +484,35 @@ int emu_lazy_update(emu_field* f) {
   }
   for (uint32_t i : order(0, f->nA, f->shuffle_seed))
     scatter_append_one(&f->rep, 1, app_prefix, A, i, f->cursor.data(), R2);
+  // k_sort_bins: each bin into (cell y, id) order, ties by position
+  std::vector<uint32_t> idx;
+  std::vector<uint64_t> key;
+  for (uint32_t b = 0; b < f->g.nbins; ++b) {
+    const uint32_t s = f->offset2[b], m = f->offset2[b + 1] - s;
+    if (m <= 1) continue;
+    idx.resize(m);
+    key.resize(m);
+    for (uint32_t j = 0; j < m; ++j) {
+      idx[j] = j;
+      key[j] = bin_key(f->g, R2.y[s + j], R2.id[s + j]);
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t c) { return key[a] < key[c]; });
+    Buf T = W;  // the write buffer is consumed: scratch, as on the device
+    for (uint32_t j = 0; j < m; ++j) {
+      T.id[s + j] = R2.id[s + idx[j]];
+      T.x[s + j] = R2.x[s + idx[j]];
+      T.y[s + j] = R2.y[s + idx[j]];
+      T.hx[s + j] = R2.hx[s + idx[j]];
+      T.hy[s + j] = R2.hy[s + idx[j]];
+    }
+    for (uint32_t j = 0; j < m; ++j) {
+      R2.id[s + j] = T.id[s + j];
+      R2.x[s + j] = T.x[s + j];
+      R2.y[s + j] = T.y[s + j];
+      R2.hx[s + j] = T.hx[s + j];
+      R2.hy[s + j] = T.hy[s + j];
+    }
+  }
   std::swap(f->R, f->R2);
   std::swap(f->offset, f->offset2);
   f->nR = run;
@@ -515,10 +551,11 @@ int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step) {
       ++f->lds_tiles;
       std::fill(smem.begin(), smem.end(), 0xA5);
       const FlockSmem S = flock_carve(smem.data(), f->lds_cap);
-      for (uint32_t k = 0; k < L; ++k)
-        flock_stage(rp, ft, k, R, seg_start.data(), seg_len.data(), col_base.data(), S);
       uint32_t nown = 0;
-      for (uint32_t k = 0; k < L; ++k) flock_rank(ft, k, col_base.data(), S, &nown);
+      for (uint32_t k = 0; k < L; ++k) {
+        flock_stage(rp, ft, k, R, seg_start.data(), seg_len.data(), col_base.data(), S);
+        flock_own(ft, k, S, &nown);
+      }
       for (uint32_t o = 0; o < nown; ++o)
         flock_agent_lds(rp, ft, S.own[o], col_base.data(), S, W, f->count.data(), &f->err);
     } else {

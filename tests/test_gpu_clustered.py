@@ -33,3 +33,55 @@ def test_dense_blobs_match_oracle(oracle, binned):
     assert np.array_equal(ids, np.arange(n, dtype=np.uint32))
     bad = np.flatnonzero(np.any(st.view(np.uint32) != ref.view(np.uint32), axis=0))
     assert bad.size == 0, f"{bad.size} birds differ, first {bad[:5]}"
+
+
+@pytest.mark.parametrize("binned", [False, True])
+def test_crowded_cells_match_oracle(oracle, binned):
+    """A few single cells holding hundreds to thousands of birds (bins far
+    wider than a wave, and wider than one LDS pass of the bin sort) in a
+    sparse field: two steps and the neighbour lists equal the oracle's."""
+    O = oracle
+    rng = np.random.default_rng(21)
+    side = 400.0
+    crowds = [(100.0, 100.0, 3000), (250.0, 300.0, 700), (33.0, 370.0, 90)]
+    xs, ys = [rng.uniform(0.0, side, 1500)], [rng.uniform(0.0, side, 1500)]
+    for cx, cy, m in crowds:
+        # inside one cell of side D around (cx, cy)
+        x0 = np.floor(cx / D) * D
+        y0 = np.floor(cy / D) * D
+        xs.append(x0 + rng.uniform(0.05, 0.95, m) * D)
+        ys.append(y0 + rng.uniform(0.05, 0.95, m) * D)
+    x = np.concatenate(xs).astype(np.float32)
+    y = np.concatenate(ys).astype(np.float32)
+    n = x.size
+    a = rng.uniform(0.0, 2.0 * np.pi, n)
+    hx = np.cos(a).astype(np.float32)
+    hy = np.sin(a).astype(np.float32)
+    perm = rng.permutation(n).astype(np.uint32)  # ids unrelated to position
+    # neighbour lists straight after binning
+    q = Field2D(side, side, D, True, n)
+    q.set_object_locations(perm, x, y, hx, hy)
+    q.lazy_update()
+    # the device's bags are in ascending id, the oracle's in insertion order
+    of = O.Field2D(side, side, D, True)
+    for i in np.argsort(perm):
+        of.set_object_location(int(perm[i]), float(x[i]), float(y[i]))
+    of.lazy_update()
+    for qx, qy in [(100.0, 100.0), (250.0, 300.0), (33.0, 370.0), (5.0, 5.0)]:
+        want = [b[0] for b in of.get_neighbors_within_distance(qx, qy, 10.0)]
+        assert list(q.get_neighbors_within_distance(qx, qy, 10.0)) == want
+    q.close()
+    p = FlockParams(seed=21)
+    f = Field2D(side, side, D, True, n)
+    f.set_object_locations(perm, x, y, hx, hy)
+    f.schedule_run(p, 0, 2, binned=binned)
+    st, ids = f.state_by_id(0)
+    f.close()
+    order = np.argsort(perm)
+    fl = O.Flocker(side, side, D, True, x[order], y[order], hx[order], hy[order],
+                   O.params(seed=21))
+    fl.step(2)
+    ref = fl.state()
+    assert np.array_equal(ids, np.arange(n, dtype=np.uint32))
+    bad = np.flatnonzero(np.any(st.view(np.uint32) != ref.view(np.uint32), axis=0))
+    assert bad.size == 0, f"{bad.size} birds differ, first {bad[:5]}"
