@@ -49,7 +49,7 @@ def candidates(mean_agents_per_cell):
         for hb in (8, 16):
             if hb > tr_:
                 continue
-            for lds in (32 * 1024, 40 * 1024):
+            for lds in (24 * 1024, 32 * 1024):
                 out.append(Tuning(tile_cols=tc, tile_rows=tr_, bin_rows=hb, lds_normal=lds))
     return out
 

@@ -326,7 +326,7 @@ int rebuild(kg_ensemble* e) {
   RtPlan plan;
   rt_plan(counts, nb, expect, &plan);
   RtClassPlan cls;
-  const size_t lds_normal = e->tuning.lds_normal ? e->tuning.lds_normal : 40 * 1024;
+  const size_t lds_normal = e->tuning.lds_normal ? e->tuning.lds_normal : kRtLdsNormal;
   rt_classes(&plan, counts, infos, lds_normal, e->lds_limit, &cls);
   if (!cls.fits)
     return set_error(KG_E_ADMIT, "the runs' tiles do not fit the LDS budget of the tile-resident step");

@@ -252,6 +252,17 @@ int set_params(kg_field* f, const kg_flock_params* p, uint64_t step_base) {
 
 // ---- tile-resident layout -------------------------------------------------
 
+// Deferred tiles in one step that ask for a new plan; KG_RT_REPLAN_DEFERRED
+// overrides it for tuning.
+uint32_t rt_replan_deferred() {
+  static const uint32_t v = [] {
+    const char* e = std::getenv("KG_RT_REPLAN_DEFERRED");
+    const long n = e ? std::atol(e) : 0;
+    return n >= 1 && n <= (long)kRtOvfMax ? (uint32_t)n : kRtReplanDeferred;
+  }();
+  return v;
+}
+
 // Reads the tile-resident flags (synchronises): a location out of the grid
 // poisons the field; exceeded capacities ask for a new plan.
 int rt_check_flags(kg_field* f) {
@@ -266,7 +277,7 @@ int rt_check_flags(kg_field* f) {
   // the dense launch; many deferrals mean the plan no longer fits the
   // density (left alone they grow into global-path tiles), and spilling or
   // the global path costs every tile: those ask for a new plan
-  if ((fl & (kRtFlagSpill | kRtFlagGlobal)) || fl2[1] >= kRtReplanDeferred) f->rt_replan = true;
+  if ((fl & (kRtFlagSpill | kRtFlagGlobal)) || fl2[1] >= rt_replan_deferred()) f->rt_replan = true;
   return check_poison(f);
 }
 
@@ -276,7 +287,7 @@ size_t rt_lds_normal() {
   static const size_t v = [] {
     const char* e = std::getenv("KG_RT_LDS_NORMAL");
     const long n = e ? std::atol(e) : 0;
-    return n >= 4096 && n <= 160 * 1024 ? (size_t)n : (size_t)40 * 1024;
+    return n >= 4096 && n <= 160 * 1024 ? (size_t)n : kRtLdsNormal;
   }();
   return v;
 }

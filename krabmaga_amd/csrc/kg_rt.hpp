@@ -58,12 +58,15 @@ constexpr uint32_t kRtFlagRange = 4u;    // a location left the bag grid
 constexpr uint32_t kRtFlagStayCap = 8u;  // a tile's stayers exceeded its capacity
 constexpr uint32_t kRtFlagDefer = 16u;   // a normal tile outgrew its class and was deferred
 
-// Normal tiles that outgrow their launch's capacities defer to the dense
-// launch of the same step through a per-parity list ([0] count, then ids);
-// the dense launch has kRtOvfMax work-groups beyond its static tiles.
+// Tiles that outgrow their class's capacities defer to the same step's
+// launch for deferred tiles through a per-parity list ([0] count, then ids);
+// that launch has kRtOvfMax work-groups.
 constexpr uint32_t kRtOvfMax = 256;
+// Dynamic LDS budget of a class-0 work-group: four per CU (160 KB) with the
+// kernel's static LDS beside them (at 40 KB only three fit).
+constexpr size_t kRtLdsNormal = 28 * 1024;
 // Deferrals per step past which the host re-plans.
-constexpr uint32_t kRtReplanDeferred = kRtOvfMax / 2;
+constexpr uint32_t kRtReplanDeferred = 4;
 
 // Per-tile plan entry, two 16-byte loads.
 struct alignas(16) RtMeta {
@@ -562,18 +565,25 @@ KG_HD int rt_check_from_sq(float sx_nw, float sx_sw, float sy_nw, float sy_ne, f
 // One neighbour candidate of query (qx, qy) in bag-scan order: the
 // check_circle of its cell (computed when the cell row changes, from the
 // corner terms when `pre`), then the distance test and the accumulation.
-KG_HD void rt_visit(const Geom& g, int32_t i, float T, bool pre, int32_t min_j, float sxn,
-                    float sxs, float syn0, float syn1, float syn2, float sye0, float sye1,
+KG_HD void rt_visit(const Geom& g, int32_t i, float T, bool pre, int32_t min_j, float* sxn,
+                    float* sxs, float syn0, float syn1, float syn2, float sye0, float sye1,
                     float sye2, float qx, float qy, uint32_t qid, float ex_, float ey_, float ehx_,
                     float ehy_, uint32_t eid_, int32_t ecy_, int32_t* cur, int* chk,
                     FlockAcc* acc) {
   if (ecy_ != *cur) {
+    if (pre && *cur == -1) {
+      // the column's corner terms, at its first candidate (most columns of
+      // a sparse field have none): nw_x = i * d, sw_x = min(nw_x + d, w)
+      const float nwx = (float)i * g.d;
+      *sxn = rt_corner_sq(nwx, qx, g.w);
+      *sxs = rt_corner_sq(fminf(nwx + g.d, g.w), qx, g.w);
+    }
     *cur = ecy_;
     if (pre) {
       const int k = ecy_ - min_j;
       const float yn = k == 0 ? syn0 : (k == 1 ? syn1 : syn2);
       const float ye = k == 0 ? sye0 : (k == 1 ? sye1 : sye2);
-      *chk = rt_check_from_sq(sxn, sxs, yn, ye, T);
+      *chk = rt_check_from_sq(*sxn, *sxs, yn, ye, T);
     } else {
       *chk = check_circle_sq(g, i, ecy_, qx, qy, T);
     }
@@ -631,25 +641,39 @@ KG_HD void rt_agent_lds(const RepDev& rp, const RtTile& rt, const RtSegs& sg, ui
     const uint16_t* tab = S.bintab + (size_t)c * (size_t)sg.nbc;
     const uint32_t base = col_base[c];
     uint32_t a = base + tab[x_lo], a_end = base + tab[x_hi];
-    while (a < a_end && S.scy[a] < min_j) ++a;  // rows of min_j's bin below the window
+    // skip the rows of min_j's bin below the window; rows ascend along the
+    // column, so the items below min_j among the next eight are a prefix
+    // (eight loads per round trip instead of one)
+    while (a < a_end) {
+      uint32_t n = 0;
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) {
+        const uint32_t q = a + u < a_end ? a + u : a;
+        n += (a + u < a_end && S.scy[q] < min_j) ? 1u : 0u;
+      }
+      a += n;
+      if (n < 8u) break;
+    }
     uint32_t b = S.ecbase[c], b_end = S.ecbase[c + 1];
     while (b < b_end && S.ecy[S.eperm[b]] < min_j) ++b;
-    // corner columns: nw_x = i * d, sw_x = min(nw_x + d, w)
-    const float nwx = (float)i * g.d;
-    const float swx = fminf(nwx + g.d, g.w);
-    const float sxn = rt_corner_sq(nwx, qx, g.w), sxs = rt_corner_sq(swx, qx, g.w);
+    float sxn = 0.0f, sxs = 0.0f;  // set by rt_visit at the column's first candidate
     int32_t cur = -1;
     int chk = 0;
     if (!(b < b_end && S.ecy[S.eperm[b]] <= max_j)) {
       // no extras in this column's window rows (the common case): the
-      // stayers alone, each item's fields loaded together
-      for (; a < a_end; ++a) {
-        const int32_t ecy_ = S.scy[a];
-        const float ex_ = S.sx[a], ey_ = S.sy[a], ehx_ = S.shx[a], ehy_ = S.shy[a];
-        const uint32_t eid_ = S.sid[a];
-        if (ecy_ > max_j) break;
-        rt_visit(g, i, T, pre, min_j, sxn, sxs, syn0, syn1, syn2, sye0, sye1, sye2, qx, qy, qid,
-                 ex_, ey_, ehx_, ehy_, eid_, ecy_, &cur, &chk, &acc);
+      // stayers alone, two items per round trip with all their fields
+      for (; a < a_end; a += 2) {
+        const uint32_t a1 = a + 1 < a_end ? a + 1 : a;
+        const int32_t ecy0 = S.scy[a], ecy1 = S.scy[a1];
+        const float ex0 = S.sx[a], ey0 = S.sy[a], ehx0 = S.shx[a], ehy0 = S.shy[a];
+        const float ex1 = S.sx[a1], ey1 = S.sy[a1], ehx1 = S.shx[a1], ehy1 = S.shy[a1];
+        const uint32_t eid0 = S.sid[a], eid1 = S.sid[a1];
+        if (ecy0 > max_j) break;
+        rt_visit(g, i, T, pre, min_j, &sxn, &sxs, syn0, syn1, syn2, sye0, sye1, sye2, qx, qy, qid,
+                 ex0, ey0, ehx0, ehy0, eid0, ecy0, &cur, &chk, &acc);
+        if (a + 1 >= a_end || ecy1 > max_j) break;
+        rt_visit(g, i, T, pre, min_j, &sxn, &sxs, syn0, syn1, syn2, sye0, sye1, sye2, qx, qy, qid,
+                 ex1, ey1, ehx1, ehy1, eid1, ecy1, &cur, &chk, &acc);
       }
       continue;
     }
@@ -677,7 +701,7 @@ KG_HD void rt_agent_lds(const RepDev& rp, const RtTile& rt, const RtSegs& sg, ui
       const int32_t ecy_ = pcy[src];
       a += take_a ? 1u : 0u;
       b += take_a ? 0u : 1u;
-      rt_visit(g, i, T, pre, min_j, sxn, sxs, syn0, syn1, syn2, sye0, sye1, sye2, qx, qy, qid,
+      rt_visit(g, i, T, pre, min_j, &sxn, &sxs, syn0, syn1, syn2, sye0, sye1, sye2, qx, qy, qid,
                ex_, ey_, ehx_, ehy_, eid_, ecy_, &cur, &chk, &acc);
     }
   }
