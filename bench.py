@@ -100,13 +100,13 @@ def timed_run(hip, f, p, first, steps, binned):
 
 
 def profiled_traffic(name="k_rt_step_flockers_1M.md"):
-    """HBM bytes per launch of the tile-resident step, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a 1 GiB
-    copy), or None."""
+    """HBM bytes per schedule step of the tile-resident step (all its
+    k_rt_step launches), from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a 1 GiB copy), or None."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith("traffic_bytes_per_step:"):
                 return int(line.split(":")[1]), "profiles/" + name
     except OSError:
         pass
@@ -336,13 +336,18 @@ def main():
         hip.copy_calibration(1 << 30)
     n, steps = w["n"], args.steps
     value = n * steps / (ms * 1e-3)
-    # dominant kernel of the timed path: the tile-resident step, or the
-    # fused flock kernel when the field ran the three-kernel path
+    # dominant kernel of the timed path: the tile-resident step -- one
+    # schedule step is its class launches side by side plus the launch for
+    # deferred tiles, so its duration is the timed region's time per step
+    # (HIP events on the field's stream, which forks to and joins the class
+    # streams) -- or the fused flock kernel when the field ran the
+    # three-kernel path
     if kern["rt_step"]["launches"]:
         dom, dom_bytes = "rt_step", BYTES_RT
+        dom_us = 1e3 * ms / steps
     else:
         dom, dom_bytes = "flock", BYTES_FLOCK
-    dom_us = kern[dom]["avg_us"]
+        dom_us = kern[dom]["avg_us"]
     traffic, traffic_src = profiled_traffic()
     achieved = dom_bytes * n / (dom_us * 1e-6) if dom_us > 0 else 0.0
     out = {
@@ -366,6 +371,9 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": "k_" + dom,
+            "launch_us": dom_us,
+            "unit_of_launch": "one schedule step (all k_rt_step launches of the step)"
+                              if dom == "rt_step" else "one launch",
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",

@@ -101,6 +101,7 @@ struct kg_ensemble {
   std::vector<RepDev> h_reps;
   std::vector<uint32_t> h_prefix;
   uint32_t nreps = 0, ntiles = 0, nbins = 0, nagents = 0;
+  uint64_t dead_tiles = 0;  // tiles of retired runs still in the layout
   RtState rt;
   bool replan = false;
   uint64_t builds = 0, flags_seen = 0, steps_run = 0;
@@ -291,6 +292,7 @@ int rebuild(kg_ensemble* e) {
   e->ntiles = tile0;
   e->nbins = bin_base;
   e->nagents = base;
+  e->dead_tiles = 0;
   e->built = true;
   e->dirty = false;
   e->replan = false;
@@ -394,7 +396,7 @@ int check_flags(kg_ensemble* e) {
   e->flags_seen |= fl;
   if (fl & kRtFlagRange)
     return set_error(KG_E_RANGE, "an agent left its run's bag grid");
-  if ((fl & (kRtFlagSpill | kRtFlagGlobal)) || fl2[1] >= kRtReplanDeferred) e->replan = true;
+  if ((fl & (kRtFlagSpill | kRtFlagGlobal)) || fl2[1] >= kRtEnsReplanDeferred) e->replan = true;
   return KG_OK;
 }
 
@@ -489,10 +491,13 @@ int kg_ensemble_retire(kg_ensemble* e, uint32_t id) {
   if (!r) return set_error(KG_E_INVALID, "no run %u", id);
   if (e->built && !e->dirty && idx < e->nreps) {
     // the run stays in the layout as a dead replica (its tiles exit at
-    // once) until the next build, so retiring costs no rebuild
+    // once) until the next build, so retiring costs no rebuild; once dead
+    // tiles are a third of the layout, the next run compacts it
     r->dead = true;
     e->h_reps[idx].dead = 1u;
     KG_HIP(launch_set_rep(e->d_reps + idx, e->h_reps[idx], e->stream));
+    e->dead_tiles += r->ntiles;
+    if (3ull * e->dead_tiles > e->ntiles) e->dirty = true;
     return KG_OK;
   }
   e->runs.erase(e->runs.begin() + (ptrdiff_t)idx);
@@ -531,13 +536,19 @@ int kg_ensemble_read(kg_ensemble* e, uint32_t id, uint32_t* ids, float* x, float
                      float* hy, size_t cap, size_t* n) {
   clear_error();
   if (!e || !n) return set_error(KG_E_INVALID, "NULL argument");
+  {
+    const EnsRun* r0 = find_run(e, id);
+    if (!r0 || r0->dead) return set_error(KG_E_INVALID, "no run %u", id);
+    *n = r0->desc.agents;
+    if (cap < r0->desc.agents)
+      return set_error(KG_E_CAPACITY, "buffer holds %zu of %u agents", cap, r0->desc.agents);
+  }
+  // a rebuild drops retired runs from the run list: look the run up after it
+  if (e->dirty || !e->built)
+    if (int rc = rebuild(e)) return rc;
   size_t idx = 0;
   EnsRun* r = find_run(e, id, &idx);
   if (!r) return set_error(KG_E_INVALID, "no run %u", id);
-  *n = r->desc.agents;
-  if (cap < r->desc.agents) return set_error(KG_E_CAPACITY, "buffer holds %zu of %u agents", cap, r->desc.agents);
-  if (e->dirty || !e->built)
-    if (int rc = rebuild(e)) return rc;
   if (int rc = unload(e, e->A.b, r->tile0, r->ntiles)) return rc;
   const size_t m = r->desc.agents;
   const uint32_t b = r->app_base;

@@ -1,0 +1,177 @@
+"""Summaries of rocprofv3 output for the tile-resident step.
+
+One schedule step of the tile-resident path is several launches of
+k_rt_step: one per tile class (side by side on their own streams) and one
+for the tiles deferred in that step.  This module groups the launches of a
+kernel trace into steps -- each step has exactly one class-0 launch (the
+largest 256-thread grid, one work-group per tile); a launch belongs to the
+last class-0 launch that started before it, give or take `slack_us` -- and
+reports per-step spans (first start to last end) beside per-launch
+durations.  With FETCH_SIZE / WRITE_SIZE passes it reports the HBM bytes per
+step, scaled by what the same counters read for a device-to-device copy of
+known size (the copy the bench issues with --calibrate-copy).
+
+    python -m krabmaga_amd.profsum --trace T.csv [--fetch F.csv --write W.csv]
+        --agents N --skip S --count K --title "..." --cmd "..." > out.md
+"""
+import argparse
+import collections
+import csv
+import statistics
+
+KERNEL = "kg::k_rt_step"
+COPY = "__amd_rocclr_copyBuffer"
+BYTES_PER_AGENT = 40  # read the agent's 20 bytes, write 20 back
+
+
+def _name(r):
+    return r["Kernel_Name"].split("(")[0].replace("void ", "").split("<")[0]
+
+
+def _grid(r):
+    return int(r.get("Grid_Size") or r.get("Grid_Size_X"))
+
+
+def _wg(r):
+    return int(r.get("Workgroup_Size") or r.get("Workgroup_Size_X"))
+
+
+def load_trace(path):
+    rows = list(csv.DictReader(open(path)))
+    out = []
+    for r in rows:
+        out.append({"name": _name(r), "grid": _grid(r), "wg": _wg(r),
+                    "start": int(r["Start_Timestamp"]), "end": int(r["End_Timestamp"])})
+    out.sort(key=lambda r: r["start"])
+    return out
+
+
+def steps_of(trace, slack_us=20.0):
+    """Launch groups of k_rt_step, one per schedule step, in order."""
+    rt = [r for r in trace if r["name"] == KERNEL]
+    if not rt:
+        return []
+    c0_grid = max(r["grid"] for r in rt if r["wg"] == 256)
+    c0 = [r for r in rt if r["wg"] == 256 and r["grid"] == c0_grid]
+    starts = [r["start"] for r in c0]
+    groups = [[r] for r in c0]
+    slack = int(slack_us * 1000)
+    j = 0
+    for r in rt:
+        if r["wg"] == 256 and r["grid"] == c0_grid:
+            continue
+        while j + 1 < len(starts) and starts[j + 1] <= r["start"] + slack:
+            j += 1
+        # launches before the first class-0 launch belong to it
+        groups[j].append(r)
+    return groups
+
+
+def pmc_totals(path, counter):
+    """Per-kernel sums and dispatch counts of one counter (KiB)."""
+    tot = collections.defaultdict(float)
+    n = collections.Counter()
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        k = (_name(r), _grid(r))
+        tot[k] += float(r["Counter_Value"])
+        n[k] += 1
+    return tot, n
+
+
+def summarize(a):
+    tr = load_trace(a.trace)
+    lines = [f"# {a.title}\n", "Commands (repository root, MI355X, ROCm 7.2):\n", "```"]
+    lines.append(f"rocprofv3 --kernel-trace --stats --output-format csv -d <dir> -o trace -- {a.cmd}")
+    if a.fetch:
+        lines.append(f"rocprofv3 --pmc FETCH_SIZE --kernel-trace --output-format csv -d <dir> -o pmc -- "
+                     f"{a.pmc_cmd or a.cmd}")
+        lines.append(f"rocprofv3 --pmc WRITE_SIZE --kernel-trace --output-format csv -d <dir> -o pmc -- "
+                     f"{a.pmc_cmd or a.cmd}")
+    lines.append(f"python -m krabmaga_amd.profsum --trace <trace_kernel_trace.csv> ... "
+                 f"--agents {a.agents} --skip {a.skip} --count {a.count}")
+    lines.append("```\n")
+    # per-launch table
+    d = collections.defaultdict(list)
+    for r in tr:
+        d[(r["name"], r["grid"], r["wg"])].append((r["end"] - r["start"]) / 1000.0)
+    lines.append("Launches (kernel trace):\n")
+    lines.append("| kernel | grid (threads) | work-group | launches | avg us | min us | max us |")
+    lines.append("|---|---|---|---|---|---|---|")
+    for k in sorted(d, key=lambda k: -sum(d[k])):
+        v = d[k]
+        if sum(v) < 50:
+            continue
+        lines.append(f"| {k[0]} | {k[1]} | {k[2]} | {len(v)} | {statistics.mean(v):.1f} | "
+                     f"{min(v):.1f} | {max(v):.1f} |")
+    # per-step spans
+    groups = steps_of(tr)
+    sel = groups[a.skip:a.skip + a.count] if a.count else groups[a.skip:]
+    spans = [(max(r["end"] for r in g) - min(r["start"] for r in g)) / 1000.0 for g in sel]
+    per_step = None
+    if spans:
+        per_step = statistics.mean(spans)
+        nl = statistics.mean(len(g) for g in sel)
+        lines.append(f"\nk_rt_step per schedule step (launches of one step grouped; steps "
+                     f"{a.skip}..{a.skip + len(sel) - 1} of {len(groups)} in the trace):\n")
+        lines.append(f"- launches per step: {nl:.2f}")
+        lines.append(f"- step span (first start to last end): mean {per_step:.1f} us, "
+                     f"min {min(spans):.1f}, max {max(spans):.1f}")
+        alg = BYTES_PER_AGENT * a.agents
+        lines.append(f"- algorithmic bytes: {BYTES_PER_AGENT} B x {a.agents} agents = {alg / 1e6:.2f} MB "
+                     f"-> {alg / per_step / 1e3:.1f} GB/s = {alg / per_step / 1e3 / 8000 * 100:.2f} % "
+                     f"of 8 TB/s")
+        lines.append(f"step_span_us: {per_step:.1f}")
+    if a.fetch and a.write:
+        f, fn = pmc_totals(a.fetch, "FETCH_SIZE")
+        w, wn = pmc_totals(a.write, "WRITE_SIZE")
+        cf = [k for k in f if k[0] == COPY and k[1] == max(g for (nm, g) in f if nm == COPY)]
+        cw = [k for k in w if k[0] == COPY and k[1] == max(g for (nm, g) in w if nm == COPY)]
+        kf = (1 << 30) / (f[cf[0]] / fn[cf[0]] * 1024)
+        kw = (1 << 30) / (w[cw[0]] / wn[cw[0]] * 1024)
+        lines.append("\nPMC byte counters, KiB per dispatch (mean):\n")
+        lines.append("| kernel | grid | FETCH_SIZE KiB | WRITE_SIZE KiB |")
+        lines.append("|---|---|---|---|")
+        for k in sorted(set(f) | set(w)):
+            fv = f.get(k, 0.0) / max(fn.get(k, 0), 1)
+            wv = w.get(k, 0.0) / max(wn.get(k, 0), 1)
+            if fv < 1 and wv < 1:
+                continue
+            lines.append(f"| {k[0]} | {k[1]} | {fv:.1f} | {wv:.1f} |")
+        lines.append(f"\nCalibration: the 1 GiB copy reads FETCH_SIZE = {f[cf[0]] / fn[cf[0]]:.0f} KiB and "
+                     f"WRITE_SIZE = {w[cw[0]] / wn[cw[0]]:.0f} KiB, so bytes = FETCH_SIZE x {kf:.3f} x 1024 "
+                     f"and WRITE_SIZE x {kw:.3f} x 1024.\n")
+        # per step: all k_rt_step dispatches over the class-0 dispatches
+        ktot_f = sum(v for k, v in f.items() if k[0] == KERNEL)
+        ktot_w = sum(v for k, v in w.items() if k[0] == KERNEL)
+        c0 = max((k for k in fn if k[0] == KERNEL), key=lambda k: k[1])
+        nsteps_f = fn[c0]
+        nsteps_w = wn[max((k for k in wn if k[0] == KERNEL), key=lambda k: k[1])]
+        fetch = ktot_f / nsteps_f * 1024 * kf
+        write = ktot_w / nsteps_w * 1024 * kw
+        alg = BYTES_PER_AGENT * a.agents
+        lines.append(f"k_rt_step HBM traffic per schedule step (all launches of the step, "
+                     f"{nsteps_f} steps): fetch {fetch / 1e6:.2f} MB + write {write / 1e6:.2f} MB = "
+                     f"{(fetch + write) / 1e6:.2f} MB ({(fetch + write) / alg:.2f}x the algorithmic "
+                     f"{alg / 1e6:.2f} MB)")
+        lines.append(f"traffic_bytes_per_step: {int(fetch + write)}")
+    return "\n".join(lines) + "\n"
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--trace", required=True)
+    p.add_argument("--fetch")
+    p.add_argument("--write")
+    p.add_argument("--agents", type=int, required=True)
+    p.add_argument("--skip", type=int, default=0, help="steps to skip (warm-up)")
+    p.add_argument("--count", type=int, default=0, help="steps to summarise (0: all)")
+    p.add_argument("--title", required=True)
+    p.add_argument("--cmd", required=True, help="the profiled command, for the header")
+    p.add_argument("--pmc-cmd", help="the command of the counter passes, if different")
+    print(summarize(p.parse_args()), end="")
+
+
+if __name__ == "__main__":
+    main()

@@ -94,13 +94,42 @@ def test_ensemble_runs_join_and_leave():
         assert_same(ens, c, s_c, 25, "c")
         assert_same(ens, d, s_d, 10, "d")
         assert ens.stats()["builds"] >= 3
-        # retiring without an admission keeps the run as a dead replica
+        # retiring the smallest run (at most a third of the tiles) keeps it
+        # as a dead replica: no rebuild
+        live = sorted([(s_b.agents, b, s_b, 45), (s_c.agents, c, s_c, 25), (s_d.agents, d, s_d, 10)])
         builds = ens.stats()["builds"]
-        ens.retire(c)
+        ens.retire(live[0][1])
         ens.run(9)
         assert ens.stats()["builds"] == builds and ens.stats()["runs"] == 2
-        assert_same(ens, b, s_b, 54, "b after c left")
-        assert_same(ens, d, s_d, 19, "d after c left")
+        for _, rid, s, k in live[1:]:
+            assert_same(ens, rid, s, k + 9, f"run {rid} after the smallest left")
+        # retiring the largest as well leaves dead tiles past a third of
+        # the layout: the next run compacts it
+        tiles = ens.stats()["tiles"]
+        ens.retire(live[2][1])
+        ens.run(5)
+        st = ens.stats()
+        assert st["builds"] == builds + 1 and st["runs"] == 1 and st["tiles"] < tiles
+        _, rid, s, k = live[1]
+        assert_same(ens, rid, s, k + 14, "last run after compaction")
+
+
+def test_ensemble_read_after_retire_and_admit():
+    # a read that rebuilds the layout (an admission is pending) while a
+    # retired run still sits in it: the read must address the run by id
+    # after the rebuild has dropped the retired one
+    s_a, s_b, s_c, s_d = uneven_specs(4, base_seed=130)
+    with Ensemble() as ens:
+        a, b, c = ens.admit([s_a, s_b, s_c])
+        ens.run(10)
+        ens.retire(a)
+        (d,) = ens.admit([s_d])
+        assert_same(ens, c, s_c, 10, "c read during the rebuild")
+        ens.run(5)
+        assert_same(ens, b, s_b, 15, "b")
+        assert_same(ens, d, s_d, 5, "d")
+        with pytest.raises(Exception):
+            ens.read(a)
 
 
 def test_ensemble_tuning_does_not_change_results():
