@@ -117,7 +117,11 @@ def summarize(a):
                      f"{a.skip}..{a.skip + len(sel) - 1} of {len(groups)} in the trace):\n")
         lines.append(f"- launches per step: {nl:.2f}")
         lines.append(f"- step span (first start to last end): mean {per_step:.1f} us, "
-                     f"min {min(spans):.1f}, max {max(spans):.1f}")
+                     f"median {statistics.median(spans):.1f}, min {min(spans):.1f}, "
+                     f"max {max(spans):.1f}")
+        region = (max(r["end"] for r in sel[-1]) - min(r["start"] for r in sel[0])) / 1000.0
+        lines.append(f"- the steps' region, first start to last end, per step: {region / len(sel):.1f} us "
+                     f"(with the gaps between steps: flag reads, re-plans)")
         alg = BYTES_PER_AGENT * a.agents
         lines.append(f"- algorithmic bytes: {BYTES_PER_AGENT} B x {a.agents} agents = {alg / 1e6:.2f} MB "
                      f"-> {alg / per_step / 1e3:.1f} GB/s = {alg / per_step / 1e3 / 8000 * 100:.2f} % "
