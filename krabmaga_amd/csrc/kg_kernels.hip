@@ -871,7 +871,6 @@ hipError_t launch_rt_step(const WorkList& wl, const RtDev& D, const RtCaps& caps
                           int cls, const uint32_t* list, uint32_t nlist, int mode, hipStream_t s) {
   const uint32_t grid = mode == kRtModeDeferred ? kRtOvfMax : (cls ? nlist : wl.ntiles);
   if (grid == 0) return hipSuccess;
-  if (grid == 0) return hipSuccess;
   const size_t lds = rt_lds_bytes(caps);
   if (caps.threads == 1024)
     hipLaunchKernelGGL(k_rt_step<1024>, dim3(grid), dim3(1024), lds, s, wl.reps, wl.nreps,
