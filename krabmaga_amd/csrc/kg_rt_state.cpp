@@ -57,28 +57,36 @@ int RtState::setup(const RtPlan& pl, const RtClassPlan& cls, uint64_t spill_cap,
   drop_graphs();
   if (!allocated || pl.stay_total > stay_alloc || pl.inbox_total > inbox_alloc ||
       pl.boff_total > boff_alloc || spill_cap > spill_alloc || nt > tiles_alloc) {
+    // grow every array together, by half again what this plan needs (at
+    // least what is held): ensembles re-plan as runs join, and each
+    // reallocation frees and allocates every array
+    auto grown = [](uint64_t need, uint64_t held) {
+      return std::max<uint64_t>(need + need / 2 + 64, held);
+    };
+    const uint64_t st = grown(pl.stay_total, stay_alloc), ib = grown(pl.inbox_total, inbox_alloc);
+    const uint64_t bo = grown(pl.boff_total, boff_alloc), sp = std::max(spill_cap, spill_alloc);
+    const uint64_t ti = grown(nt, tiles_alloc);
     release();
-    // grow with headroom so that re-planning rarely reallocates
-    stay_alloc = pl.stay_total + pl.stay_total / 8 + 64;
-    inbox_alloc = pl.inbox_total + pl.inbox_total / 8 + 64;
-    boff_alloc = pl.boff_total;
-    spill_alloc = spill_cap;
-    tiles_alloc = nt;
+    stay_alloc = st;
+    inbox_alloc = ib;
+    boff_alloc = bo;
+    spill_alloc = sp;
+    tiles_alloc = ti;
     hipError_t e = hipSuccess;
     auto ok = [&](hipError_t r) {
       if (e == hipSuccess) e = r;
     };
-    ok(dmalloc(&meta, nt));
-    ok(dmalloc(&dense, nt));
+    ok(dmalloc(&meta, tiles_alloc));
+    ok(dmalloc(&dense, tiles_alloc));
     for (int p = 0; p < 2; ++p) {
-      ok(dmalloc(&cnt[p], nt));
+      ok(dmalloc(&cnt[p], tiles_alloc));
       ok(dmalloc(&boff[p], boff_alloc));
       ok(stay[p].alloc(stay_alloc));
       ok(inbox[p].alloc(inbox_alloc));
       ok(spill[p].alloc(spill_alloc));
       ok(dmalloc(&spill_rep[p], spill_alloc));
     }
-    for (int k = 0; k < 3; ++k) ok(dmalloc(&icnt[k], nt));
+    for (int k = 0; k < 3; ++k) ok(dmalloc(&icnt[k], tiles_alloc));
     ok(dmalloc(&scnt, 3));
     ok(dmalloc(&flags, 2));
     ok(dmalloc(&ovf, 2 * (kRtOvfMax + 1)));
