@@ -52,6 +52,7 @@ struct kg_field {
   RepDev* d_rep = nullptr;
   uint32_t* scratch = nullptr;  // small query outputs
   uint32_t* tcount = nullptr;   // per-tile agent counts (tile-resident planning)
+  uint32_t* gflag = nullptr;    // per tile: k_flock left its birds to k_flock_global
   RepDev h_rep{};
 
   uint32_t nR = 0, nW = 0, nA = 0;
@@ -235,7 +236,7 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
 int flock_impl(kg_field* f, const uint64_t* gstep) {
   ProfScope ps(f, KG_KERNEL_FLOCK);
   KG_HIP(launch_flock(f->worklist(), f->R.b, f->offset, f->W.b, f->count, gstep, f->err,
-                      f->lds_cap, f->stream));
+                      f->lds_cap, f->gflag, (uint32_t)f->nR, f->stream));
   f->nW = f->nR;
   f->w_aligned = true;
   return KG_OK;
@@ -449,6 +450,7 @@ int kg_field_create(const kg_field_desc* desc, kg_field** out) {
   if ((e = f->R2.alloc(f->cap)) != hipSuccess) return fail(e);
   if ((e = f->W.alloc(f->cap)) != hipSuccess) return fail(e);
   if ((e = f->A.alloc(f->cap)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->gflag, std::max<uint32_t>(f->g.ntiles, 1u))) != hipSuccess) return fail(e);
   const uint32_t nb = f->g.nbins;
   const uint32_t nscan = (nb + kScanTile - 1) / kScanTile;
   if ((e = dmalloc(&f->count, nb)) != hipSuccess) return fail(e);
@@ -505,6 +507,7 @@ void kg_field_destroy(kg_field* f) {
   (void)hipFree(f->d_rep);
   (void)hipFree(f->scratch);
   (void)hipFree(f->tcount);
+  (void)hipFree(f->gflag);
   if (f->stream) (void)hipStreamDestroy(f->stream);
   delete f;
 }

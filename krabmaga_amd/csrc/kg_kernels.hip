@@ -211,10 +211,14 @@ hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint
 
 size_t flock_lds_bytes(int cap) { return flock_smem_bytes(cap); }
 
+// Tiles whose staged set exceeds the LDS capacity only set their flag in
+// `gflag`; k_flock_global then steps their birds one thread per bird, so a
+// crowded tile is spread over the whole GPU instead of one work-group.
 __global__ __launch_bounds__(kFlockThreads) void k_flock(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
     const uint32_t* __restrict__ offset, Buf W, uint32_t* __restrict__ count_next,
-    const uint64_t* __restrict__ gstep, uint32_t* __restrict__ err, int lds_cap) {
+    const uint64_t* __restrict__ gstep, uint32_t* __restrict__ err, int lds_cap,
+    uint32_t* __restrict__ gflag) {
   extern __shared__ __align__(16) uint8_t smem_raw[];
   __shared__ uint32_t s_seg_start[3 * kMaxStageCols];
   __shared__ uint32_t s_seg_len[3 * kMaxStageCols];
@@ -243,6 +247,7 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
     use_lds = total <= (uint32_t)lds_cap;
     __syncthreads();
   }
+  if (t == 0) gflag[item] = use_lds ? 0u : 1u;
   if (use_lds) {
     const FlockSmem S = flock_carve(smem_raw, lds_cap);
     for (uint32_t k = t; k < L; k += kFlockThreads) {
@@ -253,21 +258,35 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
     const uint32_t nown = s_nown;
     for (uint32_t o = t; o < nown; o += kFlockThreads)
       flock_agent_lds(rp, ft, S.own[o], s_col_base, S, W, count_next, err);
-    return;
   }
-  // global path
-  const uint32_t lo = offset[ft.tb0], hi = offset[ft.tb1];
-  for (uint32_t gi = lo + t; gi < hi; gi += kFlockThreads)
-    flock_agent_global(rp, ft.step, gi, R, offset, W, count_next, err);
+}
+
+// The birds of flagged tiles, one thread per read-buffer slot (one replica).
+__global__ __launch_bounds__(kFlockThreads) void k_flock_global(
+    const RepDev* __restrict__ rep, Buf R, const uint32_t* __restrict__ offset, Buf W,
+    uint32_t* __restrict__ count_next, const uint64_t* __restrict__ gstep,
+    uint32_t* __restrict__ err, const uint32_t* __restrict__ gflag, uint32_t n) {
+  const uint32_t gi = blockIdx.x * kFlockThreads + threadIdx.x;
+  if (gi >= n) return;
+  const RepDev& rp = *rep;
+  int32_t cx, cy;
+  discretize(rp.g, R.x[gi], R.y[gi], &cx, &cy);
+  if (!cell_valid(rp.g, cx, cy) || !gflag[tile_of(rp.g, cx, cy)]) return;
+  flock_agent_global(rp, rp.step_base + *gstep, gi, R, offset, W, count_next, err);
 }
 
 hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W,
                         uint32_t* count_next, const uint64_t* gstep, uint32_t* err, int lds_cap,
-                        hipStream_t s) {
+                        uint32_t* gflag, uint32_t nagents, hipStream_t s) {
   if (wl.ntiles == 0) return hipSuccess;
+  if (wl.nreps != 1) return hipErrorInvalidValue;  // k_flock_global steps one replica
   hipLaunchKernelGGL(k_flock, dim3(wl.ntiles), dim3(kFlockThreads), flock_lds_bytes(lds_cap), s,
                      wl.reps, wl.nreps, wl.tile_prefix, R, offset, W, count_next, gstep, err,
-                     lds_cap);
+                     lds_cap, gflag);
+  if (nagents)
+    hipLaunchKernelGGL(k_flock_global, dim3((nagents + kFlockThreads - 1) / kFlockThreads),
+                       dim3(kFlockThreads), 0, s, wl.reps, R, offset, W, count_next, gstep, err,
+                       gflag, nagents);
   return hipGetLastError();
 }
 

@@ -530,6 +530,7 @@ int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step) {
   std::vector<uint32_t> seg_start(3 * kMaxStageCols), seg_len(3 * kMaxStageCols),
       col_base(kMaxStageCols + 1);
   std::vector<uint8_t> smem(flock_smem_bytes(f->lds_cap));
+  std::vector<uint8_t> gflag(f->g.ntiles, 0);
   for (uint32_t item = 0; item < f->g.ntiles; ++item) {
     const FlockTile ft = flock_tile(&f->rep, 1, prefix, item, 0);
     const RepDev& rp = f->rep;
@@ -560,10 +561,16 @@ int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step) {
         flock_agent_lds(rp, ft, S.own[o], col_base.data(), S, W, f->count.data(), &f->err);
     } else {
       ++f->global_tiles;
-      const uint32_t lo = f->offset[ft.tb0], hi = f->offset[ft.tb1];
-      for (uint32_t gi = lo; gi < hi; ++gi)
-        flock_agent_global(rp, ft.step, gi, R, f->offset.data(), W, f->count.data(), &f->err);
+      gflag[item] = 1;
     }
+  }
+  // k_flock_global: the flagged tiles' birds, one per thread, in any order
+  for (uint32_t gi : order(0, f->nR, f->shuffle_seed)) {
+    int32_t cx, cy;
+    discretize(f->rep.g, R.x[gi], R.y[gi], &cx, &cy);
+    if (!cell_valid(f->rep.g, cx, cy) || !gflag[tile_of(f->rep.g, cx, cy)]) continue;
+    flock_agent_global(f->rep, f->rep.step_base, gi, R, f->offset.data(), W, f->count.data(),
+                       &f->err);
   }
   f->w_aligned = true;
   return f->err ? -3 : 0;
