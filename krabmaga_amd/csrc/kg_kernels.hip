@@ -293,12 +293,14 @@ hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W
 // ---------------------------------------------------------------------------
 // bin sort: every bin of a freshly scattered read buffer into (cell y, id)
 // order, ties by position; one block per (replica, tile).  A bin of at most
-// one wave's width is ranked by one wave in registers and rewritten in place
-// (every lane has loaded its entry before any lane stores); larger bins are
+// four times a wave's width is ranked by one wave in registers and rewritten
+// in place (every lane has loaded its entries before any lane stores); larger bins are
 // ranked by the whole block against keys staged in LDS, written to the
 // scratch buffer T at the same indices and copied back.
 
 constexpr int kSortThreads = 256;
+constexpr int kSortSlots = 4;                              // entries per lane on the wave path
+constexpr uint32_t kSortWaveMax = kSortSlots * kWave;      // largest bin one wave sorts
 constexpr uint32_t kSortChunk = 2048;  // keys staged per pass for the large bins
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t lane) {
@@ -322,33 +324,50 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
   const uint32_t nb = 1u << (g.tx_log2 + g.tr_log2);
   const uint32_t b0 = reps[r].bin_base + ((item - tile_prefix[r]) << (g.tx_log2 + g.tr_log2));
 
-  // bins of up to 64 entries: one wave each
+  // bins of up to kSortWaveMax entries: one wave each, entry lane + 64 u in
+  // slot u of the lane's registers
   for (uint32_t k = wid; k < nb; k += kSortThreads / kWave) {
     const uint32_t s = offset[b0 + k];
     const uint32_t m = offset[b0 + k + 1] - s;
-    if (m <= 1u || m > (uint32_t)kWave) continue;
-    const bool act = lane < m;
-    uint32_t id = 0;
-    float x = 0.0f, y = 0.0f, hx = 0.0f, hy = 0.0f;
-    if (act) {
-      id = R.id[s + lane];
-      x = R.x[s + lane];
-      y = R.y[s + lane];
-      hx = R.hx[s + lane];
-      hy = R.hy[s + lane];
+    if (m <= 1u || m > kSortWaveMax) continue;
+    uint32_t idv[kSortSlots], rk[kSortSlots];
+    float xv[kSortSlots], yv[kSortSlots], hxv[kSortSlots], hyv[kSortSlots];
+    uint64_t kv[kSortSlots];
+#pragma unroll
+    for (int u = 0; u < kSortSlots; ++u) {
+      const uint32_t e = lane + (uint32_t)u * kWave;
+      idv[u] = 0;
+      xv[u] = yv[u] = hxv[u] = hyv[u] = 0.0f;
+      if (e < m) {
+        idv[u] = R.id[s + e];
+        xv[u] = R.x[s + e];
+        yv[u] = R.y[s + e];
+        hxv[u] = R.hx[s + e];
+        hyv[u] = R.hy[s + e];
+      }
+      kv[u] = e < m ? bin_key(g, yv[u], idv[u]) : ~0ull;
+      rk[u] = 0;
     }
-    const uint64_t key = act ? bin_key(g, y, id) : ~0ull;
-    uint32_t rank = 0;
     for (uint32_t j = 0; j < m; ++j) {
-      const uint64_t kj = readlane64(key, j);
-      rank += (kj < key) || (kj == key && j < lane);
+      const uint32_t ju = j / kWave;  // the same on every lane
+      const uint64_t src = ju == 0 ? kv[0] : (ju == 1 ? kv[1] : (ju == 2 ? kv[2] : kv[3]));
+      const uint64_t kj = readlane64(src, j & (kWave - 1));
+#pragma unroll
+      for (int u = 0; u < kSortSlots; ++u) {
+        const uint32_t e = lane + (uint32_t)u * kWave;
+        rk[u] += (kj < kv[u]) || (kj == kv[u] && j < e);
+      }
     }
-    if (act) {
-      R.id[s + rank] = id;
-      R.x[s + rank] = x;
-      R.y[s + rank] = y;
-      R.hx[s + rank] = hx;
-      R.hy[s + rank] = hy;
+#pragma unroll
+    for (int u = 0; u < kSortSlots; ++u) {
+      const uint32_t e = lane + (uint32_t)u * kWave;
+      if (e < m) {
+        R.id[s + rk[u]] = idv[u];
+        R.x[s + rk[u]] = xv[u];
+        R.y[s + rk[u]] = yv[u];
+        R.hx[s + rk[u]] = hxv[u];
+        R.hy[s + rk[u]] = hyv[u];
+      }
     }
   }
 
@@ -358,7 +377,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
     __syncthreads();
     if (k0 + (uint32_t)t < nb) {
       const uint32_t b = b0 + k0 + (uint32_t)t;
-      if (offset[b + 1] - offset[b] > (uint32_t)kWave) s_big[atomicAdd(&s_nbig, 1u)] = b;
+      if (offset[b + 1] - offset[b] > kSortWaveMax) s_big[atomicAdd(&s_nbig, 1u)] = b;
     }
     __syncthreads();
     const uint32_t nbig = s_nbig;
