@@ -92,8 +92,50 @@ __global__ __launch_bounds__(256) void k_scatter_aligned(const RepDev* __restric
   const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
   const TileRange tr = tile_range(reps, nreps, tile_prefix, item, offset_cur);
   const RepDev& rp = reps[tr.r];
-  for (uint32_t i = tr.lo + threadIdx.x; i < tr.hi; i += blockDim.x)
-    scatter_aligned_one(rp, R, W, i, cursor, R2);
+  const Geom& g = rp.g;
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  // The read buffer is in bin order and birds move little, so neighbouring
+  // lanes mostly share a destination bin: one cursor atomic per run of
+  // equal bins in a wave (the bin sort that follows fixes the order inside
+  // a bin).  Every lane takes part in every round for the wave operations.
+  for (uint32_t i0 = tr.lo; i0 < tr.hi; i0 += blockDim.x) {
+    const uint32_t i = i0 + threadIdx.x;
+    bool valid = i < tr.hi;
+    float x = 0.0f, y = 0.0f;
+    uint32_t bin = 0xFFFFFFFFu;
+    if (valid) {
+      x = W.x[i];
+      y = W.y[i];
+      int32_t cx, cy;
+      discretize(g, x, y, &cx, &cy);
+      valid = cell_valid(g, cx, cy);  // an invalid cell was reported by the flock kernel
+      if (valid) bin = rp.bin_base + bin_of(g, cx, cy);
+    }
+    const uint32_t prev = __shfl_up(bin, 1, kWave);
+    const bool head = valid && (lane == 0 || prev != bin);
+    const uint64_t heads = __ballot(head);
+    const uint64_t vmask = __ballot(valid);
+    const uint64_t upto = lane == kWave - 1 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const uint64_t below = heads & upto;
+    const int h = below ? 63 - __clzll((long long)below) : 0;
+    // the run of head h ends at the next head
+    const uint64_t after = heads & ~upto;
+    const int nh = after ? __ffsll((long long)after) - 1 : kWave;
+    const uint64_t from_h = ~((1ull << h) - 1ull);
+    const uint64_t to_nh = nh == kWave ? ~0ull : ((1ull << nh) - 1ull);
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&cursor[bin], (uint32_t)__popcll(vmask & from_h & to_nh));
+    base = __shfl(base, h, kWave);
+    if (valid) {
+      // base plus the valid lanes of the run before this one
+      const uint32_t pos = base + (uint32_t)__popcll(vmask & from_h & (upto >> 1));
+      R2.x[pos] = x;
+      R2.y[pos] = y;
+      R2.hx[pos] = W.hx[i];
+      R2.hy[pos] = W.hy[i];
+      R2.id[pos] = R.id[i];
+    }
+  }
 }
 
 hipError_t launch_scatter_aligned(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
