@@ -1,0 +1,66 @@
+"""krabmaga_amd.profsum: rocprofv3 CSVs grouped into schedule steps."""
+import csv
+
+from krabmaga_amd import profsum
+
+TRACE_COLS = ["Kernel_Name", "Grid_Size", "Workgroup_Size", "Start_Timestamp", "End_Timestamp"]
+PMC_COLS = ["Kernel_Name", "Grid_Size", "Counter_Name", "Counter_Value"]
+RT = "void kg::k_rt_step<256>(kg::RepDev const*, int)"
+RT512 = "void kg::k_rt_step<512>(kg::RepDev const*, int)"
+
+
+def write(path, cols, rows):
+    with open(path, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(cols)
+        w.writerows(rows)
+
+
+def trace_rows():
+    rows = []
+    # three steps: class 0 (the largest 256-thread grid), class 1 started
+    # just before class 0, and the deferred-tile launch after both
+    for k in range(3):
+        t = 1_000_000 * (k + 1)
+        rows.append([RT512, 4096, 512, t - 2_000, t + 150_000])
+        rows.append([RT, 1235200, 256, t, t + 120_000])
+        rows.append([RT512, 131072, 512, t + 160_000, t + 200_000])
+    rows.append(["kg::k_add_u64(unsigned long*, unsigned long)", 64, 64, 5_000_000, 5_001_000])
+    return rows
+
+
+def test_launches_group_into_steps(tmp_path):
+    p = tmp_path / "t.csv"
+    write(p, TRACE_COLS, trace_rows())
+    groups = profsum.steps_of(profsum.load_trace(str(p)))
+    assert len(groups) == 3
+    for g in groups:
+        assert len(g) == 3
+        span = (max(r["end"] for r in g) - min(r["start"] for r in g)) / 1000.0
+        assert span == 202.0
+
+
+def test_summary_reports_span_and_traffic(tmp_path):
+    t = tmp_path / "t.csv"
+    write(t, TRACE_COLS, trace_rows())
+    copy = "__amd_rocclr_copyBuffer"
+    fetch = [[copy, 131072, "FETCH_SIZE", 524288.0]]  # a 1 GiB copy reads half in this counter
+    write_ = [[copy, 131072, "WRITE_SIZE", 1048576.0]]
+    for _ in range(3):
+        fetch += [[RT, 1235200, "FETCH_SIZE", 1000.0], [RT512, 4096, "FETCH_SIZE", 24.0]]
+        write_ += [[RT, 1235200, "WRITE_SIZE", 800.0], [RT512, 4096, "WRITE_SIZE", 0.0]]
+    fp, wp = tmp_path / "f.csv", tmp_path / "w.csv"
+    write(fp, PMC_COLS, fetch)
+    write(wp, PMC_COLS, write_)
+
+    class A:
+        pass
+
+    a = A()
+    a.trace, a.fetch, a.write = str(t), str(fp), str(wp)
+    a.agents, a.skip, a.count = 1000, 0, 0
+    a.title, a.cmd, a.pmc_cmd = "test", "python3 bench.py", None
+    out = profsum.summarize(a)
+    assert "step_span_us: 202.0" in out
+    # per step: (1000 + 24) KiB x 2 fetched + 800 KiB written
+    assert f"traffic_bytes_per_step: {(1024 * 2 + 800) * 1024}" in out
