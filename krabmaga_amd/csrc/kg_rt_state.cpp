@@ -64,7 +64,7 @@ int RtState::setup(const RtPlan& pl, const RtClassPlan& cls, uint64_t spill_cap,
       return std::max<uint64_t>(need + need / 2 + 64, held);
     };
     const uint64_t st = grown(pl.stay_total, stay_alloc), ib = grown(pl.inbox_total, inbox_alloc);
-    const uint64_t bo = grown(pl.boff_total, boff_alloc), sp = std::max(spill_cap, spill_alloc);
+    const uint64_t bo = grown(pl.boff_total, boff_alloc), sp = grown(spill_cap, spill_alloc);
     const uint64_t ti = grown(nt, tiles_alloc);
     release();
     stay_alloc = st;
