@@ -212,6 +212,36 @@ hipError_t copy_buf(Buf dst, uint32_t dbase, Buf src, uint32_t sbase, uint32_t n
                         hipMemcpyDeviceToDevice, s);
 }
 
+// Halves the tile height of the runs holding the fullest tiles (at least
+// half the fullest tile's count), for runs whose flocks have outgrown every
+// LDS class; false if none of them can shrink.
+constexpr int kShrinkAttempts = 8;
+
+bool shrink_crowded(kg_ensemble* e, const std::vector<uint32_t>& counts) {
+  uint32_t top = 0;
+  for (uint32_t c : counts) top = std::max(top, c);
+  bool any = false;
+  for (EnsRun& r : e->runs) {
+    uint32_t m = 0;
+    for (uint32_t t = r.tile0; t < r.tile0 + r.ntiles; ++t) m = std::max(m, counts[t]);
+    if (2ull * m < top || r.g.ty <= 1) continue;
+    const uint32_t ty = (uint32_t)r.g.ty / 2;
+    const uint32_t hb = std::min<uint32_t>((uint32_t)r.g.hb, ty);
+    Geom g;
+    std::string why;
+    if (!make_geom_rt(r.desc.width, r.desc.height, r.desc.discretization, 1, r.desc.agents,
+                      (uint32_t)r.g.tx, ty, hb, &g, &why))
+      continue;
+    RepDev probe{};
+    probe.g = g;
+    set_rep_params(&probe, r.desc.params);
+    if (!probe.lds_ok) continue;
+    r.g = g;
+    any = true;
+  }
+  return any;
+}
+
 // Builds the layout for the current run list: every run's agents go through
 // the append buffer, the bins and a fresh tile-resident plan.
 int rebuild(kg_ensemble* e) {
@@ -254,84 +284,99 @@ int rebuild(kg_ensemble* e) {
     r.app_base = base;
     base += r.desc.agents;
   }
-  // the old layout is no longer read: grow the tables and the unload buffer
-  if (int rc = reserve(e, std::max<uint64_t>(bins, 1), std::max<size_t>(e->runs.size(), 1),
-                       std::max<uint64_t>(tiles, 1)))
-    return rc;
-  if (int rc = reserve_buf(e, &e->A, &e->cap_a, std::max<uint64_t>(agents, 1))) return rc;
-  // replica table
-  const uint32_t nreps = (uint32_t)e->runs.size();
-  e->h_reps.assign(nreps, RepDev{});
-  e->h_prefix.assign(2 * (nreps + 1), 0u);
-  std::vector<RtRepInfo> infos(nreps);
-  uint32_t bin_base = 0, tile0 = 0, boff_base = 0;
-  for (uint32_t i = 0; i < nreps; ++i) {
-    EnsRun& r = e->runs[i];
-    RepDev& rp = e->h_reps[i];
-    rp.g = r.g;
-    set_rep_params(&rp, r.desc.params);
-    rp.bin_base = bin_base;
-    rp.app_base = r.app_base;
-    rp.app_n = r.desc.agents;
-    rp.boff_base = boff_base;
-    // the first tile-resident launch after this build runs step G; a run's
-    // next schedule step is steps_done
-    rp.step_base = r.steps_done - G;
-    r.tile0 = tile0;
-    r.ntiles = r.g.ntiles;
-    infos[i] = RtRepInfo{r.g, rp.disc_dist, tile0};
-    e->h_prefix[i] = tile0;
-    e->h_prefix[nreps + 1 + i] = r.app_base;
-    bin_base += r.g.nbins;
-    boff_base += r.g.ntiles * (uint32_t)(r.g.tx * r.g.tr + 1);
-    tile0 += r.g.ntiles;
-  }
-  e->h_prefix[nreps] = tile0;
-  e->h_prefix[2 * nreps + 1] = base;
-  e->nreps = nreps;
-  e->ntiles = tile0;
-  e->nbins = bin_base;
-  e->nagents = base;
-  e->dead_tiles = 0;
-  e->built = true;
-  e->dirty = false;
-  e->replan = false;
-  if (nreps == 0) return KG_OK;
-  KG_HIP(hipMemcpyAsync(e->d_reps, e->h_reps.data(), nreps * sizeof(RepDev),
-                        hipMemcpyHostToDevice, e->stream));
-  KG_HIP(hipMemcpyAsync(e->d_prefix, e->h_prefix.data(), e->h_prefix.size() * sizeof(uint32_t),
-                        hipMemcpyHostToDevice, e->stream));
-  // Field::lazy_update (field_2d.rs:905-921) of every run's agents at once
-  const WorkList w = e->wl();
-  KG_HIP(launch_count_append(w, e->A2.b, e->count, e->err, e->stream));
-  KG_HIP(launch_scan(e->count, e->offset, e->cursor, e->nbins, e->ss, nullptr, e->stream));
-  KG_HIP(launch_scatter_append(w, e->A2.b, e->cursor, e->R.b, e->stream));
-  // plan from the per-tile counts
-  KG_HIP(launch_tile_counts(w, e->offset, e->tcount, e->stream));
-  std::vector<uint32_t> counts(e->ntiles), nb(e->ntiles);
-  KG_HIP(hipMemcpyAsync(counts.data(), e->tcount, e->ntiles * sizeof(uint32_t),
-                        hipMemcpyDeviceToHost, e->stream));
-  uint32_t errflag = 0;
-  KG_HIP(hipMemcpyAsync(&errflag, e->err, sizeof(errflag), hipMemcpyDeviceToHost, e->stream));
-  KG_HIP(hipStreamSynchronize(e->stream));
-  if (errflag)
-    return set_error(KG_E_RANGE, "an agent discretized outside its run's bag grid");
-  std::vector<double> expect(e->ntiles);
-  for (uint32_t i = 0; i < nreps; ++i) {
-    const EnsRun& r = e->runs[i];
-    std::vector<double> ex;
-    rt_expected_counts(r.g, r.desc.agents, &ex);
-    std::copy(ex.begin(), ex.end(), expect.begin() + r.tile0);
-    std::fill(nb.begin() + r.tile0, nb.begin() + r.tile0 + r.ntiles,
-              (uint32_t)(r.g.tx * r.g.tr));
-  }
   RtPlan plan;
-  rt_plan(counts, nb, expect, &plan);
   RtClassPlan cls;
-  const size_t lds_normal = e->tuning.lds_normal ? e->tuning.lds_normal : kRtLdsNormal;
-  rt_classes(&plan, counts, infos, lds_normal, e->lds_limit, &cls);
-  if (!cls.fits)
-    return set_error(KG_E_ADMIT, "the runs' tiles do not fit the LDS budget of the tile-resident step");
+  WorkList w{};
+  for (int attempt = 0;; ++attempt) {
+    uint64_t lbins = 0, ltiles = 0;
+    for (const EnsRun& r : e->runs) {
+      lbins += r.g.nbins;
+      ltiles += r.g.ntiles;
+    }
+    if (lbins >= (1ull << 31) || ltiles >= (1ull << 31))
+      return set_error(KG_E_ADMIT, "ensemble too large for 32-bit indices");
+    // the old layout is no longer read: grow the tables and the unload buffer
+    if (int rc = reserve(e, std::max<uint64_t>(lbins, 1), std::max<size_t>(e->runs.size(), 1),
+                         std::max<uint64_t>(ltiles, 1)))
+      return rc;
+    if (int rc = reserve_buf(e, &e->A, &e->cap_a, std::max<uint64_t>(agents, 1))) return rc;
+    // replica table
+    const uint32_t nreps = (uint32_t)e->runs.size();
+    e->h_reps.assign(nreps, RepDev{});
+    e->h_prefix.assign(2 * (nreps + 1), 0u);
+    std::vector<RtRepInfo> infos(nreps);
+    uint32_t bin_base = 0, tile0 = 0, boff_base = 0;
+    for (uint32_t i = 0; i < nreps; ++i) {
+      EnsRun& r = e->runs[i];
+      RepDev& rp = e->h_reps[i];
+      rp.g = r.g;
+      set_rep_params(&rp, r.desc.params);
+      rp.bin_base = bin_base;
+      rp.app_base = r.app_base;
+      rp.app_n = r.desc.agents;
+      rp.boff_base = boff_base;
+      // the first tile-resident launch after this build runs step G; a run's
+      // next schedule step is steps_done
+      rp.step_base = r.steps_done - G;
+      r.tile0 = tile0;
+      r.ntiles = r.g.ntiles;
+      infos[i] = RtRepInfo{r.g, rp.disc_dist, tile0};
+      e->h_prefix[i] = tile0;
+      e->h_prefix[nreps + 1 + i] = r.app_base;
+      bin_base += r.g.nbins;
+      boff_base += r.g.ntiles * (uint32_t)(r.g.tx * r.g.tr + 1);
+      tile0 += r.g.ntiles;
+    }
+    e->h_prefix[nreps] = tile0;
+    e->h_prefix[2 * nreps + 1] = base;
+    e->nreps = nreps;
+    e->ntiles = tile0;
+    e->nbins = bin_base;
+    e->nagents = base;
+    e->dead_tiles = 0;
+    e->built = true;
+    e->dirty = false;
+    e->replan = false;
+    if (nreps == 0) return KG_OK;
+    KG_HIP(hipMemcpyAsync(e->d_reps, e->h_reps.data(), nreps * sizeof(RepDev),
+                          hipMemcpyHostToDevice, e->stream));
+    KG_HIP(hipMemcpyAsync(e->d_prefix, e->h_prefix.data(), e->h_prefix.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, e->stream));
+    // Field::lazy_update (field_2d.rs:905-921) of every run's agents at once
+    w = e->wl();
+    KG_HIP(launch_count_append(w, e->A2.b, e->count, e->err, e->stream));
+    KG_HIP(launch_scan(e->count, e->offset, e->cursor, e->nbins, e->ss, nullptr, e->stream));
+    KG_HIP(launch_scatter_append(w, e->A2.b, e->cursor, e->R.b, e->stream));
+    // plan from the per-tile counts
+    KG_HIP(launch_tile_counts(w, e->offset, e->tcount, e->stream));
+    std::vector<uint32_t> counts(e->ntiles), nb(e->ntiles);
+    KG_HIP(hipMemcpyAsync(counts.data(), e->tcount, e->ntiles * sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, e->stream));
+    uint32_t errflag = 0;
+    KG_HIP(hipMemcpyAsync(&errflag, e->err, sizeof(errflag), hipMemcpyDeviceToHost, e->stream));
+    KG_HIP(hipStreamSynchronize(e->stream));
+    if (errflag)
+      return set_error(KG_E_RANGE, "an agent discretized outside its run's bag grid");
+    std::vector<double> expect(e->ntiles);
+    for (uint32_t i = 0; i < nreps; ++i) {
+      const EnsRun& r = e->runs[i];
+      std::vector<double> ex;
+      rt_expected_counts(r.g, r.desc.agents, &ex);
+      std::copy(ex.begin(), ex.end(), expect.begin() + r.tile0);
+      std::fill(nb.begin() + r.tile0, nb.begin() + r.tile0 + r.ntiles,
+                (uint32_t)(r.g.tx * r.g.tr));
+    }
+    plan = RtPlan{};
+    rt_plan(counts, nb, expect, &plan);
+    cls = RtClassPlan{};
+    const size_t lds_normal = e->tuning.lds_normal ? e->tuning.lds_normal : kRtLdsNormal;
+    rt_classes(&plan, counts, infos, lds_normal, e->lds_limit, &cls);
+    if (cls.fits) break;
+    // tiles too crowded for every LDS class: shorter tiles for the runs that
+    // hold the fullest ones, and bin again
+    if (attempt >= kShrinkAttempts || !shrink_crowded(e, counts))
+      return set_error(KG_E_ADMIT, "the runs' tiles do not fit the LDS budget of the tile-resident step");
+  }
   if (int rc = e->rt.setup(plan, cls, e->nagents, e->stream)) return rc;
   if (int rc = e->rt.reset(G, e->stream)) return rc;
   const RtDev D = e->rt.dev();

@@ -423,7 +423,7 @@ int kg_field_create(const kg_field_desc* desc, kg_field** out) {
   }
   if (!make_geom_rt(desc->width, desc->height, desc->discretization, desc->toroidal,
                     desc->capacity, desc->tile_cols, desc->tile_rows, desc->bin_rows, &f->g,
-                    &why)) {
+                    &why, true)) {
     delete f;
     return set_error(KG_E_INVALID, "%s", why.c_str());
   }

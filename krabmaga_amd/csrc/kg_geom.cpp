@@ -338,7 +338,7 @@ void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts,
 }
 
 bool make_geom_rt(float w, float h, float d, int tor, uint64_t expected_agents, uint32_t tx,
-                  uint32_t ty, uint32_t hb, Geom* g, std::string* why) {
+                  uint32_t ty, uint32_t hb, Geom* g, std::string* why, bool rows_nearest) {
   if (!make_geom(w, h, d, tor, expected_agents, hb, g, why)) return false;
   const double cells = (double)g->dw * (double)g->dh;
   const double dens = expected_agents ? (double)expected_agents / cells : 0.1;
@@ -349,8 +349,11 @@ bool make_geom_rt(float w, float h, float d, int tor, uint64_t expected_agents, 
     int tl = 0;
     while ((1 << (tl + 1)) <= wdt && tl < 6) ++tl;
     tx = 1u << tl;
+    // rows: the power of two at or below the wanted height, or the nearest
+    // one (in ratio) when asked
     int yl = 0;
     while ((double)(1 << (yl + 1)) <= tile_cells / tx && yl < 12) ++yl;
+    if (rows_nearest && yl < 12 && tile_cells / tx >= 1.5 * (double)(1 << yl)) ++yl;
     ty = 1u << yl;
   }
   if (!hb) {

@@ -78,8 +78,13 @@ void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts,
                 const std::vector<RtRepInfo>& reps, size_t lds_normal, size_t lds_limit,
                 RtClassPlan* out);
 
-// Geometry for the tile-resident layout: tiles of about `own_target` agents.
+// Geometry for the tile-resident layout: tiles of about 448 agents at the
+// expected density unless tx, ty, hb are given (powers of two).  The tile
+// height is the power of two at or below the wanted height, or the nearest
+// one with rows_nearest (single fields: taller tiles share the per-tile
+// work over more birds; ensembles keep the shorter tiles, which stay inside
+// the LDS classes as their runs' flocks densify).
 bool make_geom_rt(float w, float h, float d, int tor, uint64_t expected_agents, uint32_t tx,
-                  uint32_t ty, uint32_t hb, Geom* g, std::string* why);
+                  uint32_t ty, uint32_t hb, Geom* g, std::string* why, bool rows_nearest = false);
 
 }  // namespace kg

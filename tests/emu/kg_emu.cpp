@@ -109,7 +109,7 @@ emu_field* emu_create_rt(float w, float h, float d, int tor, uint32_t cap, uint3
                          uint32_t ty, uint32_t hb) {
   emu_field* f = new emu_field();
   std::string why;
-  if (!make_geom_rt(w, h, d, tor, cap, tx, ty, hb, &f->g, &why)) {
+  if (!make_geom_rt(w, h, d, tor, cap, tx, ty, hb, &f->g, &why, true)) {
     delete f;
     return nullptr;
   }

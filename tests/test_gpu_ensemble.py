@@ -141,6 +141,18 @@ def test_ensemble_tuning_does_not_change_results():
             assert_same(ens, rid, s, 25, f"tuned run {rid}")
 
 
+def test_ensemble_shortens_tiles_that_fit_no_lds_class():
+    # 20,000 birds on a 200 x 200 torus (22 per cell) with 4 x 64-cell tiles:
+    # about 5,600 birds a tile, beyond every LDS class, so the build halves
+    # the tile height until the plan fits; the run keeps its bits
+    s = spec(20000, 150, density=0.5)
+    with Ensemble(Tuning(tile_cols=4, tile_rows=64, bin_rows=8)) as ens:
+        (rid,) = ens.admit([s])
+        ens.run(12)
+        assert ens.stats()["tiles"] > 8  # 4 x 64 tiles would be 8
+        assert_same(ens, rid, s, 12, "dense run with shortened tiles")
+
+
 def test_ensemble_rejects_non_toroidal_runs():
     s = spec(500, 1)
     s_bad = RunSpec(width=s.width, height=s.height, agents=500, params=s.params)
