@@ -3,9 +3,9 @@
 One schedule step of the tile-resident path is several launches of
 k_rt_step: one per tile class (side by side on their own streams) and one
 for the tiles deferred in that step.  This module groups the launches of a
-kernel trace into steps -- each step has exactly one class-0 launch (the
-largest 256-thread grid, one work-group per tile); a launch belongs to the
-last class-0 launch that started before it, give or take `slack_us` -- and
+kernel trace into steps -- each step has exactly one class-0 launch (one
+work-group per tile: the most work-groups of any launch); a launch belongs
+to the last class-0 launch that started before it, give or take `slack_us` -- and
 reports per-step spans (first start to last end) beside per-launch
 durations.  With FETCH_SIZE / WRITE_SIZE passes it reports the HBM bytes per
 step, scaled by what the same counters read for a device-to-device copy of
@@ -51,14 +51,15 @@ def steps_of(trace, slack_us=20.0):
     rt = [r for r in trace if r["name"] == KERNEL]
     if not rt:
         return []
-    c0_grid = max(r["grid"] for r in rt if r["wg"] == 256)
-    c0 = [r for r in rt if r["wg"] == 256 and r["grid"] == c0_grid]
+    c0_wgs = max(r["grid"] // r["wg"] for r in rt)
+    is_c0 = [r["grid"] // r["wg"] == c0_wgs for r in rt]
+    c0 = [r for r, c in zip(rt, is_c0) if c]
     starts = [r["start"] for r in c0]
     groups = [[r] for r in c0]
     slack = int(slack_us * 1000)
     j = 0
-    for r in rt:
-        if r["wg"] == 256 and r["grid"] == c0_grid:
+    for r, c in zip(rt, is_c0):
+        if c:
             continue
         while j + 1 < len(starts) and starts[j + 1] <= r["start"] + slack:
             j += 1
@@ -146,14 +147,13 @@ def summarize(a):
         lines.append(f"\nCalibration: the 1 GiB copy reads FETCH_SIZE = {f[cf[0]] / fn[cf[0]]:.0f} KiB and "
                      f"WRITE_SIZE = {w[cw[0]] / wn[cw[0]]:.0f} KiB, so bytes = FETCH_SIZE x {kf:.3f} x 1024 "
                      f"and WRITE_SIZE x {kw:.3f} x 1024.\n")
-        # per step: all k_rt_step dispatches over the class-0 dispatches
+        # per step: all k_rt_step dispatches over the steps of the trace (the
+        # counter passes run the same command)
         ktot_f = sum(v for k, v in f.items() if k[0] == KERNEL)
         ktot_w = sum(v for k, v in w.items() if k[0] == KERNEL)
-        c0 = max((k for k in fn if k[0] == KERNEL), key=lambda k: k[1])
-        nsteps_f = fn[c0]
-        nsteps_w = wn[max((k for k in wn if k[0] == KERNEL), key=lambda k: k[1])]
+        nsteps_f = len(groups)
         fetch = ktot_f / nsteps_f * 1024 * kf
-        write = ktot_w / nsteps_w * 1024 * kw
+        write = ktot_w / nsteps_f * 1024 * kw
         alg = BYTES_PER_AGENT * a.agents
         lines.append(f"k_rt_step HBM traffic per schedule step (all launches of the step, "
                      f"{nsteps_f} steps): fetch {fetch / 1e6:.2f} MB + write {write / 1e6:.2f} MB = "
