@@ -65,8 +65,10 @@ constexpr uint32_t kRtOvfMax = 256;
 // Dynamic LDS budget of a class-0 work-group: four per CU (160 KB) with the
 // kernel's static LDS beside them (at 40 KB only three fit).
 constexpr size_t kRtLdsNormal = 28 * 1024;
-// Deferrals per step past which the host re-plans a field.
-constexpr uint32_t kRtReplanDeferred = 4;
+// Deferrals per step past which the host re-plans a field: any (a step
+// with a deferred tile pays one more work-group's latency after its class
+// launches; a field's re-plan costs about one step's worth every few dozen).
+constexpr uint32_t kRtReplanDeferred = 1;
 // The same for an ensemble, whose re-plan rebuilds every run's layout (many
 // milliseconds) while a step's deferred-tile launch costs tens of
 // microseconds.
