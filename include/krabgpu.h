@@ -223,7 +223,7 @@ typedef struct kg_tuning {
     uint32_t tile_cols;  /* tile width in cells (power of two) */
     uint32_t tile_rows;  /* tile height in cells (power of two) */
     uint32_t bin_rows;   /* bin height in cells (power of two) */
-    uint32_t lds_normal; /* LDS bytes per normal-class work-group */
+    uint32_t lds_normal; /* LDS bytes per class-0 work-group; 0: what keeps a CU full */
 } kg_tuning;
 
 /* Device bytes a run of this description occupies in an ensemble. */

@@ -369,7 +369,7 @@ int rebuild(kg_ensemble* e) {
     plan = RtPlan{};
     rt_plan(counts, nb, expect, &plan);
     cls = RtClassPlan{};
-    const size_t lds_normal = e->tuning.lds_normal ? e->tuning.lds_normal : kRtLdsNormal;
+    const size_t lds_normal = e->tuning.lds_normal ? e->tuning.lds_normal : (size_t)0;
     rt_classes(&plan, counts, infos, lds_normal, e->lds_limit, &cls);
     if (cls.fits) break;
     // tiles too crowded for every LDS class: shorter tiles for the runs that

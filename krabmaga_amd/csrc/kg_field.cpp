@@ -282,13 +282,14 @@ int rt_check_flags(kg_field* f) {
   return check_poison(f);
 }
 
-// LDS budget of a normal-class work-group: four fit one CU's 160 KB.
-// KG_RT_LDS_NORMAL (bytes) overrides it for tuning.
+// LDS budget of a class-0 work-group: 0 sizes it to keep a CU full for the
+// class's work-group size (rt_lds_budget); KG_RT_LDS_NORMAL (bytes) fixes
+// it for tuning.
 size_t rt_lds_normal() {
   static const size_t v = [] {
     const char* e = std::getenv("KG_RT_LDS_NORMAL");
     const long n = e ? std::atol(e) : 0;
-    return n >= 4096 && n <= 160 * 1024 ? (size_t)n : kRtLdsNormal;
+    return n >= 4096 && n <= 160 * 1024 ? (size_t)n : (size_t)0;
   }();
   return v;
 }

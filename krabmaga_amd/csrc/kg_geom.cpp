@@ -256,6 +256,16 @@ RtCaps rt_caps_for(const std::vector<RtRepInfo>& reps, uint32_t own, uint32_t st
   return c;
 }
 
+size_t rt_lds_budget(int threads) {
+  // A CU holds 16 of the step kernel's waves (about 100 VGPRs each): four
+  // 256-thread or two 512-thread work-groups, each beside about 4 KB of
+  // static LDS, in 160 KB.  Measured at 1M boids and on the 512-replica
+  // ensemble: 28 KB for 256 threads (best of 24-40 KB), 76 KB for 512
+  // (two fill the CU; best of 28-96 KB).  Class 0 never takes 1024-thread
+  // work-groups: one per CU leaves the CU idle between its phases.
+  return threads <= 256 ? (size_t)28 * 1024 : (size_t)(160 * 1024 / 2 - 4096);
+}
+
 void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts,
                 const std::vector<RtRepInfo>& reps, size_t lds_normal, size_t lds_limit,
                 RtClassPlan* out) {
@@ -271,17 +281,24 @@ void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts,
     return;
   }
   // class 0: the largest own-count quantile whose capacities fit the normal
-  // LDS budget
+  // LDS budget (lds_normal, or 0: what keeps a CU full for the class's
+  // work-group size, rt_lds_budget)
   std::vector<uint32_t> so(own);
   std::sort(so.begin(), so.end());
-  RtCaps cn{};
-  for (double qtl : {1.0, 0.999, 0.995, 0.99, 0.98, 0.95, 0.9, 0.8, 0.5}) {
+  auto caps_at = [&](double qtl) {
     const uint32_t own_n = so[std::min<size_t>(nt - 1, (size_t)std::floor(qtl * (nt - 1)))];
     uint32_t st_n = 0;
     for (uint32_t t = 0; t < nt; ++t)
       if (own[t] <= own_n) st_n = std::max(st_n, staged[t]);
-    cn = rt_caps_for(reps, own_n, st_n);
-    if (rt_smem_bytes(cn) <= lds_normal) break;
+    return rt_caps_for(reps, own_n, st_n);
+  };
+  // class 0 keeps the work-group size of the median tile (at most 512)
+  const int t0 = std::min(512, caps_at(0.5).threads);
+  const size_t budget = lds_normal ? lds_normal : std::min(lds_limit, rt_lds_budget(t0));
+  RtCaps cn{};
+  for (double qtl : {1.0, 0.999, 0.995, 0.99, 0.98, 0.95, 0.9, 0.8, 0.5}) {
+    cn = caps_at(qtl);
+    if (cn.threads <= t0 && rt_smem_bytes(cn) <= budget) break;
   }
   uint32_t own_max = 0, st_max = 0;
   for (uint32_t t = 0; t < nt; ++t) {

@@ -63,7 +63,8 @@ void rt_needs(const RtPlan& plan, const std::vector<uint32_t>& counts,
 RtCaps rt_caps_for(const std::vector<RtRepInfo>& reps, uint32_t own, uint32_t staged);
 
 // Splits the tiles into classes (sets plan->meta[].cls): class 0's
-// capacities fit `lds_normal` bytes of LDS per work-group, further classes
+// capacities fit `lds_normal` bytes of LDS per work-group (0: rt_lds_budget
+// of its work-group size), further classes
 // grow geometrically to the fullest tile, and the last one also has room for
 // tiles that outgrow class 0 before the next plan.  fits = every class fits
 // `lds_limit`.
@@ -74,6 +75,9 @@ struct RtClassPlan {
   uint32_t off[kRtMaxClasses + 1] = {};  // class k >= 1: dense[off[k] .. off[k + 1])
   bool fits = false;
 };
+// Dynamic LDS of a class-0 work-group of `threads` (256 or 512): what keeps
+// a CU's waves busy, as measured.
+size_t rt_lds_budget(int threads);
 void rt_classes(RtPlan* plan, const std::vector<uint32_t>& counts,
                 const std::vector<RtRepInfo>& reps, size_t lds_normal, size_t lds_limit,
                 RtClassPlan* out);

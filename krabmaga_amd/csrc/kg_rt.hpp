@@ -62,9 +62,6 @@ constexpr uint32_t kRtFlagDefer = 16u;   // a normal tile outgrew its class and 
 // launch for deferred tiles through a per-parity list ([0] count, then ids);
 // that launch has kRtOvfMax work-groups.
 constexpr uint32_t kRtOvfMax = 256;
-// Dynamic LDS budget of a class-0 work-group: four per CU (160 KB) with the
-// kernel's static LDS beside them (at 40 KB only three fit).
-constexpr size_t kRtLdsNormal = 28 * 1024;
 // Deferrals per step past which the host re-plans a field: any (a step
 // with a deferred tile pays one more work-group's latency after its class
 // launches; a field's re-plan costs about one step's worth every few dozen).

@@ -318,7 +318,7 @@ int emu_rt_enter(emu_field* f, int caps_runs, int caps_own, int caps_extras, int
     E.plan.inbox_total = ib;
   }
   RtClassPlan cls;
-  rt_classes(&E.plan, counts, {RtRepInfo{f->g, f->rep.disc_dist, 0u}}, (mode & 2) ? 2048 : 32768,
+  rt_classes(&E.plan, counts, {RtRepInfo{f->g, f->rep.disc_dist, 0u}}, (mode & 2) ? 2048 : 0,
              157696, &cls);
   E.ncls = cls.ncls;
   for (int k = 0; k < kRtMaxClasses; ++k) E.caps[k] = cls.caps[k];
