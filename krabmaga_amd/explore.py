@@ -274,7 +274,8 @@ def explore(nstep, rep_conf, inputs, outputs=("polarisation",), mode=ExploreMode
         i, j = runs[k]
         vals = {name: fn(st, specs[k]) for name, fn in outs.items()}
         dt32 = np.float32(dt)
-        sps = np.float32(nstep) / dt32 if dt32 > 0 else np.float32(np.inf)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            sps = np.float32(nstep) / dt32  # f32 division as in Rust: x / 0 = inf, 0 / 0 = NaN
         rows[k] = FrameRow(i, j if parallel else j + 1, dict(confs[i]), vals, float(dt32),
                            float(sps), st if keep_state else None)
 
@@ -288,7 +289,8 @@ def explore(nstep, rep_conf, inputs, outputs=("polarisation",), mode=ExploreMode
                 f.schedule_run(s.params, 0, nstep)
                 f.sync()
                 dt = time.perf_counter() - t0
-                st, _ = f.state_by_id(0)
+                # before its first lazy_update the birds are in the write buffer
+                st, _ = f.state_by_id(0 if nstep > 0 else 1)
             finally:
                 f.close()
             finish(k, st, dt)

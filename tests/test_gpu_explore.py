@@ -65,3 +65,13 @@ def test_small_budget_splits_into_batches_with_the_same_bits(tmp_path):
     (pol,) = load_csv(str(tmp_path / "r.csv"), ("x", int), ("y", int), ("agents", int),
                       ("polarisation", float))[3:]
     assert pol == [r.outputs["polarisation"] for r in many]
+
+
+def test_zero_steps_reports_the_initial_flock(oracle):
+    for parallel in (True, False):
+        (r,) = explore(0, 1, {"agents": [500]}, seed=5, parallel=parallel, keep_state=True)
+        s = flockers_spec(r.inputs, 5 << 32)
+        init = np.stack(oracle.init_uniform(s.agents, s.width, s.height, s.init_seed,
+                                            s.heading_mode))
+        assert np.array_equal(bits(r.state), bits(init)), f"parallel={parallel}"
+        assert r.step_per_sec == 0.0
