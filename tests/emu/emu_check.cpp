@@ -58,7 +58,8 @@ struct Scenario {
   int lds_cap;
   int force_global;
   uint64_t shuffle;
-  int layout;  // 0 uniform, 1 clustered, 2 hugging the far edges (padding cells)
+  int layout;  // 0 uniform, 1 clustered, 2 hugging the far edges (padding cells),
+               // 3 stacked birds and cell-boundary positions
   float radius;
 };
 
@@ -83,6 +84,24 @@ static void make_positions(const Scenario& s, std::vector<float>& x, std::vector
       const int k = (int)(i % 6);
       x[i] = or_toroidal_transform((float)(cx[k] + nd(rng)), s.side);
       y[i] = or_toroidal_transform((float)(cy[k] + nd(rng)), s.side);
+    }
+  } else if (s.layout == 3) {
+    for (uint32_t i = 0; i < s.n; ++i) {
+      const uint32_t r = i % 4;
+      if (r == 1 || r == 2) {
+        // exact duplicates of the group's first bird (distance 0, equal cells)
+        x[i] = x[i - r];
+        y[i] = y[i - r];
+      } else if (r == 3) {
+        // on a cell corner, or one ulp either side of it
+        const float d = s.d;
+        float bx = (float)(int)(x[i] / d) * d, by = (float)(int)(y[i] / d) * d;
+        const int m = (int)((i / 4) % 3);
+        if (m == 1) { bx = std::nextafter(bx, 0.0f); by = std::nextafter(by, s.side); }
+        if (m == 2) { bx = std::nextafter(bx, s.side); by = std::nextafter(by, 0.0f); }
+        x[i] = bx < s.side ? bx : 0.0f;
+        y[i] = by < s.side ? by : 0.0f;
+      }
     }
   } else if (s.layout == 2) {
     for (uint32_t i = 0; i < s.n; i += 3) {
@@ -231,6 +250,9 @@ int main() {
       {"padding-reachable", 4000, 1024.0f, 1.1596828699111938f, 1, 10, 10, 0, 0, 0, 0, 2, 2.0f},
       {"padding-reachable-global", 4000, 1024.0f, 1.1596828699111938f, 1, 4, 10, 0, 0, 1, 0, 2,
        2.0f},
+      {"stacked-boundaries", 3000, 200.0f, D, 1, 6, 15, 0, 0, 0, 0, 3, 10.0f},
+      {"stacked-boundaries-global", 3000, 200.0f, D, 1, 4, 15, 0, 0, 1, 0, 3, 10.0f},
+      {"empty", 0, 200.0f, D, 1, 3, 16, 0, 0, 0, 0, 0, 10.0f},
   };
   int fails = 0;
   for (const Scenario& s : sc) fails += run(s);
@@ -257,6 +279,8 @@ int main() {
       {"rt-deferred", 3000, 300.0f, D, 12, 13, 0, 0, 0, 64, 24, 8, 4, 0, 1, 10.0f, 0.7f},
       {"rt-deferred-shuffled", 1500, 200.0f, D, 10, 14, 0, 0, 0, 96, 40, 16, 4, 21, 0, 10.0f,
        0.7f},
+      {"rt-stacked-boundaries", 3000, 200.0f, D, 20, 15, 0, 0, 0, 0, 0, 0, 0, 0, 3, 10.0f, 0.7f},
+      {"rt-empty", 0, 200.0f, D, 3, 16, 0, 0, 0, 0, 0, 0, 0, 0, 0, 10.0f, 0.7f},
   };
   for (const RtScenario& s : rs) fails += run_rt(s);
   std::printf("%d scenario(s) failed\n", fails);

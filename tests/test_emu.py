@@ -3,8 +3,9 @@
 krabmaga_amd/csrc/kg_phases.hpp and kg_rt.hpp hold the bodies of the
 binning, flock and tile-resident kernels as __host__ __device__ functions.
 tests/emu drives them block by block on the host; emu_check (built with
-AddressSanitizer and UBSan) compares 28 scenarios bit for bit with the
+AddressSanitizer and UBSan) compares 35 scenarios bit for bit with the
 oracle: both toroidal modes, padding cells, clustered and dense layouts,
+stacked birds and positions on cell corners,
 tiny LDS capacities (global paths), overflowing tile capacities (inbox and
 spill), jumps longer than a cell, shuffled block and atomic orders.
 """

@@ -85,3 +85,63 @@ def test_crowded_cells_match_oracle(oracle, binned):
     assert np.array_equal(ids, np.arange(n, dtype=np.uint32))
     bad = np.flatnonzero(np.any(st.view(np.uint32) != ref.view(np.uint32), axis=0))
     assert bad.size == 0, f"{bad.size} birds differ, first {bad[:5]}"
+
+
+def stacked_boundary_layout(O, n, side, d, seed):
+    """Uniform birds where, in every group of four, the second and third sit
+    exactly on the first (distance 0, the same cell) and the fourth sits on
+    a cell corner or one ulp to either side of it."""
+    x, y, hx, hy = O.init_uniform(n, side, side, seed, 1)
+    d32, s32 = np.float32(d), np.float32(side)
+    for i in range(n):
+        r = i % 4
+        if r in (1, 2):
+            x[i], y[i] = x[i - r], y[i - r]
+        elif r == 3:
+            bx = np.float32(np.float32(int(x[i] / d32)) * d32)
+            by = np.float32(np.float32(int(y[i] / d32)) * d32)
+            m = (i // 4) % 3
+            if m == 1:
+                bx, by = np.nextafter(bx, np.float32(0)), np.nextafter(by, s32)
+            elif m == 2:
+                bx, by = np.nextafter(bx, s32), np.nextafter(by, np.float32(0))
+            x[i] = bx if bx < s32 else np.float32(0)
+            y[i] = by if by < s32 else np.float32(0)
+    return x, y, hx, hy
+
+
+@pytest.mark.parametrize("binned", [False, True])
+def test_stacked_birds_and_cell_boundaries_match_oracle(oracle, binned):
+    """Birds at identical positions (squared distance 0 in Bird::step's
+    avoidance term, ties in the bag order broken by id) and on cell
+    corners, where discretize and check_circle meet their boundaries: the
+    neighbour lists and 15 steps equal the oracle's."""
+    O = oracle
+    n, side, steps = 3000, 200.0, 15
+    x, y, hx, hy = stacked_boundary_layout(O, n, side, D, 17)
+    assert np.unique(np.stack([x, y]), axis=1).shape[1] < n * 3 // 4 + 10
+    q = Field2D(side, side, D, True, n)
+    q.set_object_locations(np.arange(n, dtype=np.uint32), x, y, hx, hy)
+    q.lazy_update()
+    of = O.Field2D(side, side, D, True)
+    for i in range(n):
+        of.set_object_location(i, float(x[i]), float(y[i]))
+    of.lazy_update()
+    storage, row_ptr, ids = q.neighbor_sets(10.0)
+    a = q.read_agents(0)
+    for k in range(0, len(storage), 7):
+        want = [b[0] for b in of.get_neighbors_within_distance(float(a["x"][k]), float(a["y"][k]),
+                                                               10.0)]
+        assert list(ids[row_ptr[k]:row_ptr[k + 1]]) == want, (k, storage[k])
+    q.close()
+    f = Field2D(side, side, D, True, n)
+    f.set_object_locations(np.arange(n, dtype=np.uint32), x, y, hx, hy)
+    f.schedule_run(FlockParams(seed=17), 0, steps, binned=binned)
+    st, got_ids = f.state_by_id(0)
+    f.close()
+    fl = O.Flocker(side, side, D, True, x, y, hx, hy, O.params(seed=17))
+    fl.step(steps)
+    assert np.array_equal(got_ids, np.arange(n, dtype=np.uint32))
+    ref = fl.state()
+    bad = np.flatnonzero(np.any(st.view(np.uint32) != ref.view(np.uint32), axis=0))
+    assert bad.size == 0, f"{bad.size} birds differ, first {bad[:5]}"

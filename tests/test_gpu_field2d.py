@@ -117,3 +117,22 @@ def test_out_of_grid_location_is_an_error():
         f.set_object_location(1, 20.0, 1.0)
     with pytest.raises(KgError):
         f.set_object_location(1, -1.0, 1.0)
+
+
+@pytest.mark.parametrize("binned", [False, True])
+def test_empty_flock_steps(binned):
+    # Flocker::new(dim, 0): init places no bird, so every Schedule::step has
+    # nothing to run and the field stays empty
+    from krabmaga_amd import Field2D, FlockParams
+    f = Field2D(200.0, 200.0, 10.0 / 1.5, True, 1)
+    f.init_uniform(0, seed=3)
+    f.schedule_run(FlockParams(seed=3), 0, 6, binned=binned)
+    f.sync()
+    assert f.count(0) == 0 and f.count(1) == 0
+    assert f.get_neighbors_within_distance(100.0, 100.0, 10.0).size == 0
+    assert f.get_neighbors_within_relax_distance(0.0, 0.0, 10.0).size == 0
+    dw = int(np.ceil(np.float32(200.0) / np.float32(10.0 / 1.5))) + 1
+    assert f.num_empty_bags() == dw * dw
+    st, ids = f.state_by_id(0)
+    assert st.shape == (4, 0) and ids.size == 0
+    f.close()
