@@ -326,8 +326,9 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the engine "
+              "runs every field and ensemble on one GPU (no sharding across GPUs)",
+              file=sys.stderr)
         sys.exit(2)
 
     hip = Hip()

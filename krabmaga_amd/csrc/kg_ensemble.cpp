@@ -315,7 +315,7 @@ int rebuild(kg_ensemble* e) {
       rp.app_base = r.app_base;
       rp.app_n = r.desc.agents;
       rp.boff_base = boff_base;
-      // the first tile-resident launch after this build runs step G; a run's
+      // the first tile-resident launch after this layout rebuild runs step G; a run's
       // next schedule step is steps_done
       rp.step_base = r.steps_done - G;
       r.tile0 = tile0;
