@@ -282,7 +282,10 @@ def run_serving(args, hip):
     sm = serve.summary(done, srv.wall)
     log(f"serving: {sm}")
     k = min(len(reqs), args.one_by_one_runs)
-    sample = reqs[:: max(1, len(reqs) // k)][:k]
+    # k requests spread evenly over the arrival order
+    m = min(k, len(reqs))
+    idx = sorted({round(i * (len(reqs) - 1) / max(m - 1, 1)) for i in range(m)})
+    sample = [reqs[i] for i in idx]
     obo_ms = _one_by_one(hip, [(r.spec, r.steps) for r in sample], args.warmup, whole_run=True)
     obo_value = sum(r.spec.agents * r.steps for r in sample) / (obo_ms * 1e-3)
     log(f"serving one by one: {len(sample)} runs, {obo_value:.3e} agent-updates/s")
@@ -294,9 +297,9 @@ def run_serving(args, hip):
         "hbm_budget_bytes": srv.budget, "peak_admitted_bytes": srv.peak_bytes,
         "ticks": srv.ticks, "admissions": srv.admissions, "layout": srv.layout,
         "one_by_one": {"value": obo_value, "runs": len(sample),
-                       "sample": "every %d-th request run alone on its own field: creation, "
-                                 "init, all steps and read-back timed, as a served request"
-                                 % max(1, len(reqs) // k)},
+                       "sample": "%d of the %d requests, evenly spaced in arrival order, each "
+                                 "run alone on its own field: creation, init, all steps and "
+                                 "read-back timed, as a served request" % (len(sample), len(reqs))},
         "vs_one_by_one": sm["agent_updates_per_s"] / obo_value,
     })
     return sm
