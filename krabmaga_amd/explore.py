@@ -263,6 +263,8 @@ def explore(nstep, rep_conf, inputs, outputs=("polarisation",), mode=ExploreMode
     hbm_budget bytes."""
     if nstep < 0 or rep_conf < 0:
         raise ValueError("nstep and rep_conf must be non-negative")
+    if isinstance(outputs, str):
+        outputs = (outputs,)
     outs = ({k: FLOCKERS_OUTPUTS[k] for k in outputs} if not isinstance(outputs, dict)
             else dict(outputs))
     confs = configurations(inputs, mode)

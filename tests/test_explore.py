@@ -125,3 +125,9 @@ def test_flockers_spec_defaults_and_names():
     assert abs(s.discretization - 10.0 / 1.5) < 1e-12
     with pytest.raises(KeyError):
         flockers_spec({"agent": 5}, 0)
+
+
+def test_sweep_without_inputs_is_one_default_configuration():
+    # n_conf starts at 1 and no input changes it (model_exploration.rs:369-381)
+    assert configurations({}, ExploreMode.Exaustive) == [{}]
+    assert configurations({}, ExploreMode.Matched) == [{}]
