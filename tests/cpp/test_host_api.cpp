@@ -1,0 +1,313 @@
+// Tests of the C++ host interface (include/krabgpu.hpp).
+//
+// --cpu: Schedule::step semantics (src/engine/schedule.rs:347-413) and the
+//        simulate! loop (src/lib.rs:1158-1173) with host-only agents; no
+//        device call.
+// (no flag): additionally the reference's own Field2D tests
+//        (tests/engine/field_2d.rs:30-247) through the C++ Field2D, and a
+//        flockers run through Schedule/simulate and through Flocker::run,
+//        each compared bit for bit with the oracle.
+// Prints one line per test and "<k> failed"; exit status 0 when none failed.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "krabgpu.hpp"
+#include "oracle.h"
+
+using krabgpu::Real2D;
+
+static int g_failed = 0;
+static int g_checks_failed = 0;
+
+#define EXPECT(c)                                                         \
+  do {                                                                    \
+    if (!(c)) {                                                           \
+      std::printf("    check failed at line %d: %s\n", __LINE__, #c);     \
+      ++g_checks_failed;                                                  \
+    }                                                                     \
+  } while (0)
+
+static void run_test(const char* name, const std::function<void()>& fn) {
+  const int before = g_checks_failed;
+  try {
+    fn();
+  } catch (const std::exception& e) {
+    std::printf("    exception: %s\n", e.what());
+    ++g_checks_failed;
+  }
+  const bool ok = g_checks_failed == before;
+  if (!ok) ++g_failed;
+  std::printf("%-44s %s\n", name, ok ? "ok" : "FAILED");
+  std::fflush(stdout);
+}
+
+static std::vector<uint32_t> sorted(std::vector<uint32_t> v) {
+  std::sort(v.begin(), v.end());
+  return v;
+}
+
+// ---- Schedule semantics, host only ---------------------------------------
+
+struct Log {
+  std::vector<std::string> ev;
+};
+
+struct LogState : krabgpu::State {
+  Log* log;
+  uint64_t stop_after;
+  explicit LogState(Log* l, uint64_t stop = ~0ull) : log(l), stop_after(stop) {}
+  std::vector<std::pair<std::shared_ptr<krabgpu::Agent>, std::pair<float, int>>> agents;
+  bool repeating = true;
+  void init(krabgpu::Schedule& s) override {
+    log->ev.push_back("init");
+    for (auto& a : agents) {
+      if (repeating)
+        s.schedule_repeating(a.first, a.second.first, a.second.second);
+      else
+        s.schedule_once(a.first, a.second.first, a.second.second);
+    }
+  }
+  void update(uint64_t step) override { log->ev.push_back("update" + std::to_string(step)); }
+  void before_step(krabgpu::Schedule&) override { log->ev.push_back("before"); }
+  void after_step(krabgpu::Schedule&) override { log->ev.push_back("after"); }
+  bool end_condition(krabgpu::Schedule& s) override { return s.step_count >= stop_after; }
+};
+
+struct LogAgent : krabgpu::Agent {
+  Log* log;
+  std::string name;
+  int stop_at = -1;  // is_stopped after this many steps
+  int steps = 0;
+  LogAgent(Log* l, std::string n, int stop = -1) : log(l), name(std::move(n)), stop_at(stop) {}
+  void step(krabgpu::State&) override {
+    ++steps;
+    log->ev.push_back(name);
+  }
+  bool is_stopped(krabgpu::State&) override { return stop_at >= 0 && steps >= stop_at; }
+};
+
+static std::string join(const std::vector<std::string>& v) {
+  std::string s;
+  for (const auto& x : v) s += (s.empty() ? "" : " ") + x;
+  return s;
+}
+
+static void cpu_tests() {
+  run_test("schedule: order, update timing", [] {
+    Log log;
+    LogState st(&log);
+    // ascending (time, ordering); equal priorities in scheduling order
+    st.agents = {{std::make_shared<LogAgent>(&log, "b"), {0.0f, 1}},
+                 {std::make_shared<LogAgent>(&log, "a"), {0.0f, 0}},
+                 {std::make_shared<LogAgent>(&log, "c"), {0.0f, 1}},
+                 {std::make_shared<LogAgent>(&log, "late"), {1.0f, -5}}};
+    krabgpu::simulate(st, 2);
+    EXPECT(join(log.ev) ==
+           "init update0 before a b c after update1 before late a b c after update2");
+  });
+  run_test("schedule: once, stopped agents, empty queue", [] {
+    Log log;
+    LogState st(&log);
+    st.repeating = false;
+    st.agents = {{std::make_shared<LogAgent>(&log, "x"), {0.0f, 0}}};
+    krabgpu::simulate(st, 3);
+    // an event scheduled once runs once; later steps find the queue empty
+    EXPECT(join(log.ev) ==
+           "init update0 before x after update1 before after update2 before after update3");
+    Log log2;
+    LogState st2(&log2);
+    st2.agents = {{std::make_shared<LogAgent>(&log2, "s", 2), {0.0f, 0}}};
+    krabgpu::simulate(st2, 3);
+    EXPECT(join(log2.ev) ==
+           "init update0 before s after update1 before s after update2 before after update3");
+  });
+  run_test("simulate: end_condition and repetitions", [] {
+    Log log;
+    LogState st(&log, 2);
+    st.agents = {{std::make_shared<LogAgent>(&log, "a"), {0.0f, 0}}};
+    krabgpu::simulate(st, 10, 2);
+    const std::string one = "init update0 before a after update1 before a after update2";
+    EXPECT(join(log.ev) == one + " " + one);
+  });
+}
+
+// ---- device tests ---------------------------------------------------------
+
+static krabgpu::Field2D make_small(uint32_t cap = 16) {
+  // tests/engine/field_2d.rs fixture: 10 x 10, discretization 0.5, toroidal
+  return krabgpu::Field2D(10.0f, 10.0f, 0.5f, true, cap);
+}
+
+static kg_flock_params params(float radius, uint64_t seed, float jump = 0.7f) {
+  kg_flock_params p;
+  p.cohesion = p.avoidance = p.consistency = p.randomness = p.momentum = 1.0f;
+  p.jump = jump;
+  p.radius = radius;
+  p.reserved = 0;
+  p.seed = seed;
+  return p;
+}
+
+static void field_tests() {
+  run_test("field_2d_single_step (field_2d.rs:30-50)", [] {
+    auto f = make_small(10);
+    f.init_uniform(10, 7, 0);
+    f.schedule_run(params(10.0f, 7), 0, 1);
+    EXPECT(f.nagents() == 10);
+    EXPECT(f.get_neighbors_within_distance({5.0f, 5.0f}, 10.0f).size() == 10);
+    EXPECT(f.get_neighbors_within_relax_distance({5.0f, 5.0f}, 10.0f).size() == 10);
+  });
+  run_test("field_2d_neighbors (field_2d.rs:57-117)", [] {
+    for (int fly = 5; fly < 10; ++fly) {
+      auto f = make_small();
+      f.set_object_location(1, {0.0f, 0.0f});
+      f.set_object_location(2, {0.0f, 0.0f});
+      f.lazy_update();
+      EXPECT(f.nagents() == 2);
+      EXPECT(f.get_neighbors_within_distance({5.0f, 5.0f}, 1.0f).empty());
+      EXPECT(f.get_neighbors_within_relax_distance({5.0f, 5.0f}, 1.0f).empty());
+      const Real2D b1{(float)fly, (float)fly};
+      f.set_object_location(1, b1);
+      f.set_object_location(2, {0.0f, 0.0f});
+      f.lazy_update();
+      EXPECT(f.get_neighbors_within_distance(b1, 1.0f) == std::vector<uint32_t>{1});
+      EXPECT(f.get_neighbors_within_distance({0.0f, 0.0f}, 1.0f) == std::vector<uint32_t>{2});
+      EXPECT(sorted(f.get_neighbors_within_distance({5.0f, 5.0f}, 10.0f)) ==
+             (std::vector<uint32_t>{1, 2}));
+      EXPECT(sorted(f.get_neighbors_within_relax_distance({5.0f, 5.0f}, 10.0f)) ==
+             (std::vector<uint32_t>{1, 2}));
+    }
+  });
+  run_test("field_2d_gets (field_2d.rs:124-178)", [] {
+    auto f = make_small();
+    f.set_object_location(1, {0.0f, 0.0f});
+    f.set_object_location(2, {5.0f, 5.0f});
+    f.set_object_location(3, {5.0f, 5.0f});
+    f.lazy_update();
+    EXPECT(f.nagents() == 3);
+    EXPECT(sorted(f.get_objects({5.0f, 5.0f})) == (std::vector<uint32_t>{2, 3}));
+    EXPECT(f.get_objects({10.0f, 0.0f}).empty());
+    EXPECT(f.num_objects_at_location({5.0f, 5.0f}) == 2);
+    EXPECT(f.num_objects_at_location({0.0f, 0.0f}) == 1);
+    f.set_object_location(4, {0.0f, 0.0f});
+    EXPECT(f.get_objects_unbuffered({0.0f, 0.0f}) == std::vector<uint32_t>{4});
+    EXPECT(f.get_objects({0.0f, 0.0f}) == std::vector<uint32_t>{1});
+    f.remove_object_location(4, {0.0f, 0.0f});
+    EXPECT(f.get_objects_unbuffered({0.0f, 0.0f}).empty());
+  });
+  run_test("field_2d_bags (field_2d.rs:185-210)", [] {
+    auto f = make_small();
+    EXPECT(f.num_empty_bags() == 21 * 21);
+    f.set_object_location(1, {0.0f, 0.0f});
+    f.set_object_location(2, {0.0f, 0.0f});
+    f.set_object_location(3, {4.0f, 4.0f});
+    f.lazy_update();
+    EXPECT(f.num_empty_bags() == 21 * 21 - 2);
+  });
+  run_test("field_2d_iter (field_2d.rs:217-247)", [] {
+    auto f = make_small();
+    f.set_object_location(1, {0.0f, 0.0f});
+    f.set_object_location(2, {0.01f, 0.01f});
+    f.set_object_location(3, {5.0f, 5.0f});
+    const krabgpu::Agents wb = f.read_agents(1);
+    for (size_t k = 0; k < wb.id.size(); ++k) {
+      const auto o = f.get_objects_unbuffered({wb.x[k], wb.y[k]});
+      EXPECT(std::find(o.begin(), o.end(), wb.id[k]) != o.end());
+    }
+    f.lazy_update();
+    EXPECT(f.get_objects({0.0f, 0.0f}).size() == 2);
+    const krabgpu::Agents rb = f.read_agents(0);
+    EXPECT(sorted(rb.id) == (std::vector<uint32_t>{1, 2, 3}));
+  });
+  run_test("out-of-grid location throws", [] {
+    auto f = make_small();
+    int code = 0;
+    try {
+      f.set_object_location(1, {20.0f, 1.0f});
+    } catch (const krabgpu::Error& e) {
+      code = e.code();
+    }
+    EXPECT(code == KG_E_RANGE);
+  });
+}
+
+// (4, n) state by id from a device buffer
+static std::vector<float> by_id(const krabgpu::Agents& a, uint32_t n) {
+  std::vector<float> s(4 * (size_t)n, NAN);
+  for (size_t k = 0; k < a.id.size(); ++k) {
+    const uint32_t i = a.id[k];
+    if (i >= n) continue;
+    s[i] = a.x[k];
+    s[n + i] = a.y[k];
+    s[2 * (size_t)n + i] = a.hx[k];
+    s[3 * (size_t)n + i] = a.hy[k];
+  }
+  return s;
+}
+
+static std::vector<float> oracle_run(uint32_t n, float side, float d, uint64_t seed,
+                                     uint32_t steps) {
+  std::vector<float> x(n), y(n), hx(n), hy(n);
+  or_init_uniform(n, side, side, seed, 1, x.data(), y.data(), hx.data(), hy.data());
+  or_flock_params op{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.7f, 10.0f, seed};
+  or_flocker* fl = or_flocker_new(n, side, side, d, 1, x.data(), y.data(), hx.data(), hy.data(),
+                                  &op);
+  for (uint32_t k = 0; k < steps; ++k) or_flocker_step(fl);
+  std::vector<float> s(4 * (size_t)n);
+  or_flocker_get(fl, s.data(), s.data() + n, s.data() + 2 * (size_t)n, s.data() + 3 * (size_t)n);
+  or_flocker_free(fl);
+  return s;
+}
+
+static bool same_bits(const std::vector<float>& a, const std::vector<float>& b) {
+  return a.size() == b.size() && std::memcmp(a.data(), b.data(), a.size() * 4) == 0;
+}
+
+static void flockers_tests() {
+  const float d = 10.0f / 1.5f;
+  run_test("flockers: simulate through Schedule", [&] {
+    const uint32_t n = 1000, steps = 12;
+    krabgpu::Flocker fl({200.0f, 200.0f}, n, d, params(10.0f, 3));
+    krabgpu::simulate(fl, steps);
+    EXPECT(fl.step == steps);
+    EXPECT(same_bits(by_id(fl.field1.read_agents(0), n), oracle_run(n, 200.0f, d, 3, steps)));
+  });
+  run_test("flockers: repetitions restart from init", [&] {
+    const uint32_t n = 800, steps = 5;
+    krabgpu::Flocker fl({180.0f, 180.0f}, n, d, params(10.0f, 4));
+    krabgpu::simulate(fl, steps, 2);
+    EXPECT(fl.field1.count(0) == n && fl.field1.count(1) == 0);
+    EXPECT(same_bits(by_id(fl.field1.read_agents(0), n), oracle_run(n, 180.0f, d, 4, steps)));
+  });
+  run_test("flockers: Flocker::run equals simulate", [&] {
+    const uint32_t n = 5000, steps = 40;
+    krabgpu::Flocker a({450.0f, 450.0f}, n, d, params(10.0f, 5));
+    a.run(steps);
+    EXPECT(a.step == steps);
+    EXPECT(same_bits(by_id(a.field1.read_agents(0), n), oracle_run(n, 450.0f, d, 5, steps)));
+  });
+  run_test("flockers: no birds", [&] {
+    krabgpu::Flocker fl({200.0f, 200.0f}, 0, d, params(10.0f, 6));
+    krabgpu::simulate(fl, 3);
+    EXPECT(fl.field1.count(0) == 0 && fl.step == 3);
+    fl.run(3);
+    EXPECT(fl.field1.count(0) == 0);
+  });
+}
+
+int main(int argc, char** argv) {
+  const bool cpu_only = argc > 1 && std::strcmp(argv[1], "--cpu") == 0;
+  cpu_tests();
+  if (!cpu_only) {
+    field_tests();
+    flockers_tests();
+  }
+  std::printf("%d failed\n", g_failed);
+  return g_failed ? 1 : 0;
+}
