@@ -364,6 +364,75 @@ class Flocker : public State {
   };
 };
 
+/* Many independent flockers runs advanced together on the device
+ * (explore_parallel!'s runs, src/explore/model_exploration.rs:354-432): one
+ * launch per schedule step moves every admitted run, and each run ends with
+ * the birds it would have alone.  Runs join with admit and leave with
+ * retire between run calls. */
+class Ensemble {
+ public:
+  explicit Ensemble(const kg_tuning& tuning = kg_tuning{0, 0, 0, 0}) {
+    check(kg_ensemble_create(&tuning, &e_));
+  }
+  ~Ensemble() {
+    if (e_) kg_ensemble_destroy(e_);
+  }
+  Ensemble(const Ensemble&) = delete;
+  Ensemble& operator=(const Ensemble&) = delete;
+
+  /* A run: Flocker::new(dim, agents) with its cell size, parameters and
+   * initial-placement key. */
+  static kg_run_desc run_desc(Real2D dim, uint32_t agents, float discretization,
+                              const kg_flock_params& params, int heading_mode = 1) {
+    return kg_run_desc{dim.x, dim.y, discretization, 1, agents, heading_mode, params.seed, params};
+  }
+  static uint64_t footprint(const kg_run_desc& run, const kg_tuning& tuning = kg_tuning{0, 0, 0, 0}) {
+    uint64_t b = 0;
+    check(kg_run_footprint(&run, &tuning, &b));
+    return b;
+  }
+  /* Returns the runs' ids; their schedule step 0 comes with the next run(). */
+  std::vector<uint32_t> admit(const std::vector<kg_run_desc>& runs) {
+    std::vector<uint32_t> ids(runs.size());
+    check(kg_ensemble_admit(e_, runs.data(), (uint32_t)runs.size(), ids.data()));
+    for (size_t k = 0; k < runs.size(); ++k) agents_.push_back({ids[k], runs[k].agents});
+    return ids;
+  }
+  void retire(uint32_t id) {
+    check(kg_ensemble_retire(e_, id));
+    agents_.erase(std::remove_if(agents_.begin(), agents_.end(),
+                                 [id](const std::pair<uint32_t, uint32_t>& a) { return a.first == id; }),
+                  agents_.end());
+  }
+  /* Every run takes `nsteps` Schedule::step calls (asynchronous). */
+  void run(uint32_t nsteps, int flags = KG_RUN_GRAPH) { check(kg_ensemble_run(e_, nsteps, flags)); }
+  uint64_t steps(uint32_t id) {
+    uint64_t s = 0;
+    check(kg_ensemble_steps(e_, id, &s));
+    return s;
+  }
+  Agents read(uint32_t id) {
+    uint32_t m = 0;
+    for (const auto& a : agents_)
+      if (a.first == id) m = a.second;
+    Agents a;
+    a.id.resize(m);
+    a.x.resize(m);
+    a.y.resize(m);
+    a.hx.resize(m);
+    a.hy.resize(m);
+    size_t n = 0;
+    check(kg_ensemble_read(e_, id, a.id.data(), a.x.data(), a.y.data(), a.hx.data(), a.hy.data(),
+                           m, &n));
+    return a;
+  }
+  void sync() { check(kg_ensemble_sync(e_)); }
+
+ private:
+  kg_ensemble* e_ = nullptr;
+  std::vector<std::pair<uint32_t, uint32_t>> agents_;  // (run id, birds)
+};
+
 }  // namespace krabgpu
 
 #endif

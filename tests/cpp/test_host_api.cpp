@@ -4,9 +4,10 @@
 //        simulate! loop (src/lib.rs:1158-1173) with host-only agents; no
 //        device call.
 // (no flag): additionally the reference's own Field2D tests
-//        (tests/engine/field_2d.rs:30-247) through the C++ Field2D, and a
-//        flockers run through Schedule/simulate and through Flocker::run,
-//        each compared bit for bit with the oracle.
+//        (tests/engine/field_2d.rs:30-247) through the C++ Field2D;
+//        flockers runs through Schedule/simulate and through Flocker::run,
+//        compared bit for bit with the oracle; and Ensemble runs (one
+//        retired on the way) against the same runs alone.
 // Prints one line per test and "<k> failed"; exit status 0 when none failed.
 #include <algorithm>
 #include <cmath>
@@ -291,6 +292,33 @@ static void flockers_tests() {
     a.run(steps);
     EXPECT(a.step == steps);
     EXPECT(same_bits(by_id(a.field1.read_agents(0), n), oracle_run(n, 450.0f, d, 5, steps)));
+  });
+  run_test("ensemble: runs equal the same runs alone", [&] {
+    struct R {
+      uint32_t n;
+      float side;
+      uint64_t seed;
+    };
+    const R rs[] = {{700, 170.0f, 11}, {3000, 350.0f, 12}, {1500, 240.0f, 13}};
+    const uint32_t steps = 30;
+    krabgpu::Ensemble ens;
+    std::vector<kg_run_desc> descs;
+    for (const R& r : rs)
+      descs.push_back(krabgpu::Ensemble::run_desc({r.side, r.side}, r.n, d, params(10.0f, r.seed)));
+    EXPECT(krabgpu::Ensemble::footprint(descs[0]) > 0);
+    const std::vector<uint32_t> ids = ens.admit(descs);
+    ens.run(steps);
+    for (size_t k = 0; k < 3; ++k) {
+      EXPECT(ens.steps(ids[k]) == steps);
+      krabgpu::Flocker alone({rs[k].side, rs[k].side}, rs[k].n, d, params(10.0f, rs[k].seed));
+      alone.run(steps);
+      EXPECT(same_bits(by_id(ens.read(ids[k]), rs[k].n), by_id(alone.field1.read_agents(0), rs[k].n)));
+    }
+    ens.retire(ids[1]);
+    ens.run(5);
+    krabgpu::Flocker alone({rs[2].side, rs[2].side}, rs[2].n, d, params(10.0f, rs[2].seed));
+    alone.run(steps + 5);
+    EXPECT(same_bits(by_id(ens.read(ids[2]), rs[2].n), by_id(alone.field1.read_agents(0), rs[2].n)));
   });
   run_test("flockers: no birds", [&] {
     krabgpu::Flocker fl({200.0f, 200.0f}, 0, d, params(10.0f, 6));
