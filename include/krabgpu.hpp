@@ -16,6 +16,7 @@
 #define KRABGPU_HPP
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>

@@ -36,3 +36,20 @@ def test_library_loads_and_reports_errors_without_a_gpu_call():
     assert L.kg_last_error() == b""
     assert L.kg_field_create(None, None) == _abi.KG_E_INVALID
     assert b"NULL" in L.kg_last_error()
+
+
+def test_rust_binding_in_integration_matches_the_header():
+    """INTEGRATION.md's extern block declares every entry point, and its
+    #[repr(C)] structs have the header's fields in the header's order."""
+    md = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    h = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "krabgpu.h")).read(),
+               flags=re.S)
+    assert sorted(set(re.findall(r"pub fn (kg_[a-z0-9_]+)\(", md))) == header_functions()
+    structs = re.findall(r"typedef struct (kg_\w+) \{(.*?)\} \1;", h, flags=re.S)
+    assert len(structs) >= 4
+    for name, body in structs:
+        m = re.search(r"pub struct " + name + r" \{(.*?)\}", md, flags=re.S)
+        assert m, f"{name} missing from the Rust binding"
+        assert re.findall(r"pub (\w+):", m.group(1)) == re.findall(r"\b(\w+);", body), name
+    for name, value in re.findall(r"#define (KG_E_\w+|KG_OK|KG_RUN_\w+) \(?(-?\d+)\)?", h):
+        assert re.search(rf"pub const {name}: c_int = {value};", md), name
