@@ -1,13 +1,15 @@
 // Tests of the C++ host interface (include/krabgpu.hpp).
 //
-// --cpu: Schedule::step semantics (src/engine/schedule.rs:347-413) and the
-//        simulate! loop (src/lib.rs:1158-1173) with host-only agents; no
-//        device call.
+// --cpu: Schedule::step semantics (src/engine/schedule.rs:347-413), the
+//        simulate! loop (src/lib.rs:1158-1173) with host-only agents, and
+//        the exploration front end's sweeps, gen_param, number text and
+//        CSV (include/krabgpu_explore.hpp); no device call.
 // (no flag): additionally the reference's own Field2D tests
 //        (tests/engine/field_2d.rs:30-247) through the C++ Field2D;
 //        flockers runs through Schedule/simulate and through Flocker::run,
-//        compared bit for bit with the oracle; and Ensemble runs (one
-//        retired on the way) against the same runs alone.
+//        compared bit for bit with the oracle; Ensemble runs (one retired
+//        on the way) against the same runs alone; and parallel and
+//        sequential sweeps against each other and against runs alone.
 // Prints one line per test and "<k> failed"; exit status 0 when none failed.
 #include <algorithm>
 #include <cmath>
@@ -19,6 +21,7 @@
 #include <vector>
 
 #include "krabgpu.hpp"
+#include "krabgpu_explore.hpp"
 #include "oracle.h"
 
 using krabgpu::Real2D;
@@ -127,6 +130,84 @@ static void cpu_tests() {
     krabgpu::simulate(st2, 3);
     EXPECT(join(log2.ev) ==
            "init update0 before s after update1 before s after update2 before after update3");
+  });
+  run_test("explore: exhaustive order, matched rules", [] {
+    using krabgpu::ExploreMode;
+    const krabgpu::Inputs in{{"agents", {100, 200}}, {"jump", {0.25, 0.5, 0.75}}};
+    const auto c = krabgpu::configurations(in, ExploreMode::Exaustive);
+    EXPECT(c.size() == 6);
+    size_t k = 0;
+    for (double a : {100.0, 200.0})
+      for (double j : {0.25, 0.5, 0.75}) {
+        EXPECT(c[k][0].second == a && c[k][1].second == j);
+        ++k;
+      }
+    const auto m = krabgpu::configurations({{"a", {1, 2, 3, 4}}, {"b", {10, 20}}},
+                                           ExploreMode::Matched);
+    EXPECT(m.size() == 2 && m[1][0].second == 2 && m[1][1].second == 20);
+    bool threw = false;
+    try {
+      krabgpu::configurations({{"a", {1}}, {"b", {10, 20}}}, ExploreMode::Matched);
+    } catch (const krabgpu::Error&) {
+      threw = true;
+    }
+    EXPECT(threw);
+    threw = false;
+    try {
+      krabgpu::configurations({{"a", {}}}, ExploreMode::Exaustive);
+    } catch (const krabgpu::Error&) {
+      threw = true;
+    }
+    EXPECT(threw);
+    EXPECT(krabgpu::configurations({}, ExploreMode::Exaustive).size() == 1);
+  });
+  run_test("explore: gen_param ranges", [] {
+    std::mt19937_64 rng(5);
+    const auto v = krabgpu::gen_param<int>(10, 0, 1000, rng);  // bounds swapped
+    EXPECT(*std::min_element(v.begin(), v.end()) == 0 && *std::max_element(v.begin(), v.end()) == 9);
+    const auto e = krabgpu::gen_param<int>(7, 7, 50, rng);     // [7, 8)
+    EXPECT(std::all_of(e.begin(), e.end(), [](int x) { return x == 7; }));
+    const auto f = krabgpu::gen_param<double>(2.0, 2.0, 100, rng);
+    EXPECT(std::all_of(f.begin(), f.end(), [](double x) { return x >= 2.0 && x < 3.0; }));
+    EXPECT(krabgpu::gen_param<float>(0.0f, 1.0f, 0, rng).empty());
+  });
+  run_test("explore: Rust number text", [] {
+    const std::pair<double, const char*> dbg[] = {
+        {1.0, "1.0"}, {0.1, "0.1"}, {0.0, "0.0"}, {-2.5, "-2.5"}, {1e16, "1e16"},
+        {1.5e-7, "1.5e-7"}, {1e-4, "0.0001"}, {123456.789, "123456.789"},
+        {9.99e15, "9990000000000000.0"}, {-2.5e20, "-2.5e20"}};
+    for (const auto& t : dbg) EXPECT(krabgpu::rust_debug(t.first) == t.second);
+    EXPECT(krabgpu::rust_debug(std::nan("")) == "NaN");
+    EXPECT(krabgpu::rust_debug_f32(16777217.0f) == "16777216.0");
+    const std::pair<float, const char*> disp[] = {
+        {1.0f, "1"}, {0.1f, "0.1"}, {3.25f, "3.25"}, {1e20f, "100000000000000000000"},
+        {1234.5678f, "1234.5677"}, {0.0f, "0"}};
+    for (const auto& t : disp) EXPECT(krabgpu::rust_display_f32(t.first) == t.second);
+  });
+  run_test("explore: write_csv text", [] {
+    std::vector<krabgpu::FrameRow> rows(2);
+    for (uint32_t j = 0; j < 2; ++j) {
+      rows[j].conf_num = 0;
+      rows[j].conf_rep = j;
+      rows[j].inputs = {{"agents", 100}, {"jump", 0.25}};
+      rows[j].outputs = {{"polarisation", 0.5 + 0.125 * j}};
+      rows[j].run_duration = 0.5f;
+      rows[j].step_per_sec = 1000.0f * (float)(j + 1);
+    }
+    const std::string name = "/tmp/krabgpu_explore_test";
+    krabgpu::write_csv(name, rows);
+    FILE* f = std::fopen((name + ".csv").c_str(), "r");
+    char line[256];
+    std::vector<std::string> lines;
+    while (f && std::fgets(line, sizeof(line), f)) lines.push_back(line);
+    if (f) std::fclose(f);
+    std::remove((name + ".csv").c_str());
+    EXPECT(lines.size() == 3);
+    if (lines.size() == 3) {
+      EXPECT(lines[0] == "Simulation,Run,agents,jump,polarisation,Run Duration,Step per sec\n");
+      EXPECT(lines[1] == "0,0,100,0.25,0.5,0.5,1000\n");
+      EXPECT(lines[2] == "0,1,100,0.25,0.625,0.5,2000\n");
+    }
   });
   run_test("simulate: end_condition and repetitions", [] {
     Log log;
@@ -319,6 +400,32 @@ static void flockers_tests() {
     krabgpu::Flocker alone({rs[2].side, rs[2].side}, rs[2].n, d, params(10.0f, rs[2].seed));
     alone.run(steps + 5);
     EXPECT(same_bits(by_id(ens.read(ids[2]), rs[2].n), by_id(alone.field1.read_agents(0), rs[2].n)));
+  });
+  run_test("explore: parallel and sequential sweeps", [&] {
+    const krabgpu::Inputs in{{"agents", {600, 1500}}, {"jump", {0.4, 0.9}}, {"cohesion", {0.5, 1.7}}};
+    const std::vector<std::pair<std::string, krabgpu::Output>> outs{{"polarisation", krabgpu::polarisation}};
+    krabgpu::ExploreOptions po;
+    po.seed = 3;
+    po.keep_state = true;
+    po.hbm_budget = 900000;  // several batches
+    krabgpu::ExploreOptions so = po;
+    so.parallel = false;
+    const uint32_t steps = 20;
+    const auto par = krabgpu::explore(steps, 2, in, outs, krabgpu::ExploreMode::Exaustive, po);
+    const auto seq = krabgpu::explore(steps, 2, in, outs, krabgpu::ExploreMode::Exaustive, so);
+    EXPECT(par.size() == 16 && seq.size() == 16);
+    for (size_t k = 0; k < par.size() && k < seq.size(); ++k) {
+      EXPECT(par[k].conf_num == k / 2 && par[k].conf_rep == k % 2);
+      EXPECT(seq[k].conf_rep == k % 2 + 1);
+      const uint32_t n = (uint32_t)par[k].inputs[0].second;
+      const auto sp = by_id(par[k].state, n);
+      EXPECT(same_bits(sp, by_id(seq[k].state, n)));
+      EXPECT(par[k].outputs[0].second == seq[k].outputs[0].second);
+      const kg_run_desc d = krabgpu::flockers_run(par[k].inputs, (3ull << 32) | k);
+      krabgpu::Flocker alone({d.width, d.height}, d.agents, d.discretization, d.params);
+      alone.run(steps);
+      EXPECT(same_bits(sp, by_id(alone.field1.read_agents(0), n)));
+    }
   });
   run_test("flockers: no birds", [&] {
     krabgpu::Flocker fl({200.0f, 200.0f}, 0, d, params(10.0f, 6));
