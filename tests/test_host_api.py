@@ -18,6 +18,10 @@ BIN = os.path.join(CPP, "test_host_api")
 
 def _binary(build):
     if build or not os.path.exists(BIN):
+        lib = os.path.join(os.path.dirname(HERE), "krabmaga_amd", "libkrabgpu.so")
+        assert os.path.exists(lib), "build with __graft_entry__.build()"
+        subprocess.run(["make", "-s", "-C", os.path.join(os.path.dirname(HERE), "oracle")],
+                       check=True)
         subprocess.run(["make", "-s", "-C", CPP], check=True)
     return BIN
 
