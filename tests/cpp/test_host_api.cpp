@@ -3,13 +3,15 @@
 // --cpu: Schedule::step semantics (src/engine/schedule.rs:347-413), the
 //        simulate! loop (src/lib.rs:1158-1173) with host-only agents, and
 //        the exploration front end's sweeps, gen_param, number text and
-//        CSV (include/krabgpu_explore.hpp); no device call.
+//        CSV (include/krabgpu_explore.hpp), and the serving loop's
+//        admission and ticks with a stand-in ensemble; no device call.
 // (no flag): additionally the reference's own Field2D tests
 //        (tests/engine/field_2d.rs:30-247) through the C++ Field2D;
 //        flockers runs through Schedule/simulate and through Flocker::run,
 //        compared bit for bit with the oracle; Ensemble runs (one retired
-//        on the way) against the same runs alone; and parallel and
-//        sequential sweeps against each other and against runs alone.
+//        on the way) against the same runs alone; parallel and sequential
+//        sweeps against each other and against runs alone; and served runs
+//        against the same runs alone.
 // Prints one line per test and "<k> failed"; exit status 0 when none failed.
 #include <algorithm>
 #include <cmath>
@@ -22,6 +24,7 @@
 
 #include "krabgpu.hpp"
 #include "krabgpu_explore.hpp"
+#include "krabgpu_serve.hpp"
 #include "oracle.h"
 
 using krabgpu::Real2D;
@@ -102,7 +105,70 @@ static std::string join(const std::vector<std::string>& v) {
   return s;
 }
 
+// A stand-in ensemble for the serving loop's host logic: each schedule step
+// advances a fake clock by 1 ms.
+struct FakeEnsemble {
+  explicit FakeEnsemble(double* c) : clock(c) {}
+  double* clock;
+  uint32_t next = 0;
+  std::vector<uint32_t> retired;
+  std::vector<uint32_t> admit(const std::vector<kg_run_desc>& runs) {
+    std::vector<uint32_t> ids;
+    for (size_t k = 0; k < runs.size(); ++k) ids.push_back(next++);
+    return ids;
+  }
+  void run(uint32_t n) { *clock += 0.001 * n; }
+  void sync() {}
+  krabgpu::Agents read(uint32_t id) {
+    krabgpu::Agents a;
+    a.id.push_back(id);
+    return a;
+  }
+  void retire(uint32_t id) { retired.push_back(id); }
+};
+
+static kg_run_desc fake_run(uint32_t agents) {
+  kg_run_desc r{};
+  r.agents = agents;
+  return r;
+}
+
 static void cpu_tests() {
+  run_test("serving: admission, ticks, queueing, idle", [] {
+    double t = 0.0;
+    FakeEnsemble ens(&t);
+    krabgpu::Server<FakeEnsemble> srv(250, 64, true);
+    srv.footprint = [](const kg_run_desc& r) { return (uint64_t)r.agents; };
+    srv.clock = [&t] { return t; };
+    srv.sleep = [&t](double s) { t += s; };
+    const std::vector<krabgpu::Request> reqs{{0.0, fake_run(100), 10},
+                                             {0.0, fake_run(100), 30},
+                                             {0.005, fake_run(100), 5},  // waits for memory
+                                             {1.0, fake_run(50), 3}};   // after an idle gap
+    const auto done = srv.serve(reqs, ens);
+    EXPECT(done.size() == 4);
+    if (done.size() == 4) {
+      EXPECT(done[0].index == 0 && done[1].index == 2 && done[2].index == 1 && done[3].index == 3);
+      EXPECT(std::fabs(done[0].done - 0.010) < 1e-9);
+      EXPECT(std::fabs(done[1].admitted - 0.010) < 1e-9 && std::fabs(done[1].done - 0.015) < 1e-9);
+      EXPECT(std::fabs(done[2].done - 0.030) < 1e-9);
+      EXPECT(done[3].admitted >= 1.0 && std::fabs(done[3].done - done[3].admitted - 0.003) < 1e-9);
+      EXPECT(done[1].state.id.size() == 1 && done[1].state.id[0] == 2);
+    }
+    EXPECT(srv.ticks == 4 && srv.admissions == 3 && srv.peak_bytes == 200);
+    EXPECT(ens.retired == (std::vector<uint32_t>{0, 2, 1, 3}));
+    bool threw = false;
+    try {
+      krabgpu::Server<FakeEnsemble> small(99);
+      small.footprint = srv.footprint;
+      FakeEnsemble e2(&t);
+      small.serve({{0.0, fake_run(100), 1}}, e2);
+    } catch (const krabgpu::Error& e) {
+      threw = e.code() == KG_E_ADMIT;
+    }
+    EXPECT(threw);
+  });
+
   run_test("schedule: order, update timing", [] {
     Log log;
     LogState st(&log);
@@ -426,6 +492,38 @@ static void flockers_tests() {
       alone.run(steps);
       EXPECT(same_bits(sp, by_id(alone.field1.read_agents(0), n)));
     }
+  });
+  run_test("serving: each run equals the run alone", [&] {
+    struct R {
+      uint32_t n;
+      float side;
+      uint64_t seed;
+      uint32_t steps;
+      double arrival;
+    };
+    const R rs[] = {{900, 190.0f, 21, 40, 0.0},  {2500, 320.0f, 22, 12, 0.0},
+                    {600, 160.0f, 23, 25, 0.002}, {4000, 400.0f, 24, 30, 0.004},
+                    {1200, 220.0f, 25, 7, 0.006}};
+    std::vector<krabgpu::Request> reqs;
+    uint64_t biggest = 0;
+    for (const R& r : rs) {
+      reqs.push_back({r.arrival,
+                      krabgpu::Ensemble::run_desc({r.side, r.side}, r.n, d, params(10.0f, r.seed)),
+                      r.steps});
+      biggest = std::max(biggest, krabgpu::Ensemble::footprint(reqs.back().run));
+    }
+    krabgpu::Ensemble ens;
+    krabgpu::Server<> srv(biggest * 2, 16, true);  // some requests wait for memory
+    const auto done = srv.serve(reqs, ens);
+    EXPECT(done.size() == 5);
+    for (const auto& c : done) {
+      const R& r = rs[c.index];
+      krabgpu::Flocker alone({r.side, r.side}, r.n, d, params(10.0f, r.seed));
+      alone.run(r.steps);
+      EXPECT(same_bits(by_id(c.state, r.n), by_id(alone.field1.read_agents(0), r.n)));
+      EXPECT(c.latency() >= 0.0);
+    }
+    EXPECT(srv.admissions >= 2);
   });
   run_test("flockers: no birds", [&] {
     krabgpu::Flocker fl({200.0f, 200.0f}, 0, d, params(10.0f, 6));
