@@ -2,8 +2,8 @@
  * krabgpu::Ensemble: gen_param! (src/lib.rs:1794-1816), ExploreMode
  * (src/lib.rs:481-486), build_configurations! (src/lib.rs:1724-1748),
  * explore_sequential! / explore_parallel! (src/explore/model_exploration.rs:
- * 231-432), the FrameRow of build_dataframe! (:451-540) and write_csv
- * (src/lib.rs:1762-1773).
+ * 231-432), the FrameRow of build_dataframe! (:451-540), write_csv
+ * (src/lib.rs:1762-1773) and load_csv! (src/lib.rs:1878-1900).
  *
  * Differences from the reference, each forced by the device engine: run r
  * of a sweep uses the Philox keys (seed << 32) | r instead of an unseeded
@@ -243,6 +243,60 @@ inline Agents sorted_by_id(const Agents& a) {
     s.hy.push_back(a.hy[k]);
   }
   return s;
+}
+
+/* load_csv!(file, x: T, ...) (src/lib.rs:1878-1900): the header row is
+ * skipped and the first `ncols` fields of every record are parsed as numbers
+ * (Rust's "NaN", "inf" and "-inf" included); one vector per column. */
+inline std::vector<std::vector<double>> load_csv(const std::string& path, size_t ncols) {
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) throw Error(KG_E_INVALID, "krabgpu: cannot open " + path);
+  std::vector<std::vector<double>> cols(ncols);
+  std::string text;
+  for (int ch; (ch = std::fgetc(f)) != EOF;) text += (char)ch;
+  std::fclose(f);
+  std::vector<std::string> rec;
+  std::string field;
+  bool quoted = false, header = true;
+  auto end_record = [&] {
+    rec.push_back(field);
+    field.clear();
+    if (!header) {
+      if (rec.size() < ncols) throw Error(KG_E_INVALID, "krabgpu: load_csv: short record");
+      for (size_t k = 0; k < ncols; ++k) {
+        const std::string& t = rec[k];
+        cols[k].push_back(t == "NaN" ? NAN : t == "inf" ? INFINITY : t == "-inf" ? -INFINITY
+                                                                                 : std::stod(t));
+      }
+    }
+    header = false;
+    rec.clear();
+  };
+  for (size_t k = 0; k < text.size(); ++k) {
+    const char ch = text[k];
+    if (quoted) {
+      if (ch == '"' && k + 1 < text.size() && text[k + 1] == '"') {
+        field += '"';
+        ++k;
+      } else if (ch == '"') {
+        quoted = false;
+      } else {
+        field += ch;
+      }
+    } else if (ch == '"') {
+      quoted = true;
+    } else if (ch == ',') {
+      rec.push_back(field);
+      field.clear();
+    } else if (ch == '\n' || ch == '\r') {
+      if (ch == '\r' && k + 1 < text.size() && text[k + 1] == '\n') ++k;
+      end_record();
+    } else {
+      field += ch;
+    }
+  }
+  if (!field.empty() || !rec.empty()) end_record();
+  return cols;
 }
 
 /* Output of a run from its final birds (in ascending id). */

@@ -267,7 +267,10 @@ static void cpu_tests() {
     std::vector<std::string> lines;
     while (f && std::fgets(line, sizeof(line), f)) lines.push_back(line);
     if (f) std::fclose(f);
+    const auto cols = krabgpu::load_csv(name + ".csv", 7);
     std::remove((name + ".csv").c_str());
+    EXPECT(cols.size() == 7 && cols[1] == (std::vector<double>{0, 1}));
+    EXPECT(cols[4] == (std::vector<double>{0.5, 0.625}) && cols[6] == (std::vector<double>{1000, 2000}));
     EXPECT(lines.size() == 3);
     if (lines.size() == 3) {
       EXPECT(lines[0] == "Simulation,Run,agents,jump,polarisation,Run Duration,Step per sec\n");
