@@ -19,6 +19,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -253,11 +254,10 @@ class Schedule {
       state.update(step_count);
       return;
     }
-    time = events_.front().time;
-    size_t k = 0;
-    while (k < events_.size() && !(events_[k].time > time)) ++k;
-    std::vector<Event> due(events_.begin(), events_.begin() + (ptrdiff_t)k);
-    events_.erase(events_.begin(), events_.begin() + (ptrdiff_t)k);
+    time = events_.begin()->time;
+    std::vector<Event> due;
+    while (!events_.empty() && !(events_.begin()->time > time))
+      due.push_back(std::move(events_.extract(events_.begin()).value()));
     for (Event& e : due) {
       e.agent->before_step(state);
       e.agent->step(state);
@@ -280,17 +280,16 @@ class Schedule {
     uint64_t id;
     bool repeating;
   };
-  static bool before(const Event& a, const Event& b) {
-    if (a.time != b.time) return a.time < b.time;
-    if (a.ordering != b.ordering) return a.ordering < b.ordering;
-    return a.id < b.id;
-  }
-  void push(Event e) {
-    auto it = std::upper_bound(events_.begin(), events_.end(), e, before);
-    events_.insert(it, std::move(e));
-  }
+  struct Before {
+    bool operator()(const Event& a, const Event& b) const {
+      if (a.time != b.time) return a.time < b.time;
+      if (a.ordering != b.ordering) return a.ordering < b.ordering;
+      return a.id < b.id;
+    }
+  };
+  void push(Event e) { events_.insert(std::move(e)); }
 
-  std::vector<Event> events_;
+  std::set<Event, Before> events_;  // ids are unique, so no two events compare equal
   uint64_t next_id_ = 0;
 };
 

@@ -278,6 +278,22 @@ static void cpu_tests() {
       EXPECT(lines[2] == "0,1,100,0.25,0.625,0.5,2000\n");
     }
   });
+  run_test("schedule: 200,000 agents", [] {
+    struct Count : krabgpu::Agent {
+      uint32_t* hits;
+      explicit Count(uint32_t* h) : hits(h) {}
+      void step(krabgpu::State&) override { ++*hits; }
+    };
+    struct Many : krabgpu::State {
+      std::vector<uint32_t> hits = std::vector<uint32_t>(200000, 0);
+      void init(krabgpu::Schedule& s) override {
+        for (size_t k = 0; k < hits.size(); ++k)
+          s.schedule_repeating(std::make_shared<Count>(&hits[k]), 0.0f, (int32_t)(k % 7));
+      }
+    } st;
+    krabgpu::simulate(st, 3);
+    EXPECT(std::all_of(st.hits.begin(), st.hits.end(), [](uint32_t h) { return h == 3; }));
+  });
   run_test("simulate: end_condition and repetitions", [] {
     Log log;
     LogState st(&log, 2);
