@@ -185,10 +185,12 @@ def test_flock_step_then_lazy_update_equals_schedule_run():
     assert np.array_equal(bits(sa), bits(sb))
 
 
-def test_one_million_boids_invariants():
-    """1,048,576 boids on a 20,480^2 torus: properties independent of size."""
+@pytest.mark.parametrize("side", [20480.0, 6476.0])  # the 1M configuration; at the 1,000-boid density
+def test_one_million_boids_invariants(side):
+    """1,048,576 boids on a torus: properties independent of size, and the
+    tile-resident and three-kernel paths agree bit for bit."""
     from krabmaga_amd import FlockParams
-    n, side = 1 << 20, 20480.0
+    n = 1 << 20
     p = FlockParams(seed=1)
     f = gpu_field(n, side, D, seed=1)
     f.schedule_run(p, 0, 20)
@@ -202,3 +204,10 @@ def test_one_million_boids_invariants():
     g.schedule_run(p, 0, 20)
     s2, _ = g.state_by_id(0)
     assert np.array_equal(bits(s1), bits(s2))
+    g.close()
+    h = gpu_field(n, side, D, seed=1)                             # the other device path
+    h.schedule_run(p, 0, 20, binned=True)
+    s3, _ = h.state_by_id(0)
+    h.close()
+    f.close()
+    assert np.array_equal(bits(s1), bits(s3))
