@@ -335,6 +335,37 @@ size_t or_get_neighbors_within_distance(const or_field2d *f, float x, float y, f
     return n;
 }
 
+int64_t or_set_object_locations(or_field2d *f, const uint32_t *ids, const float *x, const float *y,
+                                const float *hx, const float *hy, size_t n)
+{
+    for (size_t i = 0; i < n; ++i) {
+        or_bird b = {ids[i], x[i], y[i], hx ? hx[i] : 0.0f, hy ? hy[i] : 0.0f};
+        if (or_set_object_location(f, b, x[i], y[i]) != 0) return (int64_t)i;
+    }
+    return -1;
+}
+
+size_t or_neighbor_sets(const or_field2d *f, const float *qx, const float *qy, size_t n, float dist,
+                        uint64_t *row_ptr, uint32_t *ids, size_t cap)
+{
+    size_t vcap = 256, total = 0;
+    or_bird *v = (or_bird *)malloc(vcap * sizeof(or_bird));
+    for (size_t k = 0; k < n; ++k) {
+        size_t m = or_get_neighbors_within_distance(f, qx[k], qy[k], dist, v, vcap);
+        if (m > vcap) {
+            vcap = m;
+            v = (or_bird *)realloc(v, vcap * sizeof(or_bird));
+            or_get_neighbors_within_distance(f, qx[k], qy[k], dist, v, vcap);
+        }
+        row_ptr[k] = total;
+        for (size_t e = 0; e < m; ++e, ++total)
+            if (total < cap) ids[total] = v[e].id;
+    }
+    row_ptr[n] = total;
+    free(v);
+    return total;
+}
+
 /* field_2d.rs:472-516 */
 size_t or_get_neighbors_within_relax_distance(const or_field2d *f, float x, float y, float dist,
                                               or_bird *out, size_t cap)
@@ -404,6 +435,23 @@ size_t or_num_empty_bags(const or_field2d *f)
     for (int32_t i = 0; i < f->dw; ++i)
         for (int32_t j = 0; j < f->dh; ++j)
             if (f->bags[f->read][(size_t)((int64_t)i * f->dh + j)].n == 0) ++n;
+    return n;
+}
+
+/* field_2d.rs:727-762 with not_discretize (:345-353): writes up to cap
+ * positions, returns the count */
+size_t or_get_empty_bags(const or_field2d *f, float *x, float *y, size_t cap)
+{
+    size_t n = 0;
+    for (int32_t i = 0; i < f->dw; ++i)
+        for (int32_t j = 0; j < f->dh; ++j)
+            if (f->bags[f->read][(size_t)((int64_t)i * f->dh + j)].n == 0) {
+                if (n < cap) {
+                    x[n] = (float)i * f->discretization;
+                    y[n] = (float)j * f->discretization;
+                }
+                ++n;
+            }
     return n;
 }
 

@@ -92,6 +92,18 @@ size_t or_get_objects(const or_field2d *f, float x, float y, or_bird *out, size_
 size_t or_get_objects_unbuffered(const or_field2d *f, float x, float y, or_bird *out, size_t cap);
 size_t or_num_objects_at_location(const or_field2d *f, float x, float y);
 size_t or_num_empty_bags(const or_field2d *f);
+/* get_empty_bags as not_discretize positions in bag index order; writes up
+ * to cap, returns the count */
+size_t or_get_empty_bags(const or_field2d *f, float *x, float *y, size_t cap);
+/* Bulk helpers for large test inputs (the same calls in a loop):
+ * set_object_location of n birds in array order (returns the index of the
+ * first out-of-range bird, or -1), and get_neighbors_within_distance at n
+ * query points as CSR: row_ptr[n+1], ids (up to cap written); returns the
+ * total id count. */
+int64_t or_set_object_locations(or_field2d *f, const uint32_t *ids, const float *x, const float *y,
+                                const float *hx, const float *hy, size_t n);
+size_t or_neighbor_sets(const or_field2d *f, const float *qx, const float *qy, size_t n, float dist,
+                        uint64_t *row_ptr, uint32_t *ids, size_t cap);
 size_t or_nagents(const or_field2d *f);
 
 /* ---- Flockers model ---------------------------------------------------- */

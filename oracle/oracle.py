@@ -83,8 +83,15 @@ def lib():
         fn = getattr(L, name)
         fn.argtypes = [P, ctypes.c_float, ctypes.c_float, ctypes.POINTER(Bird), ctypes.c_size_t]
         fn.restype = ctypes.c_size_t
+    L.or_set_object_locations.argtypes = [P, _u32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_size_t]
+    L.or_set_object_locations.restype = ctypes.c_int64
+    L.or_neighbor_sets.argtypes = [P, _f32p, _f32p, ctypes.c_size_t, ctypes.c_float,
+                                   ctypes.POINTER(ctypes.c_uint64), _u32p, ctypes.c_size_t]
+    L.or_neighbor_sets.restype = ctypes.c_size_t
     L.or_num_objects_at_location.argtypes = [P, ctypes.c_float, ctypes.c_float]
     L.or_num_objects_at_location.restype = ctypes.c_size_t
+    L.or_get_empty_bags.argtypes = [P, _f32p, _f32p, ctypes.c_size_t]
+    L.or_get_empty_bags.restype = ctypes.c_size_t
     L.or_num_empty_bags.argtypes = [P]
     L.or_num_empty_bags.restype = ctypes.c_size_t
     L.or_nagents.argtypes = [P]
@@ -160,6 +167,34 @@ class Field2D:
         if rc != 0:
             raise IndexError("bag index out of range")
 
+    def set_object_locations(self, ids, x, y, hx=None, hy=None):
+        """set_object_location of every bird, in array order."""
+        ids = np.ascontiguousarray(ids, dtype=np.uint32)
+        x, y = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, y))
+        hx = None if hx is None else np.ascontiguousarray(hx, dtype=np.float32)
+        hy = None if hy is None else np.ascontiguousarray(hy, dtype=np.float32)
+        bad = self._L.or_set_object_locations(self._f, _up(ids), _fp(x), _fp(y),
+                                              None if hx is None else _fp(hx),
+                                              None if hy is None else _fp(hy), len(ids))
+        if bad >= 0:
+            raise IndexError(f"bag index out of range for bird {bad}")
+
+    def neighbor_sets(self, qx, qy, dist):
+        """get_neighbors_within_distance at every (qx[k], qy[k]) as CSR
+        (row_ptr, ids)."""
+        qx, qy = (np.ascontiguousarray(a, dtype=np.float32) for a in (qx, qy))
+        n = len(qx)
+        row_ptr = np.zeros(n + 1, dtype=np.uint64)
+        cap = max(16, 4 * n)
+        while True:
+            ids = np.zeros(cap, dtype=np.uint32)
+            total = self._L.or_neighbor_sets(self._f, _fp(qx), _fp(qy), n, dist,
+                                             row_ptr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                             _up(ids), cap)
+            if total <= cap:
+                return row_ptr, ids[:total].copy()
+            cap = int(total)
+
     def remove_object_location(self, bird_id, x, y):
         self._L.or_remove_object_location(self._f, bird_id, x, y)
 
@@ -192,6 +227,15 @@ class Field2D:
 
     def num_empty_bags(self):
         return self._L.or_num_empty_bags(self._f)
+
+    def get_empty_bags(self):
+        """(x, y) float32 arrays: not_discretize of every empty bag in bag
+        index order (field_2d.rs:727-762)."""
+        n = self._L.or_get_empty_bags(self._f, None, None, 0)
+        x = np.zeros(max(n, 1), np.float32)
+        y = np.zeros(max(n, 1), np.float32)
+        self._L.or_get_empty_bags(self._f, _fp(x), _fp(y), n)
+        return x[:n], y[:n]
 
     @property
     def nagents(self):
