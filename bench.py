@@ -9,10 +9,12 @@ flockers model: neighbour query + Bird::step for every agent and the
 rebinning of the field (lazy_update).  State is resident in HBM before the
 timed region starts.
 
-Beside it, in the same JSON line: the batched ensemble (512 replicas of
-uneven size, one launch per step for all of them, layout chosen by the
-autotuner) against the same runs launched one by one, and the serving loop
-(Poisson arrivals admitted by HBM footprint) with p50/p99 latency.
+Beside it, in the same JSON line: the clustered-density workload (8,388,608
+boids, 64 Gaussian blobs), the batched ensemble (512 replicas of uneven
+size, one launch per step for all of them, layout chosen by the autotuner)
+against the same runs launched one by one, and the serving loop (Poisson
+arrivals admitted by HBM footprint) with p50/p99 latency, a sweep of the
+arrival rate and the oracle's worker pool on the host cores.
 
 Prints one JSON line.  `python bench.py --gpus N` with N > 1 exits: this
 build measures one GPU.
@@ -29,11 +31,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "agent-updates/s (1M flockers f32) & ensemble p50/p99 latency, batched vs plain"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
-# Algorithmic bytes per agent and launch (f32 x, y, hx, hy + u32 id = 20 B):
+# Algorithmic bytes per agent-update of a whole schedule step (f32 x, y, hx,
+# hy + u32 id = 20 B per agent): binning reads and writes the agent (40 B),
+# the fused query + Bird::step reads it and writes position and heading
+# (36 B).  The roofline prices every kernel split of the step at this figure.
+BYTES_STEP = 76
+# What individual kernels move per agent (reported beside it):
 BYTES_RT = 20 + 20       # tile-resident step: read the agent, write it back
 BYTES_FLOCK = 20 + 16    # three-kernel path, fused query + step
-BYTES_SCATTER = 20 + 20  # three-kernel path, rebinning
-BYTES_STEP = 76          # per agent-update of a whole step (binning 40 + query/step 36)
+CPU_SHARE = 16           # host cores that go with one GPU of the box
 
 
 class Hip:
@@ -81,6 +87,16 @@ class Hip:
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cores():
+    """Cores this process may run on, and the threads a CPU baseline uses:
+    the box's share for one GPU (16), or fewer if fewer are available."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return avail, max(1, min(CPU_SHARE, avail))
 
 
 def plain_workload():
@@ -256,7 +272,8 @@ def cpu_ensemble(specs, threads):
     """explore_parallel!'s rayon path: the oracle on a thread pool, one run
     per thread at a time, bounded to one schedule step of every replica."""
     from oracle import oracle as O
-    nthreads = threads or min(16, os.cpu_count() or 1)
+    avail, share = host_cores()
+    nthreads = threads or share
     jobs = [{"n": s.agents, "side": s.width, "discretization": s.discretization, "steps": 1,
              "params": O.params(cohesion=s.params.cohesion, avoidance=s.params.avoidance,
                                 consistency=s.params.consistency,
@@ -266,9 +283,112 @@ def cpu_ensemble(specs, threads):
     wall, _, _ = O.run_ensemble(jobs, nthreads)
     agents = sum(s.agents for s in specs)
     return {"value": agents / wall, "unit": "agent-updates/s", "cores": nthreads,
-            "kind": "port",
+            "host_cores": avail, "kind": "port",
             "sample": f"1 schedule step of each of the {len(specs)} replicas "
-                      f"({agents} agent-updates, {wall:.1f} s)"}
+                      f"({agents} agent-updates, {wall:.1f} s) on a pool of {nthreads} threads "
+                      f"(the box's {CPU_SHARE}-core share per GPU; {avail} cores visible)"}
+
+
+def run_clustered(args, hip):
+    """8,388,608 boids on a 40,960 x 40,960 torus, 80 % in 64 Gaussian blobs
+    (sigma 200) and 20 % uniform: a bounded number of timed steps."""
+    import numpy as np
+    from krabmaga_amd import Field2D, FlockParams, workloads
+    n, side, seed = 8_388_608, 40_960.0, 2
+    t0 = time.perf_counter()
+    x, y, hx, hy = workloads.clustered(n, side, blobs=64, sigma=200.0, uniform_frac=0.2,
+                                       seed=seed)
+    log(f"clustered: {n} boids generated in {time.perf_counter() - t0:.1f} s")
+    p = FlockParams(seed=seed)
+    f = Field2D(side, side, 10.0 / 1.5, True, n)
+    f.set_object_locations(np.arange(n, dtype=np.uint32), x, y, hx, hy)
+    del x, y, hx, hy
+    warm = args.clustered_warmup
+    f.schedule_run(p, 0, warm)
+    f.sync()
+    steps = args.clustered_steps
+    ms, _ = timed_run(hip, f, p, warm, steps, binned=False)
+    st = f.stats()
+    # per-kernel time over a few steps launched one by one
+    f.profile(True)
+    f.schedule_run(p, warm + steps, args.clustered_profile_steps, use_graph=False)
+    kern = {}
+    for k in ("rt_step", "flock", "scan", "scatter", "sort_bins"):
+        tot, cnt = f.profile_read(k)
+        if cnt:
+            kern[k] = {"total_us_per_step": 1e3 * tot / args.clustered_profile_steps,
+                       "launches": cnt}
+    f.profile(False)
+    f.close()
+    value = n * steps / (ms * 1e-3)
+    log(f"clustered: {ms / steps:.3f} ms/step, {value:.3e} agent-updates/s")
+    return {
+        "workload": "flockers clustered density: 8,388,608 boids, 40,960 x 40,960 torus, 64 "
+                    "Gaussian blobs (sigma 200) + 20 % uniform, radius 10, weights 1, jump 0.7",
+        "agents": n, "warmup": warm, "steps": steps, "ms_per_step": ms / steps,
+        "value": value, "unit": "agent-updates/s",
+        "path": "tile-resident" if st["rt_steps"] else "three-kernel (flock + scan + scatter "
+                "+ bin sort)",
+        "roofline": {"bytes_per_agent_update": BYTES_STEP,
+                     "achieved_GBps": BYTES_STEP * value / 1e9,
+                     "frac": BYTES_STEP * value / HBM_PEAK},
+        "kernels_us_per_step": {k: round(v["total_us_per_step"], 1) for k, v in kern.items()},
+        "layout": {k: st[k] for k in ("rt_active", "rt_steps", "tiles", "dense_tiles")},
+    }
+
+
+def cpu_serving(reqs, duration, max_steps):
+    """The rayon path for the serving stream: the oracle on a worker pool,
+    one request per thread at a time, each request bounded to `max_steps`
+    schedule steps."""
+    from oracle import oracle as O
+    avail, nthreads = host_cores()
+    jobs = []
+    for r in reqs:
+        s, pp = r.spec, r.spec.params
+        jobs.append({"n": s.agents, "side": s.width, "discretization": s.discretization,
+                     "steps": min(r.steps, max_steps),
+                     "params": O.params(cohesion=pp.cohesion, avoidance=pp.avoidance,
+                                        consistency=pp.consistency, randomness=pp.randomness,
+                                        momentum=pp.momentum, jump=pp.jump, radius=pp.radius,
+                                        seed=pp.seed),
+                     "heading_mode": s.heading_mode})
+    wall, per_run, _ = O.run_ensemble(jobs, nthreads)
+    upd = sum(j["n"] * j["steps"] for j in jobs)
+    value = upd / wall
+    work = sum(r.spec.agents * r.steps for r in reqs) / len(reqs)
+    return {"value": value, "unit": "agent-updates/s", "cores": nthreads, "host_cores": avail,
+            "kind": "port",
+            "sustainable_rate_runs_per_s": value / work,
+            "sample": f"the {len(reqs)} requests of the stream, each run for min(steps, "
+                      f"{max_steps}) schedule steps on a pool of {nthreads} threads ({upd} "
+                      f"agent-updates, {wall:.1f} s); a request's full run averages "
+                      f"{work:.3g} agent-updates, so the pool sustains "
+                      f"{value / work:.3g} runs/s against {len(reqs) / duration:g} offered"}
+
+
+def serving_sweep(args):
+    """The serving loop at several Poisson arrival rates: served against
+    offered load and the latency at each."""
+    from krabmaga_amd import serve, workloads
+    out = []
+    for rate in args.sweep_rates:
+        dur = max(args.sweep_min_seconds, args.sweep_requests / rate)
+        reqs = workloads.serving(rate=rate, duration=dur, seed=40 + int(rate))
+        if not reqs:
+            continue
+        offered = sum(r.spec.agents * r.steps for r in reqs) / dur
+        srv = serve.Server(args.serve_budget_gb * 2**30, keep_state=True)
+        done = srv.serve(reqs)
+        sm = serve.summary(done, srv.wall)
+        row = {"rate_runs_per_s": rate, "requests": len(reqs), "arrivals_s": dur,
+               "offered_agent_updates_per_s": offered,
+               "served_agent_updates_per_s": sm["agent_updates_per_s"],
+               "p50_latency_s": sm["p50_latency_s"], "p99_latency_s": sm["p99_latency_s"],
+               "wall_s": sm["wall_s"]}
+        log(f"serving sweep: {row}")
+        out.append(row)
+    return out
 
 
 def run_serving(args, hip):
@@ -302,6 +422,16 @@ def run_serving(args, hip):
                                  "read-back timed, as a served request" % (len(sample), len(reqs))},
         "vs_one_by_one": sm["agent_updates_per_s"] / obo_value,
     })
+    if args.sweep_rates:
+        sweep = serving_sweep(args)
+        # the highest rate whose offered load was served (its stream drained
+        # within 1.25x its arrival window)
+        ok = [r for r in sweep if r["wall_s"] <= 1.25 * r["arrivals_s"]]
+        sm["rate_sweep"] = sweep
+        sm["sustainable"] = max(ok, key=lambda r: r["rate_runs_per_s"]) if ok else None
+    if not args.no_cpu:
+        log("serving CPU baseline (oracle worker pool) ...")
+        sm["cpu_baseline"] = cpu_serving(reqs, args.serve_seconds, args.serve_cpu_steps)
     return sm
 
 
@@ -324,6 +454,14 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--serve-cpu-steps", type=int, default=2)
+    ap.add_argument("--sweep-rates", type=float, nargs="*", default=[5.0, 10.0, 20.0, 40.0])
+    ap.add_argument("--sweep-requests", type=int, default=24)
+    ap.add_argument("--sweep-min-seconds", type=float, default=1.0)
+    ap.add_argument("--no-clustered", action="store_true")
+    ap.add_argument("--clustered-warmup", type=int, default=3)
+    ap.add_argument("--clustered-steps", type=int, default=20)
+    ap.add_argument("--clustered-profile-steps", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also copy 1 GiB device to device (a known-size reference for "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -340,20 +478,20 @@ def main():
         hip.copy_calibration(1 << 30)
     n, steps = w["n"], args.steps
     value = n * steps / (ms * 1e-3)
-    # dominant kernel of the timed path: the tile-resident step -- one
+    # dominant kernel of the timed path: the tile-resident step.  One
     # schedule step is its class launches side by side plus the launch for
-    # deferred tiles, so its duration is the timed region's time per step
-    # (HIP events on the field's stream, which forks to and joins the class
-    # streams) -- or the fused flock kernel when the field ran the
-    # three-kernel path
+    # deferred tiles, so a "launch" here is one schedule step and its
+    # duration is the timed region's time per step (HIP events on the
+    # field's stream, which forks to and joins the class streams).  Falls
+    # back to the fused flock kernel when the field ran the three-kernel path.
     if kern["rt_step"]["launches"]:
-        dom, dom_bytes = "rt_step", BYTES_RT
+        dom, kernel_bytes = "rt_step", BYTES_RT
         dom_us = 1e3 * ms / steps
     else:
-        dom, dom_bytes = "flock", BYTES_FLOCK
+        dom, kernel_bytes = "flock", BYTES_FLOCK
         dom_us = kern[dom]["avg_us"]
     traffic, traffic_src = profiled_traffic()
-    achieved = dom_bytes * n / (dom_us * 1e-6) if dom_us > 0 else 0.0
+    achieved = BYTES_STEP * n / (dom_us * 1e-6) if dom_us > 0 else 0.0
     out = {
         "metric": METRIC,
         "value": value,
@@ -378,18 +516,18 @@ def main():
             "launch_us": dom_us,
             "unit_of_launch": "one schedule step (all k_rt_step launches of the step)"
                               if dom == "rt_step" else "one launch",
+            "bytes_per_agent": BYTES_STEP,
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": traffic if dom == "rt_step" else None,
             "traffic_source": traffic_src if dom == "rt_step" else None,
-            "bytes_per_agent": dom_bytes,
-        },
-        "step_roofline": {
-            "bytes_per_agent_update": BYTES_STEP,
-            "achieved_GBps": BYTES_STEP * value / 1e9,
-            "frac": BYTES_STEP * value / HBM_PEAK,
+            "kernel_bytes_per_agent": kernel_bytes,
+            "kernel_frac": kernel_bytes * n / (dom_us * 1e-6) / HBM_PEAK if dom_us > 0 else 0.0,
+            "limiter": "latency: the step is HBM-bound by its algorithm, but each work-group "
+                       "runs a chain of dependent LDS and global round trips between barriers, "
+                       "so the launch runs far below the HBM roofline",
         },
         "kernels_us": {k: round(v["avg_us"], 3) for k, v in kern.items()},
         "three_kernel_path": {
@@ -403,6 +541,8 @@ def main():
     if not args.no_cpu:
         log("CPU baseline (oracle, 1 thread) ...")
         out["cpu_baseline"] = cpu_baseline(args)
+    if not args.no_clustered:
+        out["clustered"] = run_clustered(args, hip)
     if not args.no_batched:
         out["batched"] = run_batched(args, hip)
     if not args.no_serving:

@@ -12,7 +12,14 @@ step, scaled by what the same counters read for a device-to-device copy of
 known size (the copy the bench issues with --calibrate-copy).
 
     python -m krabmaga_amd.profsum --trace T.csv [--fetch F.csv --write W.csv]
-        --agents N --skip S --count K --title "..." --cmd "..." > out.md
+        [--counters C.csv] --agents N --skip S --count K --title "..." --cmd "..." > out.md
+
+Algorithmic bytes are 76 per agent-update (a step reads and writes each
+agent's 20 bytes once to bin it, reads them again and writes the 16 bytes of
+new position and heading), whatever the kernel split; the tile-resident
+kernel itself moves 40 (one read, one write).  --counters tabulates any
+other rocprofv3 --pmc counters (SQ_WAVES, SQ_WAIT_ANY, ...) per k_rt_step
+launch class.
 """
 import argparse
 import collections
@@ -21,7 +28,8 @@ import statistics
 
 KERNEL = "kg::k_rt_step"
 COPY = "__amd_rocclr_copyBuffer"
-BYTES_PER_AGENT = 40  # read the agent's 20 bytes, write 20 back
+BYTES_PER_AGENT = 76  # per agent-update of a whole schedule step
+KERNEL_BYTES_PER_AGENT = 40  # the tile-resident kernel: read 20 bytes, write 20 back
 
 
 def _name(r):
@@ -126,7 +134,13 @@ def summarize(a):
         alg = BYTES_PER_AGENT * a.agents
         lines.append(f"- algorithmic bytes: {BYTES_PER_AGENT} B x {a.agents} agents = {alg / 1e6:.2f} MB "
                      f"-> {alg / per_step / 1e3:.1f} GB/s = {alg / per_step / 1e3 / 8000 * 100:.2f} % "
-                     f"of 8 TB/s")
+                     f"of 8 TB/s (the kernel's own {KERNEL_BYTES_PER_AGENT} B: "
+                     f"{KERNEL_BYTES_PER_AGENT * a.agents / per_step / 1e3 / 8000 * 100:.2f} %)")
+        if len(groups) > len(sel):
+            allspans = [(max(r["end"] for r in g) - min(r["start"] for r in g)) / 1000.0
+                        for g in groups[1:]]
+            lines.append(f"- every step of the trace after the first ({len(allspans)} steps): "
+                         f"mean span {statistics.mean(allspans):.1f} us")
         lines.append(f"step_span_us: {per_step:.1f}")
     if a.fetch and a.write:
         f, fn = pmc_totals(a.fetch, "FETCH_SIZE")
@@ -160,7 +174,86 @@ def summarize(a):
                      f"{(fetch + write) / 1e6:.2f} MB ({(fetch + write) / alg:.2f}x the algorithmic "
                      f"{alg / 1e6:.2f} MB)")
         lines.append(f"traffic_bytes_per_step: {int(fetch + write)}")
+    if a.counters:
+        lines.extend(counter_table(a.counters))
+    if getattr(a, "step_kernel", None):
+        lines.extend(per_step_table(tr, a.step_kernel, a.skip, a.agents,
+                                    getattr(a, "step_grid", 0) or 0))
     return "\n".join(lines) + "\n"
+
+
+def per_step_table(tr, step_kernel, skip, agents, grid=0):
+    """For the three-kernel path: every launch of `step_kernel` (of `grid`
+    threads, if given) starts a schedule step; each kernel's time summed per
+    step over the steps after the first `skip`."""
+    starts = [r["start"] for r in tr
+              if r["name"] == step_kernel and (not grid or r["grid"] == grid)]
+    if len(starts) <= skip:
+        return []
+    starts = starts[skip:]
+    lo = starts[0]
+    tot = collections.defaultdict(float)
+    for r in tr:
+        if r["start"] >= lo and r["name"].startswith("kg::"):
+            tot[r["name"]] += (r["end"] - r["start"]) / 1000.0
+    n = len(starts)
+    out = [f"\nKernel time per schedule step ({step_kernel} launches from number {skip}: "
+           f"{n} steps; the kernels run one after another, so the times add up):\n",
+           "| kernel | us per step |", "|---|---|"]
+    for k in sorted(tot, key=lambda k: -tot[k]):
+        out.append(f"| {k} | {tot[k] / n:.1f} |")
+    step_us = sum(tot.values()) / n
+    out.append(f"\n- sum: {step_us:.1f} us per step; {BYTES_PER_AGENT} B x {agents} agents -> "
+               f"{BYTES_PER_AGENT * agents / step_us / 1e3:.1f} GB/s = "
+               f"{BYTES_PER_AGENT * agents / step_us / 1e3 / 8000 * 100:.2f} % of 8 TB/s")
+    out.append(f"kernel_us_per_step: {step_us:.1f}")
+    return out
+
+
+def counter_table(path):
+    """Per k_rt_step launch class (grid size): the mean of every counter per
+    dispatch, plus the derived shares the step kernel is judged by."""
+    vals = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(set)
+    for r in csv.DictReader(open(path)):
+        if _name(r) != KERNEL:
+            continue
+        k = _grid(r)
+        vals[k][r["Counter_Name"]] += float(r["Counter_Value"])
+        disp[k].add(r.get("Dispatch_Id") or r.get("Correlation_Id") or len(disp[k]))
+    if not vals:
+        return []
+    names = sorted({c for v in vals.values() for c in v})
+    out = ["\nHardware counters per k_rt_step dispatch (mean over dispatches of each grid):\n",
+           "| grid | dispatches | " + " | ".join(names) + " |",
+           "|---|---|" + "---|" * len(names)]
+    for k in sorted(vals, key=lambda k: -k):
+        nd = max(len(disp[k]), 1)
+        out.append(f"| {k} | {nd} | " + " | ".join(f"{vals[k][c] / nd:.4g}" for c in names) + " |")
+    big = max(vals, key=lambda k: k)
+    v = vals[big]
+    if v.get("SQ_WAVE_CYCLES"):
+        wc = v["SQ_WAVE_CYCLES"]
+        out.append("")
+        if "SQ_WAIT_ANY" in v:
+            out.append(f"- class-0 launches: waves wait on memory or dependencies "
+                       f"{v['SQ_WAIT_ANY'] / wc * 100:.1f} % of their cycles (SQ_WAIT_ANY / SQ_WAVE_CYCLES)")
+        if "SQ_ACTIVE_INST_VALU" in v:
+            out.append(f"- VALU busy {v['SQ_ACTIVE_INST_VALU'] / wc * 100:.1f} % of wave cycles "
+                       f"(SQ_ACTIVE_INST_VALU / SQ_WAVE_CYCLES)")
+        if "SQ_ACTIVE_INST_LDS" in v:
+            out.append(f"- LDS instructions issuing {v['SQ_ACTIVE_INST_LDS'] / wc * 100:.1f} % of wave "
+                       f"cycles")
+    if v.get("SQ_WAVES") and v.get("SQ_INSTS_VALU"):
+        out.append(f"- {v['SQ_INSTS_VALU'] / v['SQ_WAVES']:.0f} VALU instructions per wave"
+                   + (f", {v['SQ_INSTS_SALU'] / v['SQ_WAVES']:.0f} SALU" if v.get("SQ_INSTS_SALU")
+                      else "")
+                   + (f", {v['SQ_INSTS_LDS'] / v['SQ_WAVES']:.0f} LDS" if v.get("SQ_INSTS_LDS")
+                      else ""))
+    if v.get("SQ_LDS_BANK_CONFLICT") is not None and v.get("SQ_INSTS_LDS"):
+        out.append(f"- LDS bank-conflict cycles per LDS instruction: "
+                   f"{v['SQ_LDS_BANK_CONFLICT'] / v['SQ_INSTS_LDS']:.3f}")
+    return out
 
 
 def main():
@@ -168,6 +261,10 @@ def main():
     p.add_argument("--trace", required=True)
     p.add_argument("--fetch")
     p.add_argument("--write")
+    p.add_argument("--counters", help="a rocprofv3 --pmc counter CSV with SQ_* counters")
+    p.add_argument("--step-kernel", help="three-kernel path: the kernel that starts each step")
+    p.add_argument("--step-grid", type=int, default=0,
+                   help="only launches of --step-kernel with this grid size (threads)")
     p.add_argument("--agents", type=int, required=True)
     p.add_argument("--skip", type=int, default=0, help="steps to skip (warm-up)")
     p.add_argument("--count", type=int, default=0, help="steps to summarise (0: all)")

@@ -96,13 +96,14 @@ class Server:
                 if progress and self.ticks % 16 == 0:
                     progress(f"tick {self.ticks}: {len(active)} active, {len(done)} done, "
                              f"{len(reqs) - head} queued, {self.clock() - t0:.1f} s")
-                now = self.clock() - t0
                 for rid in [k for k, a in active.items() if a[2] == tick]:
                     i, r, _, adm, b = active.pop(rid)
                     state = ens.read(rid) if self.keep_state else None
                     ens.retire(rid)
                     used -= b
-                    done.append(Completed(i, r.arrival, adm, now, r.spec.agents, r.steps, state))
+                    # a request is done once its result is on the host
+                    done.append(Completed(i, r.arrival, adm, self.clock() - t0, r.spec.agents,
+                                          r.steps, state))
                 for a in active.values():
                     a[2] -= tick
             self.wall = self.clock() - t0

@@ -60,7 +60,24 @@ def test_summary_reports_span_and_traffic(tmp_path):
     a.trace, a.fetch, a.write = str(t), str(fp), str(wp)
     a.agents, a.skip, a.count = 1000, 0, 0
     a.title, a.cmd, a.pmc_cmd = "test", "python3 bench.py", None
+    a.counters = None
     out = profsum.summarize(a)
     assert "step_span_us: 202.0" in out
     # per step: (1000 + 24) KiB x 2 fetched + 800 KiB written
     assert f"traffic_bytes_per_step: {(1024 * 2 + 800) * 1024}" in out
+
+
+def test_counter_table_shares(tmp_path):
+    rows = []
+    for d in range(2):
+        rows += [[RT, 1235200, "SQ_WAVES", 100.0, d], [RT, 1235200, "SQ_WAVE_CYCLES", 1000.0, d],
+                 [RT, 1235200, "SQ_WAIT_ANY", 550.0, d], [RT, 1235200, "SQ_INSTS_VALU", 2000.0, d],
+                 [RT, 1235200, "SQ_ACTIVE_INST_VALU", 400.0, d],
+                 [RT512, 4096, "SQ_WAVES", 8.0, 10 + d]]
+    p = tmp_path / "c.csv"
+    write(p, PMC_COLS + ["Dispatch_Id"], rows)
+    out = "\n".join(profsum.counter_table(str(p)))
+    assert "| 1235200 | 2 |" in out
+    assert "55.0 % of their cycles" in out
+    assert "VALU busy 40.0 %" in out
+    assert "20 VALU instructions per wave" in out
