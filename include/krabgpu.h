@@ -118,6 +118,30 @@ int kg_num_objects_at_location(kg_field *f, float x, float y, size_t *n);
 /* Field2D::get_empty_bags (field_2d.rs:727-739), count only. */
 int kg_num_empty_bags(kg_field *f, size_t *n);
 
+/* Field2D::get_empty_bags (field_2d.rs:727-762): the position
+ * not_discretize(bag) = (cx * d, cy * d) (field_2d.rs:345-353) of every bag
+ * of the read buffer that holds no agent, in bag index order (cx ascending,
+ * then cy).  Writes up to cap positions to x, y; *n_out receives the full
+ * count and KG_E_CAPACITY is returned if it exceeds cap. */
+int kg_get_empty_bags(kg_field *f, float *x, float *y, size_t cap, size_t *n_out);
+
+/* Field2D::get_random_empty_bag (field_2d.rs:764-800) with a seeded draw in
+ * place of rand::rng(): the empty bag number floor(u * count / 2^64) in
+ * kg_get_empty_bags order, u = (w1 << 32) | w0 from the words w0, w1 of
+ * Philox4x32-10(ctr = {0, 0, 0, 0x42414753}, key = seed).  *found is 0 and
+ * x, y are untouched when no bag is empty (the reference returns None). */
+int kg_get_random_empty_bag(kg_field *f, uint64_t seed, float *x, float *y, int *found);
+
+/* Field2D::iter_objects / iter_objects_unbuffered (field_2d.rs:619-726):
+ * calls fn once per agent of the read (unbuffered = 0) or write
+ * (unbuffered = 1) buffer, bag by bag in bag index order, with the bag's
+ * not_discretize position and the agent.  Inside a bag the agents come in
+ * ascending id (the read buffer and a flock step's output) or in insertion
+ * order (set_object_location entries not yet swapped in). */
+typedef void (*kg_object_fn)(void *user, float bag_x, float bag_y, uint32_t id, float x, float y,
+                             float hx, float hy);
+int kg_iter_objects(kg_field *f, int unbuffered, kg_object_fn fn, void *user);
+
 /* Field2D::nagents (field_2d.rs:277): set_object_location calls counted
  * until the first lazy_update. */
 int kg_nagents(kg_field *f, size_t *n);

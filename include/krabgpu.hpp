@@ -132,6 +132,43 @@ class Field2D {
     check(kg_num_empty_bags(f_, &n));
     return n;
   }
+  /* field_2d.rs:727-762: not_discretize of every empty bag, bag index order */
+  std::vector<Real2D> get_empty_bags() {
+    std::vector<float> x(64), y(64);
+    for (;;) {
+      size_t n = 0;
+      const int rc = kg_get_empty_bags(f_, x.data(), y.data(), x.size(), &n);
+      if (rc == KG_E_CAPACITY) {
+        x.resize(n);
+        y.resize(n);
+        continue;
+      }
+      check(rc);
+      std::vector<Real2D> out(n);
+      for (size_t i = 0; i < n; ++i) out[i] = Real2D{x[i], y[i]};
+      return out;
+    }
+  }
+  /* field_2d.rs:764-800 with a seeded draw: false (None) if no bag is empty */
+  bool get_random_empty_bag(uint64_t seed, Real2D* out) {
+    int found = 0;
+    check(kg_get_random_empty_bag(f_, seed, &out->x, &out->y, &found));
+    return found != 0;
+  }
+  /* field_2d.rs:619-726: closure(bag position, id, position, heading) per
+   * agent, bag by bag */
+  template <class F>
+  void iter_objects(F&& closure, bool unbuffered = false) {
+    auto tramp = [](void* user, float bx, float by, uint32_t id, float x, float y, float hx,
+                    float hy) {
+      (*static_cast<F*>(user))(Real2D{bx, by}, id, Real2D{x, y}, Real2D{hx, hy});
+    };
+    check(kg_iter_objects(f_, unbuffered ? 1 : 0, tramp, &closure));
+  }
+  template <class F>
+  void iter_objects_unbuffered(F&& closure) {
+    iter_objects(std::forward<F>(closure), true);
+  }
   /* Field2D::nagents (field_2d.rs:277) */
   size_t nagents() {
     size_t n = 0;

@@ -28,7 +28,8 @@ EXPORTS = [
     "kg_field_create", "kg_field_destroy", "kg_set_object_locations", "kg_init_uniform",
     "kg_remove_object_location", "kg_lazy_update", "kg_get_neighbors_within_distance",
     "kg_get_neighbors_within_relax_distance", "kg_get_objects", "kg_num_objects_at_location",
-    "kg_num_empty_bags", "kg_nagents", "kg_count", "kg_neighbor_sets", "kg_flock_step",
+    "kg_num_empty_bags", "kg_get_empty_bags", "kg_get_random_empty_bag", "kg_iter_objects",
+    "kg_nagents", "kg_count", "kg_neighbor_sets", "kg_flock_step",
     "kg_schedule_run", "kg_read_agents", "kg_field_sync", "kg_field_stream",
     "kg_field_profile", "kg_field_profile_read", "kg_field_stats",
     "kg_run_footprint", "kg_ensemble_create", "kg_ensemble_destroy", "kg_ensemble_admit",
@@ -79,6 +80,9 @@ class Tuning(ctypes.Structure):
 _lib = None
 
 P = ctypes.c_void_p
+# kg_object_fn(user, bag_x, bag_y, id, x, y, hx, hy)
+OBJECT_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
+                             ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _f32p = ctypes.POINTER(ctypes.c_float)
 _szp = ctypes.POINTER(ctypes.c_size_t)
@@ -108,6 +112,10 @@ def lib():
     L.kg_get_objects.argtypes = [P, ctypes.c_float, ctypes.c_float, I, _u32p, ctypes.c_size_t, _szp]
     L.kg_num_objects_at_location.argtypes = [P, ctypes.c_float, ctypes.c_float, _szp]
     L.kg_num_empty_bags.argtypes = [P, _szp]
+    L.kg_get_empty_bags.argtypes = [P, _f32p, _f32p, ctypes.c_size_t, _szp]
+    L.kg_get_random_empty_bag.argtypes = [P, ctypes.c_uint64, _f32p, _f32p,
+                                          ctypes.POINTER(ctypes.c_int)]
+    L.kg_iter_objects.argtypes = [P, I, OBJECT_FN, P]
     L.kg_nagents.argtypes = [P, _szp]
     L.kg_count.argtypes = [P, I, _szp]
     L.kg_neighbor_sets.argtypes = [P, ctypes.c_float, _u32p, _u32p, ctypes.c_size_t, _szp]

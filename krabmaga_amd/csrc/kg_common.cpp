@@ -1,5 +1,5 @@
-// kg_common.cpp -- host-side helpers: error reporting, field geometry,
-// binning heuristics and the exact squared-distance threshold.
+// kg_common.cpp -- host-side helpers shared by the C ABI implementations:
+// per-thread error messages and the flock kernel's LDS capacity clamp.
 #include "kg_common.hpp"
 
 
@@ -18,6 +18,21 @@ int set_error(int code, const char* fmt, ...) {
 }
 
 void clear_error() { t_last_error.clear(); }
+
+// Keeps the flock kernel's LDS (dynamic staging + about 4 KB of static
+// tables) within the device's per-block limit.
+int clamp_lds_cap(int cap) {
+  int dev = 0, max_lds = 65536;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+        v > 0)
+      max_lds = v;
+  }
+  const size_t budget = (size_t)max_lds - 8192;
+  while (cap > 64 && flock_lds_bytes(cap) > budget) cap -= 64;
+  return cap;
+}
 
 }  // namespace kg
 

@@ -19,6 +19,10 @@ namespace kg {
 int set_error(int code, const char* fmt, ...);
 void clear_error();
 
+// Largest flock-kernel LDS capacity (staged agents) at or below `cap` whose
+// dynamic plus static LDS fits the device's per-block limit.
+int clamp_lds_cap(int cap);
+
 #define KG_HIP(expr)                                                                   \
   do {                                                                                 \
     hipError_t e_ = (expr);                                                            \

@@ -197,12 +197,12 @@ KG_HD void query_global(const Geom& g, const Buf& R, const uint32_t* __restrict_
       const uint32_t e = hi;
       while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (f2i(floorf(R.y[mid] / g.d)) < by) lo = mid + 1;
+        if (disc_axis(R.y[mid], g.d, g.inv_d) < by) lo = mid + 1;
         else hi = mid;
       }
       for (uint32_t q = lo; q < e; ++q) {
         const float ex = R.x[q], ey = R.y[q];
-        if (f2i(floorf(ey / g.d)) != by) break;
+        if (disc_axis(ey, g.d, g.inv_d) != by) break;
         if (check == 1 || dist2(x, y, ex, ey, g.w, g.h, tor) <= T) visit(q);
       }
     }

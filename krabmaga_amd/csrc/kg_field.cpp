@@ -141,21 +141,6 @@ int prof_collect(kg_field* f) {
   return KG_OK;
 }
 
-// Keeps the flock kernel's LDS (dynamic staging + about 4 KB of static
-// tables) within the device's per-block limit.
-int clamp_lds_cap(int cap) {
-  int dev = 0, max_lds = 65536;
-  if (hipGetDevice(&dev) == hipSuccess) {
-    int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
-        v > 0)
-      max_lds = v;
-  }
-  const size_t budget = (size_t)max_lds - 8192;
-  while (cap > 64 && flock_lds_bytes(cap) > budget) cap -= 64;
-  return cap;
-}
-
 int check_poison(kg_field* f) {
   if (f->poisoned)
     return set_error(KG_E_RANGE,
@@ -679,6 +664,131 @@ int kg_num_empty_bags(kg_field* f, size_t* n) {
   KG_HIP(hipMemcpyAsync(&distinct, f->scratch, 4, hipMemcpyDeviceToHost, f->stream));
   KG_HIP(hipStreamSynchronize(f->stream));
   *n = (size_t)f->g.dw * (size_t)f->g.dh - distinct;
+  return KG_OK;
+}
+
+// Empty bags of the read buffer on the device: *count of them, and -- when
+// xy is not NULL -- their not_discretize positions in bag index order in
+// device arrays *xy (x) and *xy + count (y), which the caller frees.
+static int empty_bags_dev(kg_field* f, uint32_t* count, float** xy) {
+  const uint64_t ncells = (uint64_t)f->g.dw * (uint64_t)f->g.dh;
+  const uint64_t nw64 = (ncells + 31) / 32;
+  if (nw64 >= 0xFFFFFFFFull) return set_error(KG_E_INVALID, "bag grid too large");
+  const uint32_t nwords = (uint32_t)nw64;
+  const uint32_t nscan = (nwords + kScanTile - 1) / kScanTile;
+  uint32_t *occ = nullptr, *cnt = nullptr, *off = nullptr, *cur = nullptr, *part = nullptr;
+  float* out = nullptr;
+  uint32_t nR = 0;
+  hipError_t e = hipMemcpy(&nR, f->offset + f->g.nbins, 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = dmalloc(&occ, nwords);
+  if (e == hipSuccess) e = dmalloc(&cnt, nwords);
+  if (e == hipSuccess) e = dmalloc(&off, nwords + 1);
+  if (e == hipSuccess) e = dmalloc(&cur, nwords);
+  if (e == hipSuccess) e = dmalloc(&part, nscan + 1);
+  if (e == hipSuccess) e = launch_fill_u32(occ, 0u, nwords, f->stream);
+  if (e == hipSuccess) e = launch_mark_bags(f->d_rep, f->R.b, nR, occ, f->stream);
+  if (e == hipSuccess) e = launch_count_empty(occ, nwords, ncells, cnt, f->stream);
+  ScanState ss;
+  ss.partial = part;
+  if (e == hipSuccess) e = launch_scan(cnt, off, cur, nwords, ss, nullptr, f->stream);
+  uint32_t total = 0;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&total, off + nwords, 4, hipMemcpyDeviceToHost, f->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
+  if (e == hipSuccess && xy) {
+    e = dmalloc(&out, 2 * (size_t)total);
+    if (e == hipSuccess)
+      e = launch_emit_empty(f->d_rep, occ, nwords, ncells, off, out, out + total, f->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(f->stream);
+  }
+  (void)hipFree(occ);
+  (void)hipFree(cnt);
+  (void)hipFree(off);
+  (void)hipFree(cur);
+  (void)hipFree(part);
+  if (e != hipSuccess) {
+    (void)hipFree(out);
+    return set_error(KG_E_HIP, "empty bags: %s", hipGetErrorString(e));
+  }
+  *count = total;
+  if (xy) *xy = out;
+  return KG_OK;
+}
+
+int kg_get_empty_bags(kg_field* f, float* x, float* y, size_t cap, size_t* n_out) {
+  clear_error();
+  if (!f || !n_out) return set_error(KG_E_INVALID, "field and n_out must not be NULL");
+  if (int rc = sync_and_check(f)) return rc;
+  uint32_t total = 0;
+  float* xy = nullptr;
+  if (int rc = empty_bags_dev(f, &total, &xy)) return rc;
+  const size_t m = std::min<size_t>(total, cap);
+  hipError_t e = hipSuccess;
+  if (m && x) e = hipMemcpy(x, xy, m * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && m && y) e = hipMemcpy(y, xy + total, m * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(xy);
+  if (e != hipSuccess) return set_error(KG_E_HIP, "get_empty_bags: %s", hipGetErrorString(e));
+  *n_out = total;
+  if (total > cap) return set_error(KG_E_CAPACITY, "%u empty bags do not fit in %zu", total, cap);
+  return KG_OK;
+}
+
+int kg_get_random_empty_bag(kg_field* f, uint64_t seed, float* x, float* y, int* found) {
+  clear_error();
+  if (!f || !x || !y || !found) return set_error(KG_E_INVALID, "arguments must not be NULL");
+  if (int rc = sync_and_check(f)) return rc;
+  uint32_t total = 0;
+  float* xy = nullptr;
+  if (int rc = empty_bags_dev(f, &total, &xy)) return rc;
+  *found = total != 0;
+  if (!total) {
+    (void)hipFree(xy);
+    return KG_OK;
+  }
+  // index = floor(u * total / 2^64) for the 64-bit word u of Philox(ctr =
+  // {0, 0, 0, "BAGS"}, key = seed)
+  const u32x4 o = philox4x32_10(0u, 0u, 0u, 0x42414753u, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint64_t u = ((uint64_t)o.v[1] << 32) | o.v[0];
+  const uint64_t idx = (uint64_t)(((unsigned __int128)u * total) >> 64);
+  hipError_t e = hipMemcpy(x, xy + idx, sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(y, xy + total + idx, sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(xy);
+  if (e != hipSuccess) return set_error(KG_E_HIP, "get_random_empty_bag: %s", hipGetErrorString(e));
+  return KG_OK;
+}
+
+int kg_iter_objects(kg_field* f, int unbuffered, kg_object_fn fn, void* user) {
+  clear_error();
+  if (!f || !fn) return set_error(KG_E_INVALID, "field and fn must not be NULL");
+  size_t n = 0;
+  const int buffer = unbuffered ? 1 : 0;
+  int rc = kg_read_agents(f, buffer, nullptr, nullptr, nullptr, nullptr, nullptr, 0, &n);
+  if (rc != KG_OK && rc != KG_E_CAPACITY) return rc;
+  std::vector<uint32_t> id(n);
+  std::vector<float> x(n), y(n), hx(n), hy(n);
+  if ((rc = kg_read_agents(f, buffer, id.data(), x.data(), y.data(), hx.data(), hy.data(), n, &n)))
+    return rc;
+  // bag index order (cx * dh + cy); inside a bag the read buffer and the
+  // flock step's output are in id order, set_object_location entries in
+  // insertion order
+  const bool by_id = buffer == 0 || f->w_aligned;
+  std::vector<uint64_t> key(n);
+  for (size_t i = 0; i < n; ++i) {
+    int32_t cx, cy;
+    discretize(f->g, x[i], y[i], &cx, &cy);
+    key[i] = (uint64_t)cx * (uint64_t)f->g.dh + (uint64_t)cy;
+  }
+  std::vector<size_t> ord(n);
+  for (size_t i = 0; i < n; ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) {
+    if (key[a] != key[b]) return key[a] < key[b];
+    return by_id && id[a] < id[b];
+  });
+  for (size_t k : ord) {
+    const float lx = (float)(int32_t)(key[k] / (uint64_t)f->g.dh) * f->g.d;
+    const float ly = (float)(int32_t)(key[k] % (uint64_t)f->g.dh) * f->g.d;
+    fn(user, lx, ly, id[k], x[k], y[k], hx[k], hy[k]);
+  }
   return KG_OK;
 }
 

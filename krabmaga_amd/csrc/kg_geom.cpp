@@ -43,6 +43,7 @@ bool make_geom(float w, float h, float d, int tor, uint64_t expected_agents, uin
   g->w = w;
   g->h = h;
   g->d = d;
+  g->inv_d = 1.0f / d;
   g->tor = tor ? 1 : 0;
   g->max_x = (int32_t)cw;
   g->max_y = (int32_t)ch;

@@ -334,11 +334,14 @@ hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W
 
 // ---------------------------------------------------------------------------
 // bin sort: every bin of a freshly scattered read buffer into (cell y, id)
-// order, ties by position; one block per (replica, tile).  A bin of at most
-// four times a wave's width is ranked by one wave in registers and rewritten
-// in place (every lane has loaded its entries before any lane stores); larger bins are
-// ranked by the whole block against keys staged in LDS, written to the
-// scratch buffer T at the same indices and copied back.
+// order, ties by position; one block per (replica, tile), the tile's bins
+// taken 256 at a time, one per thread, so every bin's size is read in one
+// round trip.  A bin of at most kSortSmall entries is sorted by its own lane
+// (sort_small_bin); a bin of up to four times a wave's width is ranked by
+// one wave in registers and rewritten in place (every lane has loaded its
+// entries before any lane stores); larger bins are ranked by the whole
+// block against keys staged in LDS, written to the scratch buffer T at the
+// same indices and copied back.
 
 constexpr int kSortThreads = 256;
 constexpr int kSortSlots = 4;                              // entries per lane on the wave path
@@ -351,12 +354,58 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t lane) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// One wave ranks bin [s, s + m) (kSortSmall < m <= kSortWaveMax): entry
+// lane + 64 u in slot u of the lane's registers.
+__device__ __forceinline__ void sort_wave_bin(const Geom& g, Buf R, uint32_t s, uint32_t m,
+                                              uint32_t lane) {
+  uint32_t idv[kSortSlots], rk[kSortSlots];
+  float xv[kSortSlots], yv[kSortSlots], hxv[kSortSlots], hyv[kSortSlots];
+  uint64_t kv[kSortSlots];
+#pragma unroll
+  for (int u = 0; u < kSortSlots; ++u) {
+    const uint32_t e = lane + (uint32_t)u * kWave;
+    idv[u] = 0;
+    xv[u] = yv[u] = hxv[u] = hyv[u] = 0.0f;
+    if (e < m) {
+      idv[u] = R.id[s + e];
+      xv[u] = R.x[s + e];
+      yv[u] = R.y[s + e];
+      hxv[u] = R.hx[s + e];
+      hyv[u] = R.hy[s + e];
+    }
+    kv[u] = e < m ? bin_key(g, yv[u], idv[u]) : ~0ull;
+    rk[u] = 0;
+  }
+  for (uint32_t j = 0; j < m; ++j) {
+    const uint32_t ju = j / kWave;  // the same on every lane
+    const uint64_t src = ju == 0 ? kv[0] : (ju == 1 ? kv[1] : (ju == 2 ? kv[2] : kv[3]));
+    const uint64_t kj = readlane64(src, j & (kWave - 1));
+#pragma unroll
+    for (int u = 0; u < kSortSlots; ++u) {
+      const uint32_t e = lane + (uint32_t)u * kWave;
+      rk[u] += (kj < kv[u]) || (kj == kv[u] && j < e);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kSortSlots; ++u) {
+    const uint32_t e = lane + (uint32_t)u * kWave;
+    if (e < m) {
+      R.id[s + rk[u]] = idv[u];
+      R.x[s + rk[u]] = xv[u];
+      R.y[s + rk[u]] = yv[u];
+      R.hx[s + rk[u]] = hxv[u];
+      R.hy[s + rk[u]] = hyv[u];
+    }
+  }
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
     const uint32_t* __restrict__ offset, Buf T) {
   __shared__ uint64_t s_key[kSortChunk];
+  __shared__ uint32_t s_mid[kSortThreads];
   __shared__ uint32_t s_big[kSortThreads];
-  __shared__ uint32_t s_nbig;
+  __shared__ uint32_t s_nmid, s_nbig;
   const int t = threadIdx.x;
   const uint32_t lane = (uint32_t)t & (kWave - 1);
   const uint32_t wid = (uint32_t)t / kWave;
@@ -366,62 +415,27 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
   const uint32_t nb = 1u << (g.tx_log2 + g.tr_log2);
   const uint32_t b0 = reps[r].bin_base + ((item - tile_prefix[r]) << (g.tx_log2 + g.tr_log2));
 
-  // bins of up to kSortWaveMax entries: one wave each, entry lane + 64 u in
-  // slot u of the lane's registers
-  for (uint32_t k = wid; k < nb; k += kSortThreads / kWave) {
-    const uint32_t s = offset[b0 + k];
-    const uint32_t m = offset[b0 + k + 1] - s;
-    if (m <= 1u || m > kSortWaveMax) continue;
-    uint32_t idv[kSortSlots], rk[kSortSlots];
-    float xv[kSortSlots], yv[kSortSlots], hxv[kSortSlots], hyv[kSortSlots];
-    uint64_t kv[kSortSlots];
-#pragma unroll
-    for (int u = 0; u < kSortSlots; ++u) {
-      const uint32_t e = lane + (uint32_t)u * kWave;
-      idv[u] = 0;
-      xv[u] = yv[u] = hxv[u] = hyv[u] = 0.0f;
-      if (e < m) {
-        idv[u] = R.id[s + e];
-        xv[u] = R.x[s + e];
-        yv[u] = R.y[s + e];
-        hxv[u] = R.hx[s + e];
-        hyv[u] = R.hy[s + e];
-      }
-      kv[u] = e < m ? bin_key(g, yv[u], idv[u]) : ~0ull;
-      rk[u] = 0;
-    }
-    for (uint32_t j = 0; j < m; ++j) {
-      const uint32_t ju = j / kWave;  // the same on every lane
-      const uint64_t src = ju == 0 ? kv[0] : (ju == 1 ? kv[1] : (ju == 2 ? kv[2] : kv[3]));
-      const uint64_t kj = readlane64(src, j & (kWave - 1));
-#pragma unroll
-      for (int u = 0; u < kSortSlots; ++u) {
-        const uint32_t e = lane + (uint32_t)u * kWave;
-        rk[u] += (kj < kv[u]) || (kj == kv[u] && j < e);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kSortSlots; ++u) {
-      const uint32_t e = lane + (uint32_t)u * kWave;
-      if (e < m) {
-        R.id[s + rk[u]] = idv[u];
-        R.x[s + rk[u]] = xv[u];
-        R.y[s + rk[u]] = yv[u];
-        R.hx[s + rk[u]] = hxv[u];
-        R.hy[s + rk[u]] = hyv[u];
-      }
-    }
-  }
-
-  // larger bins: the whole block, a batch of up to 256 bins at a time
   for (uint32_t k0 = 0; k0 < nb; k0 += kSortThreads) {
-    if (t == 0) s_nbig = 0u;
+    if (t == 0) {
+      s_nmid = 0u;
+      s_nbig = 0u;
+    }
     __syncthreads();
     if (k0 + (uint32_t)t < nb) {
       const uint32_t b = b0 + k0 + (uint32_t)t;
-      if (offset[b + 1] - offset[b] > kSortWaveMax) s_big[atomicAdd(&s_nbig, 1u)] = b;
+      const uint32_t s = offset[b], m = offset[b + 1] - s;
+      if (m >= 2u && m <= kSortSmall) sort_small_bin(g, R, s, m);
+      else if (m > kSortSmall && m <= kSortWaveMax) s_mid[atomicAdd(&s_nmid, 1u)] = b;
+      else if (m > kSortWaveMax) s_big[atomicAdd(&s_nbig, 1u)] = b;
     }
     __syncthreads();
+    // middle bins: one wave each
+    const uint32_t nmid = s_nmid;
+    for (uint32_t i = wid; i < nmid; i += kSortThreads / kWave) {
+      const uint32_t s = offset[s_mid[i]];
+      sort_wave_bin(g, R, s, offset[s_mid[i] + 1] - s, lane);
+    }
+    // larger bins: the whole block
     const uint32_t nbig = s_nbig;
     for (uint32_t i = 0; i < nbig; ++i) {
       const uint32_t s = offset[s_big[i]];
@@ -468,6 +482,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
       }
       __syncthreads();
     }
+    __syncthreads();
   }
 }
 
@@ -671,6 +686,77 @@ hipError_t launch_count_cells(const RepDev* rep, Buf R, const uint32_t* offset, 
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_count_cells, dim3((n + 255) / 256), dim3(256), 0, s, rep, R, offset, n,
                      distinct);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// empty bags (field_2d.rs:727-800): an occupancy bitmap over the bag grid in
+// the reference's bag index order (cx * dh + cy), 32 bags per word; the
+// empty bags of each word counted, scanned, then emitted in index order.
+
+__global__ void k_mark_bags(const RepDev* __restrict__ rep, Buf R, uint32_t n,
+                            uint32_t* __restrict__ occ) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Geom g = rep->g;
+  int32_t cx, cy;
+  discretize(g, R.x[i], R.y[i], &cx, &cy);
+  if (!cell_valid(g, cx, cy)) return;
+  const uint64_t idx = (uint64_t)cx * (uint64_t)g.dh + (uint64_t)cy;
+  atomicOr(&occ[idx >> 5], 1u << (idx & 31u));
+}
+
+__device__ __forceinline__ uint32_t empty_mask(const uint32_t* occ, uint32_t w, uint64_t ncells) {
+  const uint64_t lo = (uint64_t)w << 5;
+  const uint32_t valid = ncells - lo >= 32u ? 0xFFFFFFFFu : ((1u << (uint32_t)(ncells - lo)) - 1u);
+  return ~occ[w] & valid;
+}
+
+__global__ void k_count_empty(const uint32_t* __restrict__ occ, uint32_t nwords, uint64_t ncells,
+                              uint32_t* __restrict__ cnt) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < nwords) cnt[w] = (uint32_t)__popc(empty_mask(occ, w, ncells));
+}
+
+// not_discretize (field_2d.rs:345-353): (cx as f32 * d, cy as f32 * d)
+__global__ void k_emit_empty(const RepDev* __restrict__ rep, const uint32_t* __restrict__ occ,
+                             uint32_t nwords, uint64_t ncells, const uint32_t* __restrict__ off,
+                             float* __restrict__ x, float* __restrict__ y) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwords) return;
+  const Geom g = rep->g;
+  uint32_t m = empty_mask(occ, w, ncells);
+  uint32_t pos = off[w];
+  while (m) {
+    const uint32_t b = (uint32_t)__ffs(m) - 1u;
+    const uint64_t idx = ((uint64_t)w << 5) + b;
+    x[pos] = (float)(int32_t)(idx / (uint64_t)g.dh) * g.d;
+    y[pos] = (float)(int32_t)(idx % (uint64_t)g.dh) * g.d;
+    ++pos;
+    m &= m - 1u;
+  }
+}
+
+hipError_t launch_mark_bags(const RepDev* rep, Buf R, uint32_t n, uint32_t* occ, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mark_bags, dim3((n + 255) / 256), dim3(256), 0, s, rep, R, n, occ);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_empty(const uint32_t* occ, uint32_t nwords, uint64_t ncells,
+                              uint32_t* cnt, hipStream_t s) {
+  if (nwords == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_empty, dim3((nwords + 255) / 256), dim3(256), 0, s, occ, nwords,
+                     ncells, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_emit_empty(const RepDev* rep, const uint32_t* occ, uint32_t nwords,
+                             uint64_t ncells, const uint32_t* off, float* x, float* y,
+                             hipStream_t s) {
+  if (nwords == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_emit_empty, dim3((nwords + 255) / 256), dim3(256), 0, s, rep, occ, nwords,
+                     ncells, off, x, y);
   return hipGetLastError();
 }
 

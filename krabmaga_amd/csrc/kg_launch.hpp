@@ -61,6 +61,15 @@ hipError_t launch_objects_write(const RepDev* rep, Buf B, uint32_t n, int by_id,
                                 uint32_t* out, uint32_t cap, uint32_t* nout, hipStream_t s);
 hipError_t launch_count_cells(const RepDev* rep, Buf R, const uint32_t* offset, uint32_t n,
                               uint32_t* distinct, hipStream_t s);
+// Empty bags of the read buffer (bitmap `occ` of ceil(cells / 32) words,
+// zeroed by the caller; `cnt`, `off` one word and one word + 1 per bitmap
+// word; x, y receive off[nwords] positions).
+hipError_t launch_mark_bags(const RepDev* rep, Buf R, uint32_t n, uint32_t* occ, hipStream_t s);
+hipError_t launch_count_empty(const uint32_t* occ, uint32_t nwords, uint64_t ncells,
+                              uint32_t* cnt, hipStream_t s);
+hipError_t launch_emit_empty(const RepDev* rep, const uint32_t* occ, uint32_t nwords,
+                             uint64_t ncells, const uint32_t* off, float* x, float* y,
+                             hipStream_t s);
 hipError_t launch_remove(const RepDev* rep, Buf A, uint32_t n, uint32_t id, float x, float y,
                          uint32_t* n_after, hipStream_t s);
 

@@ -120,7 +120,51 @@ KG_HD void scatter_aligned_one(const RepDev& rp, Buf R, Buf W, uint32_t i, uint3
 // ascending id -- the canonical bag order -- and a query walks the run.
 
 KG_HD uint64_t bin_key(const Geom& g, float y, uint32_t id) {
-  return ((uint64_t)(uint32_t)f2i(floorf(y / g.d)) << 32) | id;
+  return ((uint64_t)(uint32_t)disc_axis(y, g.d, g.inv_d) << 32) | id;
+}
+
+// Bins of at most kSortSmall entries are sorted by one lane: keys in
+// registers (fixed indices only, so nothing lands in scratch), a bin already
+// in order left untouched, otherwise every entry loaded before any is
+// stored at its rank (ties by position).
+constexpr uint32_t kSortSmall = 8;
+
+KG_HD void sort_small_bin(const Geom& g, Buf R, uint32_t s, uint32_t m) {
+  uint64_t key[kSortSmall];
+#pragma unroll
+  for (uint32_t u = 0; u < kSortSmall; ++u)
+    key[u] = u < m ? bin_key(g, R.y[s + u], R.id[s + u]) : ~0ull;
+  bool sorted = true;
+#pragma unroll
+  for (uint32_t u = 1; u < kSortSmall; ++u)
+    if (u < m && key[u] < key[u - 1]) sorted = false;
+  if (sorted) return;
+  float x[kSortSmall], y[kSortSmall], hx[kSortSmall], hy[kSortSmall];
+  uint32_t id[kSortSmall], rank[kSortSmall];
+#pragma unroll
+  for (uint32_t u = 0; u < kSortSmall; ++u) {
+    const uint32_t q = u < m ? s + u : s;
+    x[u] = R.x[q];
+    y[u] = R.y[q];
+    hx[u] = R.hx[q];
+    hy[u] = R.hy[q];
+    id[u] = R.id[q];
+    rank[u] = 0;
+#pragma unroll
+    for (uint32_t v = 0; v < kSortSmall; ++v)
+      rank[u] += (v < m) && (key[v] < key[u] || (key[v] == key[u] && v < u));
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < kSortSmall; ++u) {
+    if (u < m) {
+      const uint32_t q = s + rank[u];
+      R.x[q] = x[u];
+      R.y[q] = y[u];
+      R.hx[q] = hx[u];
+      R.hy[q] = hy[u];
+      R.id[q] = id[u];
+    }
+  }
 }
 
 // ---- flock: fused neighbour query + Bird::step, one block per tile ---------
@@ -252,7 +296,7 @@ KG_HD void flock_stage(const RepDev& rp, const FlockTile& ft, uint32_t k, Buf R,
   S.hy[k] = R.hy[gi];
   S.id[k] = R.id[gi];
   S.gi[k] = gi;
-  S.cy[k] = f2i(floorf(y / rp.g.d));
+  S.cy[k] = disc_axis(y, rp.g.d, rp.g.inv_d);
   S.col[k] = (uint16_t)lo;
 }
 

@@ -393,7 +393,7 @@ KG_HD void rt_stage_put(const RepDev& rp, const RtTile& rt, uint32_t k, int col,
   S.shx[k] = hx;
   S.shy[k] = hy;
   S.sid[k] = id;
-  S.scy[k] = f2i(floorf(y / rp.g.d));
+  S.scy[k] = disc_axis(y, rp.g.d, rp.g.inv_d);
   const int32_t cx = rt.sc_lo + col;
   if (seg == 1 && cx >= rt.c0 && cx < rt.c1) {
     const uint32_t o = atom_add(nown, 1u);
@@ -828,13 +828,13 @@ KG_HD void rt_query_global(const RepDev& rp, const RtTile& rt, const RtDev& D, f
         const uint32_t sb = D.meta[tt].sbase;
         a = sb + bo[b];
         a_end = sb + bo[b + 1];
-        while (a < a_end && f2i(floorf(SB.y[a] / g.d)) < j) ++a;
+        while (a < a_end && disc_axis(SB.y[a], g.d, g.inv_d) < j) ++a;
       }
       // extras of cell (i, j) in ascending (id, source, index) by selection
       uint64_t last = 0;  // (id + 1) << 32 | position of the last taken extra
       bool have_last = false;
       for (;;) {
-        const bool ha = a < a_end && f2i(floorf(SB.y[a] / g.d)) == j;
+        const bool ha = a < a_end && disc_axis(SB.y[a], g.d, g.inv_d) == j;
         uint64_t best = ~0ull;
         float bx = 0, by = 0, bhx = 0, bhy = 0;
         uint32_t bid = 0;

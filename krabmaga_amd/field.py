@@ -122,6 +122,42 @@ class Field2D:
         check(self._L.kg_num_empty_bags(self._h, ctypes.byref(n)))
         return n.value
 
+    # -- field_2d.rs:727-800 --------------------------------------------------
+    def get_empty_bags(self):
+        """not_discretize positions (x, y) of every empty bag of the read
+        buffer, in bag index order (two float32 arrays)."""
+        cap = 0
+        while True:
+            x = np.zeros(max(cap, 1), np.float32)
+            y = np.zeros(max(cap, 1), np.float32)
+            n = ctypes.c_size_t()
+            rc = self._L.kg_get_empty_bags(self._h, f32p(x), f32p(y), cap, ctypes.byref(n))
+            if rc == _abi.KG_E_CAPACITY:
+                cap = int(n.value)
+                continue
+            check(rc)
+            return x[: n.value].copy(), y[: n.value].copy()
+
+    def get_random_empty_bag(self, seed):
+        """A seeded draw among the empty bags: (x, y), or None if there is
+        none (field_2d.rs:764-800)."""
+        x, y, found = ctypes.c_float(), ctypes.c_float(), ctypes.c_int()
+        check(self._L.kg_get_random_empty_bag(self._h, seed, ctypes.byref(x), ctypes.byref(y),
+                                              ctypes.byref(found)))
+        return (x.value, y.value) if found.value else None
+
+    # -- field_2d.rs:619-726 ---------------------------------------------------
+    def iter_objects(self, closure, unbuffered=False):
+        """closure((bag_x, bag_y), bird) for every agent, bag by bag; bird is
+        (id, x, y, hx, hy)."""
+        def fn(_user, bx, by, i, x, y, hx, hy):
+            closure((bx, by), (i, x, y, hx, hy))
+        cb = _abi.OBJECT_FN(fn)
+        check(self._L.kg_iter_objects(self._h, 1 if unbuffered else 0, cb, None))
+
+    def iter_objects_unbuffered(self, closure):
+        self.iter_objects(closure, unbuffered=True)
+
     @property
     def nagents(self):
         n = ctypes.c_size_t()

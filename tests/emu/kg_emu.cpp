@@ -490,6 +490,10 @@ int emu_lazy_update(emu_field* f) {
   for (uint32_t b = 0; b < f->g.nbins; ++b) {
     const uint32_t s = f->offset2[b], m = f->offset2[b + 1] - s;
     if (m <= 1) continue;
+    if (m <= kSortSmall) {  // one lane on the device
+      sort_small_bin(f->g, R2, s, m);
+      continue;
+    }
     idx.resize(m);
     key.resize(m);
     for (uint32_t j = 0; j < m; ++j) {
