@@ -178,6 +178,33 @@ int kg_flock_step(kg_field *f, const kg_flock_params *p, uint64_t step);
 int kg_schedule_run(kg_field *f, const kg_flock_params *p, uint64_t first_step, uint32_t nsteps,
                     int flags);
 
+/* One Schedule::step of a flockers field whose agents live in caller-owned
+ * device memory, allocating nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * ids, x, y, hx, hy are device pointers to n agents.  They go into a field
+ * described by `desc` (the geometry kg_field_create makes; n <= capacity)
+ * by set_object_location, the field is lazily updated, every agent takes
+ * Bird::step at schedule step `step`, and the field is lazily updated again.
+ * out_ids .. out_hy (device pointers to n entries each, not overlapping the
+ * inputs) receive the new read buffer in storage order: the agents
+ * kg_read_agents(f, 0) returns after kg_set_object_locations,
+ * kg_lazy_update, kg_flock_step(f, p, step) and kg_lazy_update on a
+ * kg_field of the same description.  `stream` is a hipStream_t (NULL: the
+ * null stream); the call synchronizes it.  KG_E_RANGE if an input location
+ * falls outside the bag grid.
+ *
+ * Every buffer beyond the inputs and outputs is carved from one static
+ * device array of 64 MiB per GPU, held until the process ends.  A field that
+ * needs more (40 B per agent plus 16 B per bin: about 1.3 M agents on a
+ * sparse field) is rejected with KG_E_CAPACITY before any work.  Calls on
+ * one GPU share that array: calls that overlap on one GPU race. */
+int kg_schedule_step_static(const kg_field_desc *desc, const kg_flock_params *p, uint64_t step,
+                            const uint32_t *ids, const float *x, const float *y, const float *hx,
+                            const float *hy, uint32_t n, uint32_t *out_ids, float *out_x,
+                            float *out_y, float *out_hx, float *out_hy, void *stream);
+
 /* Copies a buffer (0 = read, 1 = write) to the host in storage order.  Any
  * output pointer may be NULL.  *n receives the agent count; KG_E_CAPACITY
  * if it exceeds cap. */

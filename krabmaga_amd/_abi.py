@@ -30,7 +30,7 @@ EXPORTS = [
     "kg_get_neighbors_within_relax_distance", "kg_get_objects", "kg_num_objects_at_location",
     "kg_num_empty_bags", "kg_get_empty_bags", "kg_get_random_empty_bag", "kg_iter_objects",
     "kg_nagents", "kg_count", "kg_neighbor_sets", "kg_flock_step",
-    "kg_schedule_run", "kg_read_agents", "kg_field_sync", "kg_field_stream",
+    "kg_schedule_run", "kg_schedule_step_static", "kg_read_agents", "kg_field_sync", "kg_field_stream",
     "kg_field_profile", "kg_field_profile_read", "kg_field_stats",
     "kg_run_footprint", "kg_ensemble_create", "kg_ensemble_destroy", "kg_ensemble_admit",
     "kg_ensemble_retire", "kg_ensemble_run", "kg_ensemble_steps", "kg_ensemble_read",
@@ -122,6 +122,9 @@ def lib():
     L.kg_flock_step.argtypes = [P, ctypes.POINTER(FlockParams), ctypes.c_uint64]
     L.kg_schedule_run.argtypes = [P, ctypes.POINTER(FlockParams), ctypes.c_uint64,
                                   ctypes.c_uint32, I]
+    L.kg_schedule_step_static.argtypes = [ctypes.POINTER(FieldDesc), ctypes.POINTER(FlockParams),
+                                          ctypes.c_uint64, P, P, P, P, P, ctypes.c_uint32, P, P,
+                                          P, P, P, P]
     L.kg_read_agents.argtypes = [P, I, _u32p, _f32p, _f32p, _f32p, _f32p, ctypes.c_size_t, _szp]
     L.kg_field_sync.argtypes = [P]
     L.kg_field_stream.argtypes = [P]

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkrabgpu.so")
 
 SOURCES = ["kg_kernels.hip", "kg_common.cpp", "kg_geom.cpp", "kg_rt_state.cpp", "kg_field.cpp",
-           "kg_ensemble.cpp"]
+           "kg_ensemble.cpp", "kg_static.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
