@@ -1,6 +1,6 @@
 """The serving loop on the device: runs that arrive over time, overlap on
 the GPU and leave at their own step counts produce exactly the bits they
-produce alone."""
+produce alone, and the oracle's bits."""
 import numpy as np
 import pytest
 
@@ -39,3 +39,30 @@ def test_served_runs_equal_runs_alone(budget_mb):
         assert np.array_equal(got["id"][order], rids)
         bad = np.flatnonzero(np.any(st.view(np.uint32) != ref.view(np.uint32), axis=0))
         assert bad.size == 0, f"request {i} (n={r.spec.agents}, steps={r.steps}): {bad.size} differ"
+
+
+def test_served_runs_equal_the_oracle(oracle):
+    """Served runs against the oracle's Flocker (Field2D + Bird::step on the
+    CPU), every run of a stream whose runs overlap on the device."""
+    reqs = serving(rate=400.0, duration=0.04, seed=13, n_min=300, n_max=6000, s_min=5, s_max=60)
+    assert len(reqs) >= 6
+    srv = Server(4096 * 2**20, keep_state=True, max_tick=16)
+    done = srv.serve(reqs)
+    assert len(done) == len(reqs)
+    for c in done:
+        s, p = reqs[c.index].spec, reqs[c.index].spec.params
+        x, y, hx, hy = oracle.init_uniform(s.agents, s.width, s.height, s.init_seed,
+                                           s.heading_mode)
+        fl = oracle.Flocker(s.width, s.height, s.discretization, True, x, y, hx, hy,
+                            oracle.params(cohesion=p.cohesion, avoidance=p.avoidance,
+                                          consistency=p.consistency, randomness=p.randomness,
+                                          momentum=p.momentum, jump=p.jump, radius=p.radius,
+                                          seed=p.seed))
+        fl.step(c.steps)
+        ref = fl.state()
+        got = c.state
+        order = np.argsort(got["id"], kind="stable")
+        assert np.array_equal(got["id"][order], np.arange(s.agents, dtype=np.uint32))
+        st = np.stack([got["x"][order], got["y"][order], got["hx"][order], got["hy"][order]])
+        bad = np.flatnonzero(np.any(st.view(np.uint32) != ref.view(np.uint32), axis=0))
+        assert bad.size == 0, f"request {c.index} (n={s.agents}, steps={c.steps}): {bad.size} differ"
