@@ -466,10 +466,11 @@ int kg_run_footprint(const kg_run_desc* run, const kg_tuning* tuning, uint64_t* 
   std::string why;
   if (!run_geom(*run, tu, &g, nullptr, &why)) return set_error(KG_E_INVALID, "%s", why.c_str());
   const uint64_t n = run->agents;
-  // agent buffers (append x2, binned), stayer regions (two parities, about
-  // 2.5x the count) and inboxes with the layout's growth headroom of half
-  // again, spill, bins (count, offset, cursor) and the per-tile tables
-  *bytes = n * 20 * 3 + n * 40 * 3 * 3 / 2 + n * 44 + (uint64_t)g.nbins * 12 +
+  // agent buffers (append x2, binned) and stayer regions (two parities,
+  // about 2.5x the count) and inboxes, all with the growth headroom of half
+  // again that reserve_buf and the layout give them; spill, bins (count,
+  // offset, cursor) and the per-tile tables
+  *bytes = n * 20 * 3 * 3 / 2 + n * 40 * 3 * 3 / 2 + n * 44 + (uint64_t)g.nbins * 12 +
            (uint64_t)g.ntiles * ((uint64_t)(g.tx * g.tr + 1) * 8 + 64) * 3 / 2;
   return KG_OK;
 }

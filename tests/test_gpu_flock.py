@@ -108,7 +108,7 @@ def test_single_step_within_tolerance(oracle, heading_mode):
 @pytest.mark.parametrize("n,side,steps,seed", [
     (1000, 200.0, 200, 0),      # the flockers CPU configuration, whole run
     (20000, 900.0, 20, 1),
-    (50000, 8000.0, 10, 2),     # sparse, the 1M-boid density
+    (50000, 4472.0, 10, 2),     # sparse: 0.0025 birds per unit^2, the 1M-boid field's density
 ])
 def test_multi_step_bit_exact(oracle, n, side, steps, seed):
     from krabmaga_amd import FlockParams
