@@ -175,8 +175,13 @@ KG_HD Window query_window(const Geom& g, float x, float y, int32_t dd) {
 // (x, y, dist).  Bags are read from the binned read buffer; inside a bag
 // agents are visited by ascending id (equal ids in unspecified order).
 // visit(e) receives the read-buffer index.  Works for any radius and both
-// toroidal modes; cost grows with bin occupancy squared, so the hot kernels
-// use the LDS tile path instead.
+// toroidal modes.
+//
+// A bin holds rows of one column sorted by (cell y, id), so the bag of
+// (bx, by) is the run of cell y == by.  Its start is found by a linear scan
+// from the bin's start when the bin is small, by bisection otherwise; the
+// window's next row in the same bin starts where this row's run ended, so a
+// column costs one lookup per bin it touches.
 template <class Visit>
 KG_HD void query_global(const Geom& g, const Buf& R, const uint32_t* __restrict__ offset,
                              float x, float y, float dist, float T, bool relax, Visit&& visit) {
@@ -185,26 +190,40 @@ KG_HD void query_global(const Geom& g, const Buf& R, const uint32_t* __restrict_
   const Window wdw = query_window(g, x, y, dd);
   const bool tor = g.tor != 0;
   for (int64_t i = wdw.min_i; i < (int64_t)wdw.max_i + 1; ++i) {
+    const int32_t bx = t_transform((int32_t)i, g.max_x);
+    uint32_t run_b = 0xFFFFFFFFu;  // bin and row whose run ended at run_end
+    int32_t run_by = 0;
+    uint32_t run_end = 0;
     for (int64_t j = wdw.min_j; j < (int64_t)wdw.max_j + 1; ++j) {
-      const int32_t bx = t_transform((int32_t)i, g.max_x);
       const int32_t by = t_transform((int32_t)j, g.max_y);
       const int check = relax ? 1 : check_circle_sq(g, bx, by, x, y, T);
       if (check == -1) continue;
-      // a bin holds rows of one column sorted by (cell y, id): the bag is
-      // the run of cell y == by, in ascending id
       const uint32_t b = bin_of(g, bx, by);
-      uint32_t lo = offset[b], hi = offset[b + 1];
-      const uint32_t e = hi;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (disc_axis(R.y[mid], g.d, g.inv_d) < by) lo = mid + 1;
-        else hi = mid;
+      const uint32_t e = offset[b + 1];
+      uint32_t q;
+      if (b == run_b && by == run_by + 1) {
+        q = run_end;
+      } else {
+        uint32_t lo = offset[b], hi = e;
+        if (hi - lo <= 8u) {
+          while (lo < hi && disc_axis(R.y[lo], g.d, g.inv_d) < by) ++lo;
+        } else {
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (disc_axis(R.y[mid], g.d, g.inv_d) < by) lo = mid + 1;
+            else hi = mid;
+          }
+        }
+        q = lo;
       }
-      for (uint32_t q = lo; q < e; ++q) {
+      for (; q < e; ++q) {
         const float ex = R.x[q], ey = R.y[q];
         if (disc_axis(ey, g.d, g.inv_d) != by) break;
         if (check == 1 || dist2(x, y, ex, ey, g.w, g.h, tor) <= T) visit(q);
       }
+      run_b = b;
+      run_by = by;
+      run_end = q;
     }
   }
 }

@@ -16,8 +16,12 @@ Differences from the reference, each forced by the device engine:
 - the reference's runs draw from an unseeded thread RNG; here run r of a
   sweep uses the Philox keys (seed << 32) | r, so a sweep is reproducible;
 - in the parallel sweep, the runs of one batch share one timed region, so
-  their "Run Duration" and "Step per sec" are the batch's;
-- `end_condition` is not evaluated: flockers' is the default `false`.
+  their "Run Duration" and "Step per sec" are the batch's (with an
+  end_condition, a run's duration ends at the step that stopped it);
+- `end_condition` (flockers' is the default `false`) is a host callable:
+  given one, runs advance one schedule step at a time and it is consulted
+  after each step, as simulate_explore! does (model_exploration.rs:
+  173-177); without one, steps run in batches of device graphs.
 """
 import csv
 import enum
@@ -248,7 +252,8 @@ def _parse(s, kind):
 # ---- the sweep ----
 
 def explore(nstep, rep_conf, inputs, outputs=("polarisation",), mode=ExploreMode.Matched,
-            parallel=True, seed=0, hbm_budget=200 << 30, tuning=None, keep_state=False):
+            parallel=True, seed=0, hbm_budget=200 << 30, tuning=None, keep_state=False,
+            end_condition=None):
     """Runs every configuration `rep_conf` times for `nstep` schedule steps
     and returns one FrameRow per run, configurations in order, repetitions
     in order within each.
@@ -260,7 +265,14 @@ def explore(nstep, rep_conf, inputs, outputs=("polarisation",), mode=ExploreMode
     (one Ensemble; repetitions numbered from 0), False explore_sequential!
     (one Field2D per run, one after another; repetitions numbered from 1).
     Parallel batches are cut so that their device footprint stays within
-    hbm_budget bytes."""
+    hbm_budget bytes.
+
+    end_condition(step, state, spec) -> bool, if given, is called after
+    every schedule step of every run (State::end_condition after
+    schedule.step, model_exploration.rs:173-177): step is the number of
+    steps the run has taken, state() reads its (4, n) array by bird id.  A
+    run that returns True stops there; its "Step per sec" counts the steps
+    it took, as the reference divides schedule.step by the duration."""
     if nstep < 0 or rep_conf < 0:
         raise ValueError("nstep and rep_conf must be non-negative")
     if isinstance(outputs, str):
@@ -272,12 +284,12 @@ def explore(nstep, rep_conf, inputs, outputs=("polarisation",), mode=ExploreMode
     specs = [flockers_spec(confs[i], (int(seed) << 32) | (i * rep_conf + j)) for i, j in runs]
     rows = [None] * len(runs)
 
-    def finish(k, st, dt):
+    def finish(k, st, dt, steps=nstep):
         i, j = runs[k]
         vals = {name: fn(st, specs[k]) for name, fn in outs.items()}
         dt32 = np.float32(dt)
         with np.errstate(divide="ignore", invalid="ignore"):
-            sps = np.float32(nstep) / dt32  # f32 division as in Rust: x / 0 = inf, 0 / 0 = NaN
+            sps = np.float32(steps) / dt32  # f32 division as in Rust: x / 0 = inf, 0 / 0 = NaN
         rows[k] = FrameRow(i, j if parallel else j + 1, dict(confs[i]), vals, float(dt32),
                            float(sps), st if keep_state else None)
 
@@ -288,14 +300,22 @@ def explore(nstep, rep_conf, inputs, outputs=("polarisation",), mode=ExploreMode
                 f.init_uniform(s.agents, s.init_seed, s.heading_mode)
                 f.sync()
                 t0 = time.perf_counter()
-                f.schedule_run(s.params, 0, nstep)
+                done = nstep
+                if end_condition is None:
+                    f.schedule_run(s.params, 0, nstep)
+                else:
+                    for step in range(nstep):
+                        f.schedule_run(s.params, step, 1)
+                        if end_condition(step + 1, lambda: f.state_by_id(0)[0], s):
+                            done = step + 1
+                            break
                 f.sync()
                 dt = time.perf_counter() - t0
                 # before its first lazy_update the birds are in the write buffer
-                st, _ = f.state_by_id(0 if nstep > 0 else 1)
+                st, _ = f.state_by_id(0 if done > 0 else 1)
             finally:
                 f.close()
-            finish(k, st, dt)
+            finish(k, st, dt, done)
         return rows
 
     for batch in _batches([footprint(s, tuning) for s in specs], hbm_budget):
@@ -303,10 +323,30 @@ def explore(nstep, rep_conf, inputs, outputs=("polarisation",), mode=ExploreMode
             ids = ens.admit([specs[k] for k in batch])
             ens.sync()
             t0 = time.perf_counter()
-            ens.run(nstep)
+            if end_condition is None:
+                ens.run(nstep)
+                ens.sync()
+                dt = time.perf_counter() - t0
+                for k, rid in zip(batch, ids):
+                    st, _ = ens.state_by_id(rid)
+                    finish(k, st, dt)
+                continue
+            live = dict(zip(batch, ids))
+            for step in range(nstep):
+                ens.run(1)
+                ens.sync()
+                for k, rid in list(live.items()):
+                    if end_condition(step + 1, lambda rid=rid: ens.state_by_id(rid)[0], specs[k]):
+                        dt = time.perf_counter() - t0
+                        st, _ = ens.state_by_id(rid)
+                        finish(k, st, dt, step + 1)
+                        ens.retire(rid)
+                        del live[k]
+                if not live:
+                    break
             ens.sync()
             dt = time.perf_counter() - t0
-            for k, rid in zip(batch, ids):
+            for k, rid in live.items():
                 st, _ = ens.state_by_id(rid)
                 finish(k, st, dt)
     return rows

@@ -236,6 +236,8 @@ int main() {
       {"flockers-200-shuffled", 1000, 200.0f, D, 1, 20, 0, 0, 0, 0, 99, 0, 10.0f},
       {"flockers-200-hb16-cap64", 1000, 200.0f, D, 1, 10, 1, 16, 64, 0, 0, 0, 10.0f},
       {"sparse-2048", 10000, 2048.0f, D, 1, 6, 2, 0, 0, 0, 5, 0, 10.0f},
+      {"sparse-2048-hb1-global", 10000, 2048.0f, D, 1, 6, 2, 1, 0, 1, 5, 0, 10.0f},
+      {"clustered-hb2-global", 6000, 700.0f, D, 1, 4, 4, 2, 0, 1, 0, 1, 10.0f},
       {"dense-60", 3000, 60.0f, D, 1, 8, 3, 0, 0, 0, 0, 0, 10.0f},
       {"clustered", 6000, 700.0f, D, 1, 8, 4, 0, 0, 0, 0, 1, 10.0f},
       {"clustered-cap128", 6000, 700.0f, D, 1, 4, 4, 0, 128, 0, 3, 1, 10.0f},

@@ -75,3 +75,38 @@ def test_zero_steps_reports_the_initial_flock(oracle):
                                             s.heading_mode))
         assert np.array_equal(bits(r.state), bits(init)), f"parallel={parallel}"
         assert r.step_per_sec == 0.0
+
+
+def test_end_condition_stops_each_run_at_its_step(oracle):
+    """State::end_condition after every schedule.step (model_exploration.rs:
+    173-177): run k stops after 3 + 2k steps (one condition reads the state);
+    its birds are the oracle's after that many steps and its Step per sec
+    counts those steps."""
+    inputs = {"agents": [300, 500, 800]}
+    stop = {}
+
+    def end_condition(step, state, spec):
+        k = spec.init_seed & 0xFFFFFFFF
+        if k == 1:
+            assert state().shape == (4, spec.agents)
+        stop[k] = step
+        return step >= 3 + 2 * k
+
+    for parallel in (True, False):
+        rows = explore(20, 1, inputs, mode=ExploreMode.Matched, seed=9, keep_state=True,
+                       parallel=parallel, end_condition=end_condition)
+        for k, r in enumerate(rows):
+            assert stop[k] == 3 + 2 * k, f"parallel={parallel} run {k}"
+            assert r.step_per_sec == pytest.approx((3 + 2 * k) / r.run_duration, rel=1e-6)
+            s = flockers_spec(r.inputs, (9 << 32) | k)
+            x, y, hx, hy = oracle.init_uniform(s.agents, s.width, s.height, s.init_seed,
+                                                s.heading_mode)
+            p = s.params
+            fl = oracle.Flocker(s.width, s.height, s.discretization, True, x, y, hx, hy,
+                                oracle.params(cohesion=p.cohesion, avoidance=p.avoidance,
+                                              consistency=p.consistency,
+                                              randomness=p.randomness, momentum=p.momentum,
+                                              jump=p.jump, radius=p.radius, seed=p.seed))
+            fl.step(3 + 2 * k)
+            assert np.array_equal(bits(r.state), bits(fl.state())), \
+                f"parallel={parallel} run {k}"
