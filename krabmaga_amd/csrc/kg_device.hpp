@@ -71,6 +71,57 @@ KG_HD int check_circle_sq(const Geom& g, int32_t bx, int32_t by, float lx,
   return 0;
 }
 
+// check_circle_sq of every cell of a window of at most 3 x 3 cells, packed
+// two bits per cell at bit 2 * (3 * kc + kr) for cell (bxs[kc], bys[kr]):
+// 1 = all four corners within T, 2 = all four beyond T, 0 = otherwise.
+// A corner's squared distance is fl(sx + sy) with sx the squared x term of
+// its column edge and sy the squared y term of its row edge; fl(+) is
+// monotone, so the largest of the four sums is fl(max sx + max sy) and the
+// smallest fl(min sx + min sy), and the two comparisons below decide exactly
+// what the four comparisons of check_circle_sq decide (a NaN query makes
+// both terms of a pair NaN, so both tests fail, as all eight do there).
+// ncol, nrow <= 3 are the window's column and row counts; columns and rows
+// are given as cell indices (already wrapped).
+KG_HD uint32_t window_checks(const Geom& g, const int32_t* bxs, const int32_t* bys, int ncol,
+                             int nrow, float lx, float ly, float T) {
+  const bool tor = g.tor != 0;
+  float xmx[3], xmn[3], ymx[3], ymn[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float nw_x = (float)bxs[k] * g.d;
+    const float sw_x = fminf(nw_x + g.d, g.w);
+    const float nw_y = (float)bys[k] * g.d;
+    const float ne_y = fminf(nw_y + g.d, g.h);
+    const float ax = tor ? toroidal_distance(nw_x, lx, g.w) : nw_x - lx;
+    const float bx = tor ? toroidal_distance(sw_x, lx, g.w) : sw_x - lx;
+    const float ay = tor ? toroidal_distance(nw_y, ly, g.h) : nw_y - ly;
+    const float by = tor ? toroidal_distance(ne_y, ly, g.h) : ne_y - ly;
+    const float ax2 = ax * ax, bx2 = bx * bx, ay2 = ay * ay, by2 = by * by;
+    xmx[k] = fmaxf(ax2, bx2);
+    xmn[k] = fminf(ax2, bx2);
+    ymx[k] = fmaxf(ay2, by2);
+    ymn[k] = fminf(ay2, by2);
+  }
+  uint32_t bits = 0;
+#pragma unroll
+  for (int kc = 0; kc < 3; ++kc) {
+#pragma unroll
+    for (int kr = 0; kr < 3; ++kr) {
+      const float hi = xmx[kc] + ymx[kr];
+      const float lo = xmn[kc] + ymn[kr];
+      const uint32_t c = hi <= T ? 1u : (lo > T ? 2u : 0u);
+      bits |= (kc < ncol && kr < nrow ? c : 0u) << (2 * (3 * kc + kr));
+    }
+  }
+  return bits;
+}
+
+// The check_circle_sq value of window cell (kc, kr) from window_checks' bits.
+KG_HD int window_check_at(uint32_t bits, int kc, int kr) {
+  const uint32_t c = (bits >> (2 * (3 * kc + kr))) & 3u;
+  return c == 1u ? 1 : (c == 2u ? -1 : 0);
+}
+
 // Accumulators of Bird::step's neighbour loop (bird.rs:54-81).
 struct FlockAcc {
   float x_avoid = 0.0f, y_avoid = 0.0f;
@@ -189,6 +240,19 @@ KG_HD void query_global(const Geom& g, const Buf& R, const uint32_t* __restrict_
   const int32_t dd = f2i(floorf(dist / g.d));
   const Window wdw = query_window(g, x, y, dd);
   const bool tor = g.tor != 0;
+  // windows of up to 3 x 3 cells: every cell's check_circle at once
+  const bool packed = !relax && wdw.max_i - wdw.min_i < 3 && wdw.max_j - wdw.min_j < 3;
+  uint32_t wchk = 0;
+  if (packed) {
+    int32_t bxs[3], bys[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      bxs[k] = t_transform(imin(wdw.min_i + k, wdw.max_i), g.max_x);
+      bys[k] = t_transform(imin(wdw.min_j + k, wdw.max_j), g.max_y);
+    }
+    wchk = window_checks(g, bxs, bys, wdw.max_i - wdw.min_i + 1, wdw.max_j - wdw.min_j + 1, x, y,
+                         T);
+  }
   for (int64_t i = wdw.min_i; i < (int64_t)wdw.max_i + 1; ++i) {
     const int32_t bx = t_transform((int32_t)i, g.max_x);
     uint32_t run_b = 0xFFFFFFFFu;  // bin and row whose run ended at run_end
@@ -196,7 +260,9 @@ KG_HD void query_global(const Geom& g, const Buf& R, const uint32_t* __restrict_
     uint32_t run_end = 0;
     for (int64_t j = wdw.min_j; j < (int64_t)wdw.max_j + 1; ++j) {
       const int32_t by = t_transform((int32_t)j, g.max_y);
-      const int check = relax ? 1 : check_circle_sq(g, bx, by, x, y, T);
+      const int check = relax ? 1
+                        : packed ? window_check_at(wchk, (int)(i - wdw.min_i), (int)(j - wdw.min_j))
+                                 : check_circle_sq(g, bx, by, x, y, T);
       if (check == -1) continue;
       const uint32_t b = bin_of(g, bx, by);
       const uint32_t e = offset[b + 1];

@@ -337,6 +337,14 @@ KG_HD void flock_agent_lds(const RepDev& rp, const FlockTile& ft, uint32_t k,
   // clamped window (field_2d.rs:401-416)
   const int32_t min_i = imax(0, cx - dd), max_i = imin(cx + dd, g.max_x - 1);
   const int32_t min_j = imax(0, cy - dd), max_j = imin(cy + dd, g.max_y - 1);
+  // windows of up to 3 x 3 cells: every cell's check_circle once per agent
+  const bool packed = max_i - min_i < 3 && max_j - min_j < 3;
+  uint32_t wchk = 0;
+  if (packed) {
+    const int32_t bxs[3] = {min_i, imin(min_i + 1, max_i), imin(min_i + 2, max_i)};
+    const int32_t bys[3] = {min_j, imin(min_j + 1, max_j), imin(min_j + 2, max_j)};
+    wchk = window_checks(g, bxs, bys, max_i - min_i + 1, max_j - min_j + 1, qx, qy, T);
+  }
   FlockAcc acc;
   for (int32_t i = min_i; i <= max_i; ++i) {
     const int c = i - ft.sc_lo;
@@ -352,7 +360,9 @@ KG_HD void flock_agent_lds(const RepDev& rp, const FlockTile& ft, uint32_t k,
     for (uint32_t e = lo; e < end; ++e) {
       const int32_t ey_c = S.cy[e];
       if (ey_c > max_j) break;
-      if (ey_c != cur) {
+      if (packed) {
+        chk = window_check_at(wchk, i - min_i, ey_c - min_j);
+      } else if (ey_c != cur) {
         cur = ey_c;
         chk = check_circle_sq(g, i, ey_c, qx, qy, T);
       }

@@ -548,48 +548,19 @@ KG_HD void rt_ecbase(int c, uint32_t ne, RtSmem S) {
   S.ecbase[c] = (uint16_t)n;
 }
 
-// Squared toroidal distance of one corner coordinate to a query coordinate,
-// the per-axis term of field_2d.rs:974-986 (dist2 forms dx*dx + dy*dy).
-KG_HD float rt_corner_sq(float corner, float q, float dim) {
-  const float d = toroidal_distance(corner, q, dim);
-  return d * d;
-}
-
-// check_circle (field_2d.rs:934-972) from the four per-axis squared corner
-// distances; the sums are the ones dist2 forms.
-KG_HD int rt_check_from_sq(float sx_nw, float sx_sw, float sy_nw, float sy_ne, float T) {
-  const float s_nw = sx_nw + sy_nw, s_ne = sx_nw + sy_ne;
-  const float s_sw = sx_sw + sy_nw, s_se = sx_sw + sy_ne;
-  if (s_nw <= T && s_ne <= T && s_sw <= T && s_se <= T) return 1;
-  if (s_nw > T && s_ne > T && s_sw > T && s_se > T) return -1;
-  return 0;
-}
-
 // One neighbour candidate of query (qx, qy) in bag-scan order: the
-// check_circle of its cell (computed when the cell row changes, from the
-// corner terms when `pre`), then the distance test and the accumulation.
-KG_HD void rt_visit(const Geom& g, int32_t i, float T, bool pre, int32_t min_j, float* sxn,
-                    float* sxs, float syn0, float syn1, float syn2, float sye0, float sye1,
-                    float sye2, float qx, float qy, uint32_t qid, float ex_, float ey_, float ehx_,
-                    float ehy_, uint32_t eid_, int32_t ecy_, int32_t* cur, int* chk,
+// check_circle of its cell (from the window's packed checks when `packed`,
+// else computed when the cell row changes), then the distance test and the
+// accumulation.
+KG_HD void rt_visit(const Geom& g, int32_t i, float T, bool packed, uint32_t wchk, int32_t min_i,
+                    int32_t min_j, float qx, float qy, uint32_t qid, float ex_, float ey_,
+                    float ehx_, float ehy_, uint32_t eid_, int32_t ecy_, int32_t* cur, int* chk,
                     FlockAcc* acc) {
-  if (ecy_ != *cur) {
-    if (pre && *cur == -1) {
-      // the column's corner terms, at its first candidate (most columns of
-      // a sparse field have none): nw_x = i * d, sw_x = min(nw_x + d, w)
-      const float nwx = (float)i * g.d;
-      *sxn = rt_corner_sq(nwx, qx, g.w);
-      *sxs = rt_corner_sq(fminf(nwx + g.d, g.w), qx, g.w);
-    }
+  if (packed) {
+    *chk = window_check_at(wchk, i - min_i, ecy_ - min_j);
+  } else if (ecy_ != *cur) {
     *cur = ecy_;
-    if (pre) {
-      const int k = ecy_ - min_j;
-      const float yn = k == 0 ? syn0 : (k == 1 ? syn1 : syn2);
-      const float ye = k == 0 ? sye0 : (k == 1 ? sye1 : sye2);
-      *chk = rt_check_from_sq(*sxn, *sxs, yn, ye, T);
-    } else {
-      *chk = check_circle_sq(g, i, ecy_, qx, qy, T);
-    }
+    *chk = check_circle_sq(g, i, ecy_, qx, qy, T);
   }
   if (*chk == -1) return;
   const float dx = toroidal_distance(qx, ex_, g.w);
@@ -600,8 +571,8 @@ KG_HD void rt_visit(const Geom& g, int32_t i, float T, bool pre, int32_t min_j, 
 // Bird::step for own agent number o: neighbours come from the staged runs
 // (the row table gives the window rows' range of each column) merged with
 // the column's sorted extras; the new state goes to result slot o.  For
-// windows of up to three rows the corner terms of check_circle are computed
-// once per agent (rows) and once per column instead of once per cell.
+// windows of up to 3 x 3 cells every cell's check_circle is computed once per
+// agent (window_checks) instead of at each cell row the walk enters.
 KG_HD void rt_agent_lds(const RepDev& rp, const RtTile& rt, const RtSegs& sg, uint32_t o,
                         const uint32_t* col_base, RtSmem S) {
   const Geom& g = rp.g;
@@ -623,20 +594,13 @@ KG_HD void rt_agent_lds(const RepDev& rp, const RtTile& rt, const RtSegs& sg, ui
   const int32_t min_i = imax(0, cx - dd), max_i = imin(cx + dd, g.max_x - 1);
   const int32_t min_j = imax(0, cy - dd), max_j = imin(cy + dd, g.max_y - 1);
   const int32_t x_lo = rt_seg_bin(g, rt, sg, min_j), x_hi = rt_seg_bin(g, rt, sg, max_j) + 1;
-  // corner rows of the window's cells: nw_y = j * d, ne_y = min(nw_y + d, h)
-  const bool pre = max_j - min_j < 3;
-  // scalars rather than a per-lane indexed array (which would live in
-  // scratch memory)
-  float syn0 = 0.0f, syn1 = 0.0f, syn2 = 0.0f, sye0 = 0.0f, sye1 = 0.0f, sye2 = 0.0f;
-  if (pre) {
-    const float nwy0 = (float)min_j * g.d, nwy1 = (float)(min_j + 1) * g.d;
-    const float nwy2 = (float)(min_j + 2) * g.d;
-    syn0 = rt_corner_sq(nwy0, qy, g.h);
-    syn1 = rt_corner_sq(nwy1, qy, g.h);
-    syn2 = rt_corner_sq(nwy2, qy, g.h);
-    sye0 = rt_corner_sq(fminf(nwy0 + g.d, g.h), qy, g.h);
-    sye1 = rt_corner_sq(fminf(nwy1 + g.d, g.h), qy, g.h);
-    sye2 = rt_corner_sq(fminf(nwy2 + g.d, g.h), qy, g.h);
+  // windows of up to 3 x 3 cells: every cell's check_circle once per agent
+  const bool packed = max_i - min_i < 3 && max_j - min_j < 3;
+  uint32_t wchk = 0;
+  if (packed) {
+    const int32_t bxs[3] = {min_i, imin(min_i + 1, max_i), imin(min_i + 2, max_i)};
+    const int32_t bys[3] = {min_j, imin(min_j + 1, max_j), imin(min_j + 2, max_j)};
+    wchk = window_checks(g, bxs, bys, max_i - min_i + 1, max_j - min_j + 1, qx, qy, T);
   }
   FlockAcc acc;
   for (int32_t i = min_i; i <= max_i; ++i) {
@@ -659,7 +623,6 @@ KG_HD void rt_agent_lds(const RepDev& rp, const RtTile& rt, const RtSegs& sg, ui
     }
     uint32_t b = S.ecbase[c], b_end = S.ecbase[c + 1];
     while (b < b_end && S.ecy[S.eperm[b]] < min_j) ++b;
-    float sxn = 0.0f, sxs = 0.0f;  // set by rt_visit at the column's first candidate
     int32_t cur = -1;
     int chk = 0;
     if (!(b < b_end && S.ecy[S.eperm[b]] <= max_j)) {
@@ -672,11 +635,11 @@ KG_HD void rt_agent_lds(const RepDev& rp, const RtTile& rt, const RtSegs& sg, ui
         const float ex1 = S.sx[a1], ey1 = S.sy[a1], ehx1 = S.shx[a1], ehy1 = S.shy[a1];
         const uint32_t eid0 = S.sid[a], eid1 = S.sid[a1];
         if (ecy0 > max_j) break;
-        rt_visit(g, i, T, pre, min_j, &sxn, &sxs, syn0, syn1, syn2, sye0, sye1, sye2, qx, qy, qid,
-                 ex0, ey0, ehx0, ehy0, eid0, ecy0, &cur, &chk, &acc);
+        rt_visit(g, i, T, packed, wchk, min_i, min_j, qx, qy, qid, ex0, ey0, ehx0, ehy0, eid0,
+                 ecy0, &cur, &chk, &acc);
         if (a + 1 >= a_end || ecy1 > max_j) break;
-        rt_visit(g, i, T, pre, min_j, &sxn, &sxs, syn0, syn1, syn2, sye0, sye1, sye2, qx, qy, qid,
-                 ex1, ey1, ehx1, ehy1, eid1, ecy1, &cur, &chk, &acc);
+        rt_visit(g, i, T, packed, wchk, min_i, min_j, qx, qy, qid, ex1, ey1, ehx1, ehy1, eid1,
+                 ecy1, &cur, &chk, &acc);
       }
       continue;
     }
@@ -704,8 +667,8 @@ KG_HD void rt_agent_lds(const RepDev& rp, const RtTile& rt, const RtSegs& sg, ui
       const int32_t ecy_ = pcy[src];
       a += take_a ? 1u : 0u;
       b += take_a ? 0u : 1u;
-      rt_visit(g, i, T, pre, min_j, &sxn, &sxs, syn0, syn1, syn2, sye0, sye1, sye2, qx, qy, qid,
-               ex_, ey_, ehx_, ehy_, eid_, ecy_, &cur, &chk, &acc);
+      rt_visit(g, i, T, packed, wchk, min_i, min_j, qx, qy, qid, ex_, ey_, ehx_, ehy_, eid_, ecy_,
+               &cur, &chk, &acc);
     }
   }
   float nx, ny, nhx, nhy;
