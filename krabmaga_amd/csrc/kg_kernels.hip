@@ -163,11 +163,12 @@ hipError_t launch_materialize(Buf R, Buf W, uint32_t n, Buf A, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// scan: exclusive prefix over the bin counts in three launches (tile sums,
-// one-block scan of the tile sums, tile scans).  No work-group waits on
-// another, so nothing depends on inter-work-group ordering.  The last launch
-// also zeroes the counts (for the next histogram), fills the scatter cursors
-// and advances the schedule-step counter.
+// scan: exclusive prefix over the bin counts in two launches (tile sums,
+// then tile scans, each tile adding up the sums of the tiles before it).
+// No work-group waits on another, so nothing depends on inter-work-group
+// ordering.  The second launch also zeroes the counts (for the next
+// histogram), fills the scatter cursors and advances the schedule-step
+// counter.
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ count,
                                                               uint32_t n,
@@ -183,36 +184,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __
   if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
-// One block: partial[0..m) becomes its exclusive prefix, partial[m] the total.
-__global__ __launch_bounds__(1024) void k_scan_partials(uint32_t* __restrict__ partial, uint32_t m,
-                                                        uint64_t* gstep) {
-  __shared__ uint32_t s_warp[1024 / kWave];
-  __shared__ uint32_t s_carry;
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  for (uint32_t base = 0; base < m; base += 1024) {
-    const uint32_t i = base + threadIdx.x;
-    const uint32_t v = i < m ? partial[i] : 0u;
-    uint32_t total;
-    const uint32_t ex = block_exclusive_scan<1024>(v, s_warp, &total);
-    const uint32_t carry = s_carry;
-    if (i < m) partial[i] = carry + ex;
-    __syncthreads();
-    if (threadIdx.x == 0) s_carry = carry + total;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    partial[m] = s_carry;
-    if (gstep) *gstep += 1;
-  }
-}
-
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict__ count,
                                                             uint32_t* __restrict__ offset,
                                                             uint32_t* __restrict__ cursor, uint32_t n,
-                                                            const uint32_t* __restrict__ partial) {
+                                                            const uint32_t* __restrict__ partial,
+                                                            uint64_t* gstep) {
   __shared__ uint32_t s_warp[kScanThreads / kWave];
   const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  // the tiles before this one (a few thousand sums at most, cached)
+  uint32_t before = 0;
+  for (uint32_t q = threadIdx.x; q < blockIdx.x; q += kScanThreads) before += partial[q];
   uint32_t v[kScanItems];
   uint32_t sum = 0;
 #pragma unroll
@@ -223,8 +204,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k)
     if (base + k < n) count[base + k] = 0u;
+  uint32_t tile_before;
+  block_exclusive_scan<kScanThreads>(before, s_warp, &tile_before);
   uint32_t total;
-  uint32_t run = partial[blockIdx.x] + block_exclusive_scan<kScanThreads>(sum, s_warp, &total);
+  uint32_t run = tile_before + block_exclusive_scan<kScanThreads>(sum, s_warp, &total);
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     if (base + k < n) {
@@ -233,7 +216,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
     }
     run += v[k];
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) offset[n] = partial[gridDim.x];
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) offset[n] = tile_before + total;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && gstep) *gstep += 1;
 }
 
 hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint32_t nbins,
@@ -241,9 +225,8 @@ hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint
   const uint32_t ntiles = (nbins + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kScanThreads), 0, s, count, nbins,
                      ss.partial);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, ss.partial, ntiles, gstep);
   hipLaunchKernelGGL(k_scan_down, dim3(ntiles), dim3(kScanThreads), 0, s, count, offset, cursor,
-                     nbins, ss.partial);
+                     nbins, ss.partial, gstep);
   return hipGetLastError();
 }
 
