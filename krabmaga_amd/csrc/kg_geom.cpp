@@ -97,9 +97,11 @@ int pick_lds_cap(const Geom& g, uint64_t expected_agents) {
   const double cells = (double)g.dw * (double)g.dh;
   const double dens = expected_agents ? (double)expected_agents / cells : 0.1;
   const double staged = dens * (double)(g.tx + 2) * (double)(g.ty + 2 * g.hb);
-  int cap = 512;
-  while (cap < 4096 && (double)cap < staged * 1.6 + 64.0) cap *= 2;
-  return cap;
+  // room for the tile-to-tile spread of a uniform field; a tile past it
+  // takes the global path (clamp_lds_cap rounds up to what the LDS of a CU
+  // holds at the same number of work-groups)
+  const double need = std::ceil((staged * 1.3 + 64.0) / 64.0) * 64.0;
+  return (int)std::min(4096.0, std::max(256.0, need));
 }
 
 bool params_valid(const kg_flock_params& p, std::string* why) {

@@ -359,6 +359,20 @@ __device__ __forceinline__ void sort_wave_bin(const Geom& g, Buf R, uint32_t s, 
     kv[u] = e < m ? bin_key(g, yv[u], idv[u]) : ~0ull;
     rk[u] = 0;
   }
+  // already in order (the scatter keeps most bins so): nothing to write.
+  // Entry e is out of order if its predecessor e - 1 has a larger key; the
+  // predecessor of slot u's entry is lane - 1 of slot u, or lane 63 of slot
+  // u - 1 for lane 0.
+  bool inv = false;
+#pragma unroll
+  for (int u = 0; u < kSortSlots; ++u) {
+    const uint32_t e = lane + (uint32_t)u * kWave;
+    const uint64_t up = __shfl_up(kv[u], 1, kWave);
+    const uint64_t wrap = u > 0 ? readlane64(kv[u > 0 ? u - 1 : 0], kWave - 1) : 0ull;
+    const uint64_t prev = lane == 0 ? wrap : up;
+    if (e > 0 && e < m && prev > kv[u]) inv = true;
+  }
+  if (!__any(inv)) return;
   for (uint32_t j = 0; j < m; ++j) {
     const uint32_t ju = j / kWave;  // the same on every lane
     const uint64_t src = ju == 0 ? kv[0] : (ju == 1 ? kv[1] : (ju == 2 ? kv[2] : kv[3]));
