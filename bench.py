@@ -7,7 +7,10 @@ jump 0.7, positions uniform and headings unit vectors at a uniform angle
 from the seeded Philox stream.  One step = one Schedule::step of the
 flockers model: neighbour query + Bird::step for every agent and the
 rebinning of the field (lazy_update).  State is resident in HBM before the
-timed region starts.
+timed region starts.  The headline runs the field's default path (at this
+size the three-kernel path: k_flock, the bin-count scan, k_scatter_aligned,
+k_sort_bins); the tile-resident step (one k_rt_step launch per tile class)
+is timed beside it from the same initial state.
 
 Beside it, in the same JSON line: the clustered-density workload (8,388,608
 boids, 64 Gaussian blobs), the batched ensemble (512 replicas of uneven
@@ -103,22 +106,22 @@ def plain_workload():
     return dict(n=1 << 20, side=20480.0, d=10.0 / 1.5, radius=10.0, seed=1)
 
 
-def timed_run(hip, f, p, first, steps, binned):
+def timed_run(hip, f, p, first, steps, binned=False, tile_resident=False):
     e0, e1 = hip.event(), hip.event()
     hip.sync()
     t0 = time.perf_counter()
     hip.record(e0, f.stream)
-    f.schedule_run(p, first, steps, binned=binned)
+    f.schedule_run(p, first, steps, binned=binned, tile_resident=tile_resident)
     hip.record(e1, f.stream)
     ms = hip.elapsed_ms(e0, e1)
     hip.sync()
     return ms, time.perf_counter() - t0
 
 
-def profiled_traffic(name="k_rt_step_flockers_1M.md"):
-    """HBM bytes per schedule step of the tile-resident step (all its
-    k_rt_step launches), from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (calibrated against a 1 GiB copy), or None."""
+def profiled_traffic(name):
+    """HBM bytes per schedule step (all kernels of the step), from a committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a 1 GiB
+    copy), or None."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
@@ -129,45 +132,43 @@ def profiled_traffic(name="k_rt_step_flockers_1M.md"):
     return None, None
 
 
+PLAIN_KERNELS = {"default": ("flock", "scan", "scatter", "sort_bins"),
+                 "tile_resident": ("rt_step",)}
+
+
 def run_plain(args, hip):
+    """The 1M-boid field on its default path (the three-kernel path at this
+    size) and, from the same initial state, on the tile-resident step: each
+    warmed up, then timed over the same window; per-kernel durations over a
+    few steps launched one by one after it."""
     from krabmaga_amd import Field2D, FlockParams
     w = plain_workload()
     n, side = w["n"], w["side"]
     p = FlockParams(radius=w["radius"], seed=w["seed"])
-    f = Field2D(side, side, w["d"], True, n)
-    f.init_uniform(n, w["seed"], 1)
-    f.schedule_run(p, 0, args.warmup)  # step 0 includes the first lazy_update
-    f.sync()
-    log(f"plain warmup done ({args.warmup} steps)")
-    ms, wall = timed_run(hip, f, p, args.warmup, args.steps, binned=False)
-    f.sync()
-    layout = f.stats()
-    log(f"tile-resident timed region: {ms:.3f} ms for {args.steps} steps; layout {layout}")
-
-    # per-kernel durations: steps launched one by one, each launch bracketed
-    # by HIP events on the field's stream
-    first = args.warmup + args.steps
-    kprof = min(args.steps, args.profile_steps)
-    f.profile(True)
-    f.schedule_run(p, first, kprof, use_graph=False)
-    rt_ms, rt_n = f.profile_read("rt_step")
-    f.profile(False)
-    first += kprof
-
-    # the three-kernel path on the same state, for comparison
-    nb = min(args.steps, args.binned_steps)
-    bms, _ = timed_run(hip, f, p, first, nb, binned=True)
-    first += nb
-    f.profile(True)
-    f.schedule_run(p, first, kprof, use_graph=False, binned=True)
-    kern = {"rt_step": {"avg_us": 1e3 * rt_ms / max(rt_n, 1), "launches": rt_n}}
-    for k in ("flock", "scan", "scatter"):
-        tot, cnt = f.profile_read(k)
-        kern[k] = {"avg_us": 1e3 * tot / max(cnt, 1), "launches": cnt}
-    f.profile(False)
-    f.close()
-    log(f"three-kernel path: {bms / nb:.4f} ms/step")
-    return w, ms, wall, kern, bms / nb, layout
+    out = {}
+    for path in ("default", "tile_resident"):
+        kw = {"tile_resident": path == "tile_resident"}
+        f = Field2D(side, side, w["d"], True, n)
+        f.init_uniform(n, w["seed"], 1)
+        f.schedule_run(p, 0, args.warmup, **kw)  # step 0 includes the first lazy_update
+        f.sync()
+        ms, wall = timed_run(hip, f, p, args.warmup, args.steps, **kw)
+        f.sync()
+        layout = f.stats()
+        log(f"{path} path: {ms / args.steps:.4f} ms/step over {args.steps} steps; "
+            f"layout {layout}")
+        kprof = min(args.steps, args.profile_steps)
+        f.profile(True)
+        f.schedule_run(p, args.warmup + args.steps, kprof, use_graph=False, **kw)
+        kern = {}
+        for k in PLAIN_KERNELS[path]:
+            tot, cnt = f.profile_read(k)
+            kern[k] = {"avg_us": 1e3 * tot / max(cnt, 1), "launches": cnt,
+                       "us_per_step": 1e3 * tot / kprof}
+        f.profile(False)
+        f.close()
+        out[path] = {"ms": ms, "wall": wall, "layout": layout, "kern": kern}
+    return w, out
 
 
 def cpu_baseline(args):
@@ -307,7 +308,7 @@ def run_clustered(args, hip):
     f.schedule_run(p, 0, warm)
     f.sync()
     steps = args.clustered_steps
-    ms, _ = timed_run(hip, f, p, warm, steps, binned=False)
+    ms, _ = timed_run(hip, f, p, warm, steps)
     st = f.stats()
     # per-kernel time over a few steps launched one by one
     f.profile(True)
@@ -441,7 +442,6 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=100)
-    ap.add_argument("--binned-steps", type=int, default=100)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -473,25 +473,25 @@ def main():
         sys.exit(2)
 
     hip = Hip()
-    w, ms, wall, kern, binned_ms, layout = run_plain(args, hip)
+    w, plain = run_plain(args, hip)
     if args.calibrate_copy:
         hip.copy_calibration(1 << 30)
     n, steps = w["n"], args.steps
+    d, rt = plain["default"], plain["tile_resident"]
+    ms = d["ms"]
     value = n * steps / (ms * 1e-3)
-    # dominant kernel of the timed path: the tile-resident step.  One
-    # schedule step is its class launches side by side plus the launch for
-    # deferred tiles, so a "launch" here is one schedule step and its
-    # duration is the timed region's time per step (HIP events on the
-    # field's stream, which forks to and joins the class streams).  Falls
-    # back to the fused flock kernel when the field ran the three-kernel path.
-    if kern["rt_step"]["launches"]:
-        dom, kernel_bytes = "rt_step", BYTES_RT
-        dom_us = 1e3 * ms / steps
-    else:
-        dom, kernel_bytes = "flock", BYTES_FLOCK
-        dom_us = kern[dom]["avg_us"]
-    traffic, traffic_src = profiled_traffic()
-    achieved = BYTES_STEP * n / (dom_us * 1e-6) if dom_us > 0 else 0.0
+    three = d["layout"]["rt_steps"] == 0
+    # The roofline prices the whole schedule step at the 76 algorithmic bytes
+    # per agent-update: a "launch" is one schedule step -- on the default
+    # (three-kernel) path its five kernels back to back on the field's
+    # stream -- and its duration is the timed region's time per step (HIP
+    # events on that stream).  The dominant kernel's own figure goes beside.
+    step_us = 1e3 * ms / steps
+    achieved = BYTES_STEP * n / (step_us * 1e-6)
+    dom, dom_bytes = ("flock", BYTES_FLOCK) if three else ("rt_step", BYTES_RT)
+    dom_us = d["kern"][dom]["avg_us"]
+    prof = "flockers_1M_step.md" if three else "k_rt_step_flockers_1M.md"
+    traffic, traffic_src = profiled_traffic(prof)
     out = {
         "metric": METRIC,
         "value": value,
@@ -509,34 +509,43 @@ def main():
             "workload": "flockers plain GPU drop-in: 1,048,576 boids, 20,480 x 20,480 torus, "
                         "discretization 10/1.5, radius 10, weights 1, jump 0.7",
             "agents": n, "side": w["side"], "seed": w["seed"],
+            "path": "three-kernel (k_flock, scan, k_scatter_aligned, k_sort_bins), the "
+                    "field's default at this size" if three else "tile-resident (k_rt_step)",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_" + dom,
-            "launch_us": dom_us,
-            "unit_of_launch": "one schedule step (all k_rt_step launches of the step)"
-                              if dom == "rt_step" else "one launch",
+            "kernel": ("schedule step: k_flock + k_scan_reduce + k_scan_down + "
+                       "k_scatter_aligned + k_sort_bins" if three else
+                       "schedule step: all k_rt_step launches of the step"),
+            "launch_us": step_us,
+            "unit_of_launch": "one schedule step",
             "bytes_per_agent": BYTES_STEP,
             "achieved": achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": traffic if dom == "rt_step" else None,
-            "traffic_source": traffic_src if dom == "rt_step" else None,
-            "kernel_bytes_per_agent": kernel_bytes,
-            "kernel_frac": kernel_bytes * n / (dom_us * 1e-6) / HBM_PEAK if dom_us > 0 else 0.0,
-            "limiter": "latency: the step is HBM-bound by its algorithm, but each work-group "
-                       "runs a chain of dependent LDS and global round trips between barriers, "
-                       "so the launch runs far below the HBM roofline",
+            "traffic": traffic,
+            "traffic_source": traffic_src,
+            "dominant_kernel": "k_" + dom,
+            "dominant_kernel_us": dom_us,
+            "dominant_kernel_bytes_per_agent": dom_bytes,
+            "dominant_kernel_frac": dom_bytes * n / (dom_us * 1e-6) / HBM_PEAK if dom_us else 0.0,
+            "limiter": "issue and latency: the step moves its bytes once, but the neighbour "
+                       "walk and Bird::step are VALU work per bird, and every kernel runs short "
+                       "dependent chains of global and LDS round trips",
         },
-        "kernels_us": {k: round(v["avg_us"], 3) for k, v in kern.items()},
-        "three_kernel_path": {
-            "ms_per_step": binned_ms,
-            "agent_updates_per_s": n / (binned_ms * 1e-3),
+        "kernels_us": {k: round(v["avg_us"], 3) for k, v in d["kern"].items()},
+        "kernels_us_per_step": {k: round(v["us_per_step"], 3) for k, v in d["kern"].items()},
+        "tile_resident_path": {
+            "ms_per_step": rt["ms"] / steps,
+            "agent_updates_per_s": n * steps / (rt["ms"] * 1e-3),
+            "k_rt_step_us_per_step": round(rt["kern"]["rt_step"]["us_per_step"], 3),
+            "layout": {k: rt["layout"][k] for k in ("rt_active", "rt_plans", "rt_flags", "tiles",
+                                                    "dense_tiles", "lds_bytes", "rt_steps")},
         },
-        "wall_s_timed_region": wall,
-        "layout": {k: layout[k] for k in ("rt_active", "rt_plans", "rt_flags", "tiles",
-                                          "dense_tiles", "lds_bytes", "rt_steps")},
+        "wall_s_timed_region": d["wall"],
+        "layout": {k: d["layout"][k] for k in ("tiles", "tile_cols", "tile_rows", "bin_rows",
+                                               "rt_steps")},
     }
     if not args.no_cpu:
         log("CPU baseline (oracle, 1 thread) ...")

@@ -168,13 +168,18 @@ int kg_flock_step(kg_field *f, const kg_flock_params *p, uint64_t step);
  * each = [lazy_update if the step is 0] + kg_flock_step + lazy_update
  * (schedule.rs:349-351, 395-412).  Asynchronous: call kg_field_sync to wait.
  *
- * Toroidal fields whose window (floor(radius/d) cells) fits one tile of
- * halo run each whole step as ONE kernel launch on a tile-resident layout;
- * other fields run flock + scan + scatter.  Results are identical.  `flags`:
- * KG_RUN_GRAPH replays launches from HIP graphs; KG_RUN_BINNED forces the
- * three-kernel path. */
+ * Two device paths give identical results: the three-kernel path (flock +
+ * scan + scatter and bin sort) and, for toroidal fields whose window
+ * (floor(radius/d) cells) fits one tile of halo, a tile-resident layout that
+ * runs each whole step as ONE kernel launch.  By default fields of fewer
+ * than 32,768 agents take the tile-resident step and larger ones the
+ * three-kernel path (the faster of the two at each size on MI355X).
+ * `flags`: KG_RUN_GRAPH replays launches from HIP graphs; KG_RUN_BINNED
+ * forces the three-kernel path; KG_RUN_TILE_RESIDENT the tile-resident step
+ * where the field suits it (the two exclude each other). */
 #define KG_RUN_GRAPH 1
 #define KG_RUN_BINNED 2
+#define KG_RUN_TILE_RESIDENT 4
 int kg_schedule_run(kg_field *f, const kg_flock_params *p, uint64_t first_step, uint32_t nsteps,
                     int flags);
 

@@ -888,10 +888,19 @@ int kg_schedule_run(kg_field* f, const kg_flock_params* p, uint64_t first_step, 
   }
   if (f->nA) return set_error(KG_E_STATE, "the write buffer must be empty");
   if (!p) return set_error(KG_E_INVALID, "params is NULL");
+  if ((use_graph & KG_RUN_BINNED) && (use_graph & KG_RUN_TILE_RESIDENT))
+    return set_error(KG_E_INVALID, "KG_RUN_BINNED and KG_RUN_TILE_RESIDENT exclude each other");
   {
     RepDev probe = f->h_rep;
     set_rep_params(&probe, *p);
-    if (!(use_graph & KG_RUN_BINNED) && probe.lds_ok) {
+    // by default small fields take the one-launch tile-resident step (fewer
+    // launches per step) and large ones the three-kernel path (more of its
+    // work-groups fit a CU): measured at constant density on MI355X, the two
+    // cross between 16k and 64k birds
+    constexpr uint32_t kRtDefaultMaxAgents = 32768;
+    const bool want_rt = (use_graph & KG_RUN_TILE_RESIDENT) ||
+                         (!(use_graph & KG_RUN_BINNED) && f->nR < kRtDefaultMaxAgents);
+    if (want_rt && probe.lds_ok) {
       // tile-resident path: the class launches of each step; the capacity
       // flags are read every kRtCheckEvery steps (32 measured ~1.5 % faster
       // than 64 at 1M boids: deferrals are caught sooner) and the layout

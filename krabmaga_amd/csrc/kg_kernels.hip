@@ -236,9 +236,11 @@ hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint
 
 size_t flock_lds_bytes(int cap) { return flock_smem_bytes(cap); }
 
-// Tiles whose staged set exceeds the LDS capacity only set their flag in
-// `gflag`; k_flock_global then steps their birds one thread per bird, so a
-// crowded tile is spread over the whole GPU instead of one work-group.
+// A tile whose staged set exceeds the LDS capacity runs in chunks of own
+// columns (each with its halo) one after another; a tile with an own column
+// whose staged items alone exceed it only sets its flag in `gflag`, and
+// k_flock_global then steps its birds one thread per bird, so a crowded tile
+// is spread over the whole GPU instead of one work-group.
 __global__ __launch_bounds__(kFlockThreads) void k_flock(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
     const uint32_t* __restrict__ offset, Buf W, uint32_t* __restrict__ count_next,
@@ -269,20 +271,42 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
       s_nown = 0;
     }
     L = total;
-    use_lds = total <= (uint32_t)lds_cap;
     __syncthreads();
+  }
+  const int32_t own_lo = ft.c0 - ft.sc_lo, own_hi = ft.c1 - ft.sc_lo;
+  if (use_lds && L > (uint32_t)lds_cap) {
+    // chunks need every own column to fit on its own (the same on all
+    // lanes); a tile of many chunks is better spread over the GPU
+    use_lds = L <= kFlockChunkFactor * (uint32_t)lds_cap;
+    for (int32_t c = own_lo; use_lds && c < own_hi; ++c)
+      if (flock_chunk_size(rp, ft, s_col_base, c, c + 1) > (uint32_t)lds_cap) use_lds = false;
   }
   if (t == 0) gflag[item] = use_lds ? 0u : 1u;
   if (use_lds) {
     const FlockSmem S = flock_carve(smem_raw, lds_cap);
-    for (uint32_t k = t; k < L; k += kFlockThreads) {
-      flock_stage(rp, ft, k, R, s_seg_start, s_seg_len, s_col_base, S);
-      flock_own(ft, k, S, &s_nown);
+    for (int32_t oa = own_lo; oa < own_hi;) {
+      const int32_t ob = L <= (uint32_t)lds_cap
+                             ? own_hi
+                             : flock_chunk_end(rp, ft, s_col_base, oa, own_hi, (uint32_t)lds_cap);
+      int32_t sa, sb;
+      flock_chunk_cols(rp, ft, oa, ob, &sa, &sb);
+      const uint32_t base = s_col_base[sa], n = s_col_base[sb] - base;
+      for (uint32_t k = t; k < n; k += kFlockThreads) {
+        flock_stage(rp, ft, base + k, k, R, s_seg_start, s_seg_len, s_col_base, S);
+        flock_own(ft, k, oa, ob, S, &s_nown);
+      }
+      __syncthreads();
+      const uint32_t nown = s_nown;
+      for (uint32_t o = t; o < nown; o += kFlockThreads)
+        flock_agent_lds(rp, ft, S.own[o], s_col_base, base, S, W, count_next, err);
+      oa = ob;
+      if (oa < own_hi) {
+        // the next chunk reuses the LDS
+        __syncthreads();
+        if (t == 0) s_nown = 0;
+        __syncthreads();
+      }
     }
-    __syncthreads();
-    const uint32_t nown = s_nown;
-    for (uint32_t o = t; o < nown; o += kFlockThreads)
-      flock_agent_lds(rp, ft, S.own[o], s_col_base, S, W, count_next, err);
   }
 }
 
@@ -348,13 +372,10 @@ __device__ __forceinline__ void sort_wave_bin(const Geom& g, Buf R, uint32_t s, 
   for (int u = 0; u < kSortSlots; ++u) {
     const uint32_t e = lane + (uint32_t)u * kWave;
     idv[u] = 0;
-    xv[u] = yv[u] = hxv[u] = hyv[u] = 0.0f;
+    yv[u] = 0.0f;
     if (e < m) {
       idv[u] = R.id[s + e];
-      xv[u] = R.x[s + e];
       yv[u] = R.y[s + e];
-      hxv[u] = R.hx[s + e];
-      hyv[u] = R.hy[s + e];
     }
     kv[u] = e < m ? bin_key(g, yv[u], idv[u]) : ~0ull;
     rk[u] = 0;
@@ -373,6 +394,16 @@ __device__ __forceinline__ void sort_wave_bin(const Geom& g, Buf R, uint32_t s, 
     if (e > 0 && e < m && prev > kv[u]) inv = true;
   }
   if (!__any(inv)) return;
+#pragma unroll
+  for (int u = 0; u < kSortSlots; ++u) {
+    const uint32_t e = lane + (uint32_t)u * kWave;
+    xv[u] = hxv[u] = hyv[u] = 0.0f;
+    if (e < m) {
+      xv[u] = R.x[s + e];
+      hxv[u] = R.hx[s + e];
+      hyv[u] = R.hy[s + e];
+    }
+  }
   for (uint32_t j = 0; j < m; ++j) {
     const uint32_t ju = j / kWave;  // the same on every lane
     const uint64_t src = ju == 0 ? kv[0] : (ju == 1 ? kv[1] : (ju == 2 ? kv[2] : kv[3]));

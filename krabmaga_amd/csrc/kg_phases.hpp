@@ -173,8 +173,12 @@ KG_HD void sort_small_bin(const Geom& g, Buf R, uint32_t s, uint32_t m) {
 // further): the tile's own bins plus the halo cells the own agents' windows
 // reach are staged into LDS -- bins in row order, so each staged column is in
 // (cell y, id) order, the reference's bag-scan order -- and every own agent
-// scans its window from LDS.  Tiles whose staged set exceeds the LDS capacity, and fields outside
-// the LDS path's reach, take the global path (query_global per agent).
+// scans its window from LDS.  A tile whose staged set exceeds the LDS
+// capacity, up to kFlockChunkFactor times it, is stepped in chunks of
+// consecutive own columns, each staged with its own halo columns
+// (flock_chunk_end).  Larger tiles, tiles with a single own column past the
+// capacity, and fields outside the LDS path's reach take the global path
+// (query_global per agent).
 
 struct FlockTile {
   int r;
@@ -272,8 +276,41 @@ KG_HD FlockSmem flock_carve(uint8_t* p, int cap) {
   return s;
 }
 
-// Stages item k (< total staged) into LDS.
-KG_HD void flock_stage(const RepDev& rp, const FlockTile& ft, uint32_t k, Buf R,
+// A tile staged in chunks holds at most this many capacities' worth of
+// staged items; a larger one takes the global path, where its birds spread
+// over the whole GPU instead of running chunk after chunk in one work-group.
+constexpr uint32_t kFlockChunkFactor = 2;
+
+// Staged columns [*sa, *sb) (relative to sc_lo) that own staged columns
+// [oa, ob) need: the own columns plus the columns their clamped windows reach.
+KG_HD void flock_chunk_cols(const RepDev& rp, const FlockTile& ft, int32_t oa, int32_t ob,
+                            int32_t* sa, int32_t* sb) {
+  const int32_t dd = rp.disc_dist;
+  const int32_t a = ft.sc_lo + oa, b = ft.sc_lo + ob;
+  *sa = imax(0, a - dd) - ft.sc_lo;
+  *sb = imax(b - 1, imin(rp.g.max_x - 1, b - 1 + dd)) - ft.sc_lo + 1;
+}
+
+// Staged items that own columns [oa, ob) need.
+KG_HD uint32_t flock_chunk_size(const RepDev& rp, const FlockTile& ft, const uint32_t* col_base,
+                                int32_t oa, int32_t ob) {
+  int32_t sa, sb;
+  flock_chunk_cols(rp, ft, oa, ob, &sa, &sb);
+  return col_base[sb] - col_base[sa];
+}
+
+// End of the chunk of own columns that starts at oa: the most columns (up to
+// ob_max) whose staged items fit cap; oa + 1 at least (the caller has checked
+// that every single column fits).
+KG_HD int32_t flock_chunk_end(const RepDev& rp, const FlockTile& ft, const uint32_t* col_base,
+                              int32_t oa, int32_t ob_max, uint32_t cap) {
+  int32_t ob = oa + 1;
+  while (ob < ob_max && flock_chunk_size(rp, ft, col_base, oa, ob + 1) <= cap) ++ob;
+  return ob;
+}
+
+// Stages item k (< total staged) into LDS slot dst.
+KG_HD void flock_stage(const RepDev& rp, const FlockTile& ft, uint32_t k, uint32_t dst, Buf R,
                        const uint32_t* seg_start, const uint32_t* seg_len,
                        const uint32_t* col_base, FlockSmem S) {
   int lo = 0, hi = ft.ncols;  // column c with col_base[c] <= k < col_base[c+1]
@@ -290,21 +327,23 @@ KG_HD void flock_stage(const RepDev& rp, const FlockTile& ft, uint32_t k, Buf R,
   }
   const uint32_t gi = seg_start[3 * lo + seg] + off;
   const float y = R.y[gi];
-  S.x[k] = R.x[gi];
-  S.y[k] = y;
-  S.hx[k] = R.hx[gi];
-  S.hy[k] = R.hy[gi];
-  S.id[k] = R.id[gi];
-  S.gi[k] = gi;
-  S.cy[k] = disc_axis(y, rp.g.d, rp.g.inv_d);
-  S.col[k] = (uint16_t)lo;
+  S.x[dst] = R.x[gi];
+  S.y[dst] = y;
+  S.hx[dst] = R.hx[gi];
+  S.hy[dst] = R.hy[gi];
+  S.id[dst] = R.id[gi];
+  S.gi[dst] = gi;
+  S.cy[dst] = disc_axis(y, rp.g.d, rp.g.inv_d);
+  S.col[dst] = (uint16_t)lo;
 }
 
-// Appends staged item k to the own list if it is one of the tile's agents.
-KG_HD void flock_own(const FlockTile& ft, uint32_t k, FlockSmem S, uint32_t* nown) {
+// Appends LDS slot k to the own list if it holds one of the tile's agents in
+// own staged columns [oa, ob).
+KG_HD void flock_own(const FlockTile& ft, uint32_t k, int32_t oa, int32_t ob, FlockSmem S,
+                     uint32_t* nown) {
   const int32_t ky = S.cy[k];
-  const int32_t cx = ft.sc_lo + S.col[k];
-  if (cx >= ft.c0 && cx < ft.c1 && ky >= ft.y0 && ky < ft.y1) {
+  const int32_t c = S.col[k];
+  if (c >= oa && c < ob && ky >= ft.y0 && ky < ft.y1) {
     const uint32_t o = atom_add(nown, 1u);
     S.own[o] = (uint16_t)k;
   }
@@ -323,10 +362,11 @@ KG_HD void flock_emit(const Geom& g, uint32_t bin_base, Buf W, uint32_t gi, floa
   else atom_or(err, 1u);
 }
 
-// Bird::step for own agent k of the staged tile, neighbours from LDS.
+// Bird::step for own agent k of the staged tile, neighbours from LDS; LDS
+// slot j holds staged item j + base.
 KG_HD void flock_agent_lds(const RepDev& rp, const FlockTile& ft, uint32_t k,
-                           const uint32_t* col_base, FlockSmem S, Buf W, uint32_t* count_next,
-                           uint32_t* err) {
+                           const uint32_t* col_base, uint32_t base, FlockSmem S, Buf W,
+                           uint32_t* count_next, uint32_t* err) {
   const Geom& g = rp.g;
   const float T = rp.sq_thresh;
   const int32_t dd = rp.disc_dist;
@@ -348,13 +388,13 @@ KG_HD void flock_agent_lds(const RepDev& rp, const FlockTile& ft, uint32_t k,
   FlockAcc acc;
   for (int32_t i = min_i; i <= max_i; ++i) {
     const int c = i - ft.sc_lo;
-    uint32_t lo = col_base[c], hi = col_base[c + 1];
+    uint32_t lo = col_base[c] - base, hi = col_base[c + 1] - base;
     while (lo < hi) {  // first slot with cell y >= min_j
       const uint32_t mid = (lo + hi) >> 1;
       if (S.cy[mid] < min_j) lo = mid + 1;
       else hi = mid;
     }
-    const uint32_t end = col_base[c + 1];
+    const uint32_t end = col_base[c + 1] - base;
     int32_t cur = -1;
     int chk = 0;
     for (uint32_t e = lo; e < end; ++e) {

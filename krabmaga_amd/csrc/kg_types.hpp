@@ -41,7 +41,7 @@ struct ScanState {
 };
 
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
+constexpr int kScanItems = 2;
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr int kFlockThreads = 256;
 constexpr int kMaxStageCols = 136;  // tx <= 128 plus a halo of up to 4 columns

@@ -194,13 +194,16 @@ class Field2D:
         p = params.c()
         check(self._L.kg_flock_step(self._h, ctypes.byref(p), step))
 
-    def schedule_run(self, params, first_step, nsteps, use_graph=True, binned=False):
+    def schedule_run(self, params, first_step, nsteps, use_graph=True, binned=False,
+                     tile_resident=False):
         """`nsteps` Schedule::step calls from `first_step` (simulate!'s loop).
 
-        binned=True forces the three-kernel bin-grouped path instead of the
-        one-launch tile-resident step (same results)."""
+        By default fields of fewer than 32,768 agents take the one-launch
+        tile-resident step and larger ones the three-kernel path; binned=True
+        forces the three-kernel path, tile_resident=True the tile-resident
+        step where the field suits it (same results either way)."""
         p = params.c()
-        flags = (1 if use_graph else 0) | (2 if binned else 0)
+        flags = (1 if use_graph else 0) | (2 if binned else 0) | (4 if tile_resident else 0)
         check(self._L.kg_schedule_run(self._h, ctypes.byref(p), first_step, nsteps, flags))
 
     def read_agents(self, buffer=0):

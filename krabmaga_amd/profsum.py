@@ -1,4 +1,11 @@
-"""Summaries of rocprofv3 output for the tile-resident step.
+"""Summaries of rocprofv3 output for the flockers schedule step.
+
+Two device paths run a step.  The three-kernel path (the default for large
+fields) launches k_flock (plus k_flock_global for crowded tiles),
+k_scan_reduce, k_scan_down, k_scatter_aligned and k_sort_bins once each per
+step, one after another on one stream: with --step-kernel kg::k_flock a
+step runs from one k_flock launch to the next, and --fetch / --write sum
+those kernels' bytes per step.  The tile-resident path is described next.
 
 One schedule step of the tile-resident path is several launches of
 k_rt_step: one per tile class (side by side on their own streams) and one
@@ -27,9 +34,14 @@ import csv
 import statistics
 
 KERNEL = "kg::k_rt_step"
+FLOCK = "kg::k_flock"
+# the kernels of one three-kernel schedule step
+STEP_KERNELS = ("kg::k_flock", "kg::k_flock_global", "kg::k_scan_reduce", "kg::k_scan_down",
+                "kg::k_scatter_aligned", "kg::k_sort_bins")
 COPY = "__amd_rocclr_copyBuffer"
 BYTES_PER_AGENT = 76  # per agent-update of a whole schedule step
 KERNEL_BYTES_PER_AGENT = 40  # the tile-resident kernel: read 20 bytes, write 20 back
+FLOCK_BYTES_PER_AGENT = 36  # k_flock: read 20 bytes, write 16 (position, heading)
 
 
 def _name(r):
@@ -89,8 +101,26 @@ def pmc_totals(path, counter):
     return tot, n
 
 
+def three_kernel_steps(trace, step_kernel, grid=0):
+    """Launch groups of the three-kernel path: each launch of `step_kernel`
+    (of `grid` threads, if given) starts a step that runs to the next one."""
+    starts = [i for i, r in enumerate(trace)
+              if r["name"] == step_kernel and (not grid or r["grid"] == grid)]
+    groups = []
+    for a, b in zip(starts, starts[1:] + [len(trace)]):
+        groups.append([r for r in trace[a:b] if _of_three_kernel_step(r["name"])])
+    return groups
+
+
+def _of_three_kernel_step(name):
+    # the field's own kernels, not those of a tile-resident field in the same run
+    return name.startswith("kg::") and not name.startswith("kg::k_rt_")
+
+
 def summarize(a):
     tr = load_trace(a.trace)
+    three = bool(getattr(a, "step_kernel", None))
+    kname = a.step_kernel if three else KERNEL
     lines = [f"# {a.title}\n", "Commands (repository root, MI355X, ROCm 7.2):\n", "```"]
     lines.append(f"rocprofv3 --kernel-trace --stats --output-format csv -d <dir> -o trace -- {a.cmd}")
     if a.fetch:
@@ -115,14 +145,17 @@ def summarize(a):
         lines.append(f"| {k[0]} | {k[1]} | {k[2]} | {len(v)} | {statistics.mean(v):.1f} | "
                      f"{min(v):.1f} | {max(v):.1f} |")
     # per-step spans
-    groups = steps_of(tr)
+    groups = (three_kernel_steps(tr, a.step_kernel, getattr(a, "step_grid", 0) or 0) if three
+              else steps_of(tr))
     sel = groups[a.skip:a.skip + a.count] if a.count else groups[a.skip:]
     spans = [(max(r["end"] for r in g) - min(r["start"] for r in g)) / 1000.0 for g in sel]
     per_step = None
     if spans:
         per_step = statistics.mean(spans)
         nl = statistics.mean(len(g) for g in sel)
-        lines.append(f"\nk_rt_step per schedule step (launches of one step grouped; steps "
+        what = ("three-kernel step (launches from one k_flock to the next)" if three
+                else "k_rt_step per schedule step (launches of one step grouped")
+        lines.append(f"\n{what}; steps "
                      f"{a.skip}..{a.skip + len(sel) - 1} of {len(groups)} in the trace):\n")
         lines.append(f"- launches per step: {nl:.2f}")
         lines.append(f"- step span (first start to last end): mean {per_step:.1f} us, "
@@ -132,10 +165,20 @@ def summarize(a):
         lines.append(f"- the steps' region, first start to last end, per step: {region / len(sel):.1f} us "
                      f"(with the gaps between steps: flag reads, re-plans)")
         alg = BYTES_PER_AGENT * a.agents
+        own = (f"" if three else
+               f" (the kernel's own {KERNEL_BYTES_PER_AGENT} B: "
+               f"{KERNEL_BYTES_PER_AGENT * a.agents / per_step / 1e3 / 8000 * 100:.2f} %)")
         lines.append(f"- algorithmic bytes: {BYTES_PER_AGENT} B x {a.agents} agents = {alg / 1e6:.2f} MB "
                      f"-> {alg / per_step / 1e3:.1f} GB/s = {alg / per_step / 1e3 / 8000 * 100:.2f} % "
-                     f"of 8 TB/s (the kernel's own {KERNEL_BYTES_PER_AGENT} B: "
-                     f"{KERNEL_BYTES_PER_AGENT * a.agents / per_step / 1e3 / 8000 * 100:.2f} %)")
+                     f"of 8 TB/s{own}")
+        if three:
+            fl = [(r["end"] - r["start"]) / 1000.0 for g in sel for r in g if r["name"] == FLOCK]
+            if fl:
+                fb = FLOCK_BYTES_PER_AGENT * a.agents
+                lines.append(f"- k_flock: mean {statistics.mean(fl):.1f} us per launch; its own "
+                             f"{FLOCK_BYTES_PER_AGENT} B x {a.agents} agents -> "
+                             f"{fb / statistics.mean(fl) / 1e3:.1f} GB/s")
+                lines.append(f"flock_us: {statistics.mean(fl):.1f}")
         if len(groups) > len(sel):
             allspans = [(max(r["end"] for r in g) - min(r["start"] for r in g)) / 1000.0
                         for g in groups[1:]]
@@ -161,22 +204,37 @@ def summarize(a):
         lines.append(f"\nCalibration: the 1 GiB copy reads FETCH_SIZE = {f[cf[0]] / fn[cf[0]]:.0f} KiB and "
                      f"WRITE_SIZE = {w[cw[0]] / wn[cw[0]]:.0f} KiB, so bytes = FETCH_SIZE x {kf:.3f} x 1024 "
                      f"and WRITE_SIZE x {kw:.3f} x 1024.\n")
-        # per step: all k_rt_step dispatches over the steps of the trace (the
-        # counter passes run the same command)
-        ktot_f = sum(v for k, v in f.items() if k[0] == KERNEL)
-        ktot_w = sum(v for k, v in w.items() if k[0] == KERNEL)
-        nsteps_f = len(groups)
-        fetch = ktot_f / nsteps_f * 1024 * kf
-        write = ktot_w / nsteps_f * 1024 * kw
         alg = BYTES_PER_AGENT * a.agents
-        lines.append(f"k_rt_step HBM traffic per schedule step (all launches of the step, "
-                     f"{nsteps_f} steps): fetch {fetch / 1e6:.2f} MB + write {write / 1e6:.2f} MB = "
+        if three:
+            # one dispatch of each step kernel per step: the mean per dispatch
+            # of each, at the grid it ran most often (the profiled field's)
+            fetch = write = 0.0
+            for kn in STEP_KERNELS:
+                grids = [k for k in fn if k[0] == kn]
+                if not grids:
+                    continue
+                k = max(grids, key=lambda k: fn[k])
+                fetch += f[k] / fn[k] * 1024 * kf
+                if k in w:
+                    write += w[k] / wn[k] * 1024 * kw
+            what = "three-kernel step (one dispatch of each of its kernels)"
+        else:
+            # per step: all k_rt_step dispatches over the steps of the trace
+            # (the counter passes run the same command)
+            ktot_f = sum(v for k, v in f.items() if k[0] == KERNEL)
+            ktot_w = sum(v for k, v in w.items() if k[0] == KERNEL)
+            nsteps_f = len(groups)
+            fetch = ktot_f / nsteps_f * 1024 * kf
+            write = ktot_w / nsteps_f * 1024 * kw
+            what = f"k_rt_step, all launches of the step, {nsteps_f} steps"
+        lines.append(f"HBM traffic per schedule step ({what}): fetch {fetch / 1e6:.2f} MB + "
+                     f"write {write / 1e6:.2f} MB = "
                      f"{(fetch + write) / 1e6:.2f} MB ({(fetch + write) / alg:.2f}x the algorithmic "
                      f"{alg / 1e6:.2f} MB)")
         lines.append(f"traffic_bytes_per_step: {int(fetch + write)}")
     if a.counters:
-        lines.extend(counter_table(a.counters))
-    if getattr(a, "step_kernel", None):
+        lines.extend(counter_table(a.counters, kname))
+    if three:
         lines.extend(per_step_table(tr, a.step_kernel, a.skip, a.agents,
                                     getattr(a, "step_grid", 0) or 0))
     return "\n".join(lines) + "\n"
@@ -194,7 +252,7 @@ def per_step_table(tr, step_kernel, skip, agents, grid=0):
     lo = starts[0]
     tot = collections.defaultdict(float)
     for r in tr:
-        if r["start"] >= lo and r["name"].startswith("kg::"):
+        if r["start"] >= lo and _of_three_kernel_step(r["name"]):
             tot[r["name"]] += (r["end"] - r["start"]) / 1000.0
     n = len(starts)
     out = [f"\nKernel time per schedule step ({step_kernel} launches from number {skip}: "
@@ -210,13 +268,13 @@ def per_step_table(tr, step_kernel, skip, agents, grid=0):
     return out
 
 
-def counter_table(path):
-    """Per k_rt_step launch class (grid size): the mean of every counter per
-    dispatch, plus the derived shares the step kernel is judged by."""
+def counter_table(path, kernel=KERNEL):
+    """Per launch class (grid size) of `kernel`: the mean of every counter
+    per dispatch, plus the derived shares the step kernel is judged by."""
     vals = collections.defaultdict(lambda: collections.defaultdict(float))
     disp = collections.defaultdict(set)
     for r in csv.DictReader(open(path)):
-        if _name(r) != KERNEL:
+        if _name(r) != kernel:
             continue
         k = _grid(r)
         vals[k][r["Counter_Name"]] += float(r["Counter_Value"])
@@ -224,7 +282,8 @@ def counter_table(path):
     if not vals:
         return []
     names = sorted({c for v in vals.values() for c in v})
-    out = ["\nHardware counters per k_rt_step dispatch (mean over dispatches of each grid):\n",
+    short = kernel.split("::")[-1]
+    out = [f"\nHardware counters per {short} dispatch (mean over dispatches of each grid):\n",
            "| grid | dispatches | " + " | ".join(names) + " |",
            "|---|---|" + "---|" * len(names)]
     for k in sorted(vals, key=lambda k: -k):
@@ -236,7 +295,7 @@ def counter_table(path):
         wc = v["SQ_WAVE_CYCLES"]
         out.append("")
         if "SQ_WAIT_ANY" in v:
-            out.append(f"- class-0 launches: waves wait on memory or dependencies "
+            out.append(f"- largest launches: waves wait on memory or dependencies "
                        f"{v['SQ_WAIT_ANY'] / wc * 100:.1f} % of their cycles (SQ_WAIT_ANY / SQ_WAVE_CYCLES)")
         if "SQ_ACTIVE_INST_VALU" in v:
             out.append(f"- VALU busy {v['SQ_ACTIVE_INST_VALU'] / wc * 100:.1f} % of wave cycles "

@@ -16,7 +16,8 @@ emu_field* emu_create(float w, float h, float d, int tor, uint32_t cap, uint32_t
                       int lds_cap);
 void emu_destroy(emu_field* f);
 void emu_options(emu_field* f, int force_global, uint64_t shuffle_seed);
-void emu_stats(emu_field* f, uint64_t* lds_tiles, uint64_t* global_tiles);
+void emu_stats(emu_field* f, uint64_t* lds_tiles, uint64_t* global_tiles,
+               uint64_t* chunked_tiles);
 int emu_set(emu_field* f, const uint32_t* ids, const float* x, const float* y, const float* hx,
             const float* hy, uint32_t n);
 int emu_lazy_update(emu_field* f);
@@ -143,8 +144,8 @@ static int run(const Scenario& s) {
   const uint32_t m = emu_read(e, gid.data(), gx.data(), gy.data(), ghx.data(), ghy.data());
   std::vector<float> rx(s.n), ry(s.n), rhx(s.n), rhy(s.n);
   or_flocker_get(fl, rx.data(), ry.data(), rhx.data(), rhy.data());
-  uint64_t lds = 0, glob = 0;
-  emu_stats(e, &lds, &glob);
+  uint64_t lds = 0, glob = 0, chunked = 0;
+  emu_stats(e, &lds, &glob, &chunked);
   uint32_t bad = 0;
   std::vector<int> seen(s.n, 0);
   for (uint32_t k = 0; k < m; ++k) {
@@ -158,8 +159,10 @@ static int run(const Scenario& s) {
       ++bad;
   }
   if (m != s.n) bad += s.n;
-  std::printf("%-28s rc=%d agents=%u lds_tiles=%llu global_tiles=%llu mismatches=%u\n", s.name,
-              rc, m, (unsigned long long)lds, (unsigned long long)glob, bad);
+  std::printf("%-28s rc=%d agents=%u lds_tiles=%llu chunked=%llu global_tiles=%llu "
+              "mismatches=%u\n",
+              s.name, rc, m, (unsigned long long)lds, (unsigned long long)chunked,
+              (unsigned long long)glob, bad);
   emu_destroy(e);
   or_flocker_free(fl);
   return (rc != 0 || bad != 0) ? 1 : 0;
@@ -241,6 +244,9 @@ int main() {
       {"dense-60", 3000, 60.0f, D, 1, 8, 3, 0, 0, 0, 0, 0, 10.0f},
       {"clustered", 6000, 700.0f, D, 1, 8, 4, 0, 0, 0, 0, 1, 10.0f},
       {"clustered-cap128", 6000, 700.0f, D, 1, 4, 4, 0, 128, 0, 3, 1, 10.0f},
+      {"chunked-cap192", 6000, 700.0f, D, 1, 6, 4, 8, 192, 0, 3, 1, 10.0f},
+      {"chunked-uniform-cap256", 8000, 300.0f, D, 1, 6, 11, 4, 256, 0, 0, 0, 10.0f},
+      {"chunked-5x5-cap256", 4000, 160.0f, 4.0f, 1, 6, 6, 4, 256, 0, 0, 0, 10.0f},
       {"padding-edges", 3000, 200.0f, D, 1, 12, 5, 0, 0, 0, 0, 2, 10.0f},
       {"padding-edges-global", 3000, 200.0f, D, 1, 6, 5, 0, 0, 1, 0, 2, 10.0f},
       {"radius-5x5", 2500, 160.0f, 4.0f, 1, 6, 6, 0, 0, 0, 0, 0, 10.0f},

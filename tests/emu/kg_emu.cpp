@@ -91,7 +91,7 @@ struct emu_field {
   bool w_aligned = false;
   bool force_global = false;
   uint64_t shuffle_seed = 0;  // 0 = scatter in index order
-  uint64_t lds_tiles = 0, global_tiles = 0, deferred_tiles = 0;
+  uint64_t lds_tiles = 0, global_tiles = 0, deferred_tiles = 0, chunked_tiles = 0;
 };
 
 extern "C" {
@@ -429,9 +429,11 @@ void emu_options(emu_field* f, int force_global, uint64_t shuffle_seed) {
   f->shuffle_seed = shuffle_seed;
 }
 
-void emu_stats(emu_field* f, uint64_t* lds_tiles, uint64_t* global_tiles) {
+void emu_stats(emu_field* f, uint64_t* lds_tiles, uint64_t* global_tiles,
+               uint64_t* chunked_tiles) {
   *lds_tiles = f->lds_tiles;
   *global_tiles = f->global_tiles;
+  *chunked_tiles = f->chunked_tiles;
 }
 
 int emu_set(emu_field* f, const uint32_t* ids, const float* x, const float* y, const float* hx,
@@ -550,19 +552,36 @@ int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step) {
       }
       col_base[ft.ncols] = run;
       L = run;
-      use_lds = L <= (uint32_t)f->lds_cap;
+    }
+    const int32_t own_lo = ft.c0 - ft.sc_lo, own_hi = ft.c1 - ft.sc_lo;
+    const uint32_t cap = (uint32_t)f->lds_cap;
+    if (use_lds && L > cap) {
+      use_lds = L <= kFlockChunkFactor * cap;
+      for (int32_t c = own_lo; use_lds && c < own_hi; ++c)
+        if (flock_chunk_size(rp, ft, col_base.data(), c, c + 1) > cap) use_lds = false;
     }
     if (use_lds) {
       ++f->lds_tiles;
-      std::fill(smem.begin(), smem.end(), 0xA5);
+      if (L > cap) ++f->chunked_tiles;
       const FlockSmem S = flock_carve(smem.data(), f->lds_cap);
-      uint32_t nown = 0;
-      for (uint32_t k = 0; k < L; ++k) {
-        flock_stage(rp, ft, k, R, seg_start.data(), seg_len.data(), col_base.data(), S);
-        flock_own(ft, k, S, &nown);
+      for (int32_t oa = own_lo; oa < own_hi;) {
+        const int32_t ob =
+            L <= cap ? own_hi : flock_chunk_end(rp, ft, col_base.data(), oa, own_hi, cap);
+        int32_t sa, sb;
+        flock_chunk_cols(rp, ft, oa, ob, &sa, &sb);
+        const uint32_t base = col_base[sa], n = col_base[sb] - base;
+        if (n > cap) return -7;  // a chunk past the capacity: a planning error
+        std::fill(smem.begin(), smem.end(), 0xA5);
+        uint32_t nown = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+          flock_stage(rp, ft, base + k, k, R, seg_start.data(), seg_len.data(), col_base.data(), S);
+          flock_own(ft, k, oa, ob, S, &nown);
+        }
+        for (uint32_t o = 0; o < nown; ++o)
+          flock_agent_lds(rp, ft, S.own[o], col_base.data(), base, S, W, f->count.data(),
+                          &f->err);
+        oa = ob;
       }
-      for (uint32_t o = 0; o < nown; ++o)
-        flock_agent_lds(rp, ft, S.own[o], col_base.data(), S, W, f->count.data(), &f->err);
     } else {
       ++f->global_tiles;
       gflag[item] = 1;

@@ -4,8 +4,9 @@
     and the same boids on 6,476 x 6,476 (the 1,000-boid configuration's
     0.025 per unit^2): every bird's neighbour vector after binning
     (Field2D::get_neighbors_within_distance, field_2d.rs:386-440) equals the
-    oracle's element for element, and two Schedule::step calls on the
-    tile-resident path equal the oracle's two steps bit for bit.
+    oracle's element for element, and two Schedule::step calls equal the
+    oracle's two steps bit for bit, on the default (three-kernel) path and
+    on the tile-resident step.
   - 8,388,608 boids on a 40,960 x 40,960 torus, 80 % in 64 Gaussian blobs
     (sigma 200), 20 % uniform: one Schedule::step equals the oracle's bit
     for bit, and neighbour vectors at 2,000 sampled birds equal the
@@ -54,20 +55,24 @@ def test_one_million_boids_match_oracle(oracle, side):
     del of
     assert np.array_equal(row_ptr.astype(np.uint64), orp)
     assert np.array_equal(ids, oids)
-    say(f"1M boids, side {side}: {len(ids)} neighbour ids equal; two steps")
-    p = FlockParams(seed=seed)
-    g = Field2D(side, side, D, True, N1)
-    g.init_uniform(N1, seed, 1)
-    g.schedule_run(p, 0, 2)
-    assert g.stats()["rt_steps"] == 2  # the tile-resident step ran
-    st, sid = g.state_by_id(0)
-    g.close()
+    say(f"1M boids, side {side}: {len(ids)} neighbour ids equal; two steps on each path")
     fl = O.Flocker(side, side, D, True, x, y, hx, hy, O.params(seed=seed))
     fl.step(2)
     ref = fl.state()
-    assert np.array_equal(sid, np.arange(N1))
-    bad = np.flatnonzero(np.any(bits(st) != bits(ref), axis=0))
-    assert bad.size == 0, f"{bad.size} birds differ, first {bad[:5]}"
+    del fl
+    p = FlockParams(seed=seed)
+    for tile_resident in (False, True):
+        g = Field2D(side, side, D, True, N1)
+        g.init_uniform(N1, seed, 1)
+        g.schedule_run(p, 0, 2, tile_resident=tile_resident)
+        # the default at this size is the three-kernel path
+        assert g.stats()["rt_steps"] == (2 if tile_resident else 0)
+        st, sid = g.state_by_id(0)
+        g.close()
+        assert np.array_equal(sid, np.arange(N1))
+        bad = np.flatnonzero(np.any(bits(st) != bits(ref), axis=0))
+        assert bad.size == 0, f"tile_resident={tile_resident}: {bad.size} birds differ, " \
+                              f"first {bad[:5]}"
 
 
 def test_clustered_8m_boids_one_step_matches_oracle(oracle):

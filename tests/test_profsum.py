@@ -81,3 +81,57 @@ def test_counter_table_shares(tmp_path):
     assert "55.0 % of their cycles" in out
     assert "VALU busy 40.0 %" in out
     assert "20 VALU instructions per wave" in out
+
+
+FLK = "kg::k_flock(kg::RepDev const*, int)"
+STEP = [FLK, "kg::k_scan_reduce(unsigned int const*, unsigned int, unsigned int*)",
+        "kg::k_scan_down(unsigned int*)", "kg::k_scatter_aligned(kg::RepDev const*)",
+        "kg::k_sort_bins(kg::RepDev const*)"]
+
+
+def three_kernel_rows():
+    # three steps of five kernels, 100 + 10 + 10 + 20 + 20 us back to back,
+    # plus an unrelated kernel between steps 1 and 2
+    rows = []
+    for k in range(3):
+        t = 1_000_000 * (k + 1)
+        for name, dur in zip(STEP, (100_000, 10_000, 10_000, 20_000, 20_000)):
+            rows.append([name, 642304, 256, t, t + dur])
+            t += dur
+    rows.append(["kg::k_rt_step<512>(kg::RepDev const*)", 4096, 512, 2_500_000, 2_600_000])
+    return rows
+
+
+def test_three_kernel_steps_span_and_traffic(tmp_path):
+    t = tmp_path / "t.csv"
+    write(t, TRACE_COLS, three_kernel_rows())
+    groups = profsum.three_kernel_steps(profsum.load_trace(str(t)), "kg::k_flock")
+    assert [len(g) for g in groups] == [5, 5, 5]  # the k_rt_step launch is not a step kernel
+    copy = "__amd_rocclr_copyBuffer"
+    fetch = [[copy, 131072, "FETCH_SIZE", 524288.0]]
+    write_ = [[copy, 131072, "WRITE_SIZE", 1048576.0]]
+    for _ in range(3):
+        for name, kib in zip(STEP, (300.0, 10.0, 20.0, 100.0, 50.0)):
+            fetch.append([name, 642304, "FETCH_SIZE", kib])
+            write_.append([name, 642304, "WRITE_SIZE", kib / 2])
+    # a dispatch of one step kernel at another grid (another field) is left out
+    fetch.append([STEP[1], 1024, "FETCH_SIZE", 9999.0])
+    fp, wp = tmp_path / "f.csv", tmp_path / "w.csv"
+    write(fp, PMC_COLS, fetch)
+    write(wp, PMC_COLS, write_)
+
+    class A:
+        pass
+
+    a = A()
+    a.trace, a.fetch, a.write = str(t), str(fp), str(wp)
+    a.agents, a.skip, a.count = 1000, 1, 0
+    a.title, a.cmd, a.pmc_cmd = "test", "python3 bench.py", None
+    a.counters = None
+    a.step_kernel, a.step_grid = "kg::k_flock", 642304
+    out = profsum.summarize(a)
+    assert "step_span_us: 160.0" in out
+    assert "flock_us: 100.0" in out
+    # per step: 480 KiB fetched (counted twice by FETCH_SIZE) + 240 KiB written
+    assert f"traffic_bytes_per_step: {(480 * 2 + 240) * 1024}" in out
+    assert "kernel_us_per_step: 160.0" in out
