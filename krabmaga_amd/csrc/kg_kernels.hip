@@ -315,9 +315,30 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock_global(
     const RepDev* __restrict__ rep, Buf R, const uint32_t* __restrict__ offset, Buf W,
     uint32_t* __restrict__ count_next, const uint64_t* __restrict__ gstep,
     uint32_t* __restrict__ err, const uint32_t* __restrict__ gflag, uint32_t n) {
-  const uint32_t gi = blockIdx.x * kFlockThreads + threadIdx.x;
-  if (gi >= n) return;
+  __shared__ uint32_t s_any;
   const RepDev& rp = *rep;
+  const uint32_t g0 = blockIdx.x * kFlockThreads;
+  const uint32_t gi = g0 + threadIdx.x;
+  // the read buffer holds the tiles one after another, so the block's birds
+  // belong to the tiles from its first bird's to its last bird's: skip the
+  // block unless one of those took the global path
+  if (threadIdx.x == 0) {
+    const uint32_t g1 = min(n, g0 + kFlockThreads) - 1u;
+    int32_t ax, ay, bx, by;
+    discretize(rp.g, R.x[g0], R.y[g0], &ax, &ay);
+    discretize(rp.g, R.x[g1], R.y[g1], &bx, &by);
+    uint32_t any = 1u;  // an out-of-grid bird: let every thread look
+    if (cell_valid(rp.g, ax, ay) && cell_valid(rp.g, bx, by)) {
+      const uint32_t t0 = tile_of(rp.g, ax, ay), t1 = tile_of(rp.g, bx, by);
+      if (t1 - t0 < kFlockThreads) {  // else (a sparse field) every thread looks
+        any = 0u;
+        for (uint32_t t = t0; t <= t1; ++t) any |= gflag[t];
+      }
+    }
+    s_any = any;
+  }
+  __syncthreads();
+  if (!s_any || gi >= n) return;
   int32_t cx, cy;
   discretize(rp.g, R.x[gi], R.y[gi], &cx, &cy);
   if (!cell_valid(rp.g, cx, cy) || !gflag[tile_of(rp.g, cx, cy)]) return;

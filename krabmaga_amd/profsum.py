@@ -128,7 +128,13 @@ def summarize(a):
                      f"{a.pmc_cmd or a.cmd}")
         lines.append(f"rocprofv3 --pmc WRITE_SIZE --kernel-trace --output-format csv -d <dir> -o pmc -- "
                      f"{a.pmc_cmd or a.cmd}")
-    lines.append(f"python -m krabmaga_amd.profsum --trace <trace_kernel_trace.csv> ... "
+    if a.counters:
+        names = sorted({r["Counter_Name"] for r in csv.DictReader(open(a.counters))})
+        lines.append(f"rocprofv3 --pmc {' '.join(names)} --kernel-trace --output-format csv "
+                     f"-d <dir> -o pmc -- {a.pmc_cmd or a.cmd}")
+    step = (f"--step-kernel {a.step_kernel} --step-grid {getattr(a, 'step_grid', 0) or 0} "
+            if three else "")
+    lines.append(f"python -m krabmaga_amd.profsum --trace <trace_kernel_trace.csv> ... {step}"
                  f"--agents {a.agents} --skip {a.skip} --count {a.count}")
     lines.append("```\n")
     # per-launch table
@@ -179,7 +185,7 @@ def summarize(a):
                              f"{FLOCK_BYTES_PER_AGENT} B x {a.agents} agents -> "
                              f"{fb / statistics.mean(fl) / 1e3:.1f} GB/s")
                 lines.append(f"flock_us: {statistics.mean(fl):.1f}")
-        if len(groups) > len(sel):
+        if len(groups) > len(sel) and not three:
             allspans = [(max(r["end"] for r in g) - min(r["start"] for r in g)) / 1000.0
                         for g in groups[1:]]
             lines.append(f"- every step of the trace after the first ({len(allspans)} steps): "
