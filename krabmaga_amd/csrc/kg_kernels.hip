@@ -365,8 +365,9 @@ hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W
 // order, ties by position; one block per (replica, tile), the tile's bins
 // taken 256 at a time, one per thread, so every bin's size is read in one
 // round trip.  A bin of at most kSortSmall entries is sorted by its own lane
-// (sort_small_bin); a bin of up to four times a wave's width is ranked by
-// one wave in registers and rewritten in place (every lane has loaded its
+// (sort_small_bin); a bin of up to 64 entries by a quarter wave, four bins
+// side by side (sort_seg_bins); a bin of up to four times a wave's width by
+// one wave -- all in registers, rewritten in place (every lane has loaded its
 // entries before any lane stores); larger bins are ranked by the whole
 // block against keys staged in LDS, written to the scratch buffer T at the
 // same indices and copied back.
@@ -382,7 +383,90 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t lane) {
   return ((uint64_t)hi << 32) | lo;
 }
 
-// One wave ranks bin [s, s + m) (kSortSmall < m <= kSortWaveMax): entry
+__device__ __forceinline__ uint64_t shfl64(uint64_t v, uint32_t src) {
+  const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, (int)src, kWave);
+  const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), (int)src, kWave);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+constexpr uint32_t kSortSeg = 16;                           // lanes per bin on the segment path
+constexpr uint32_t kSortSegMax = kSortSeg * kSortSlots;     // largest bin a segment sorts
+
+// Four bins at once, one per 16-lane segment of the wave: the segment of
+// `lane` ranks bin [s, s + m) (m <= kSortSegMax; m = 0 for an idle
+// segment), entry sl + 16 u in slot u of lane sl of the segment.
+__device__ __forceinline__ void sort_seg_bins(const Geom& g, Buf R, uint32_t s, uint32_t m,
+                                              uint32_t lane) {
+  const uint32_t sl = lane & (kSortSeg - 1);
+  const uint32_t seg0 = lane & ~(kSortSeg - 1);
+  uint32_t idv[kSortSlots], rk[kSortSlots];
+  float xv[kSortSlots], yv[kSortSlots], hxv[kSortSlots], hyv[kSortSlots];
+  uint64_t kv[kSortSlots];
+#pragma unroll
+  for (int u = 0; u < kSortSlots; ++u) {
+    const uint32_t e = sl + (uint32_t)u * kSortSeg;
+    idv[u] = 0;
+    yv[u] = 0.0f;
+    if (e < m) {
+      idv[u] = R.id[s + e];
+      yv[u] = R.y[s + e];
+    }
+    kv[u] = e < m ? bin_key(g, yv[u], idv[u]) : ~0ull;
+    rk[u] = 0;
+  }
+  // out of order: some entry's predecessor (lane sl - 1 of the same slot,
+  // or lane 15 of the slot before for sl = 0) has a larger key
+  bool inv = false;
+#pragma unroll
+  for (int u = 0; u < kSortSlots; ++u) {
+    const uint32_t e = sl + (uint32_t)u * kSortSeg;
+    const uint64_t left = shfl64(kv[u], sl > 0 ? lane - 1 : lane);
+    const uint64_t wrap = shfl64(kv[u > 0 ? u - 1 : 0], seg0 + kSortSeg - 1);
+    const uint64_t prev = sl > 0 ? left : wrap;
+    if (e > 0 && e < m && prev > kv[u]) inv = true;
+  }
+  const uint64_t segmask = 0xFFFFull << seg0;
+  const bool todo = (__ballot(inv) & segmask) != 0;
+  if (!__any(todo)) return;
+  uint32_t mmax = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < kWave; q += kSortSeg)
+    mmax = max(mmax, (uint32_t)__builtin_amdgcn_readlane((int)(todo ? m : 0u), (int)q));
+  for (uint32_t j = 0; j < mmax; ++j) {
+    const uint32_t ju = j / kSortSeg;  // the same on every lane
+    const uint64_t src = ju == 0 ? kv[0] : (ju == 1 ? kv[1] : (ju == 2 ? kv[2] : kv[3]));
+    const uint64_t kj = shfl64(src, seg0 + (j & (kSortSeg - 1)));
+#pragma unroll
+    for (int u = 0; u < kSortSlots; ++u) {
+      const uint32_t e = sl + (uint32_t)u * kSortSeg;
+      rk[u] += (kj < kv[u]) || (kj == kv[u] && j < e);
+    }
+  }
+  if (!todo) return;
+#pragma unroll
+  for (int u = 0; u < kSortSlots; ++u) {
+    const uint32_t e = sl + (uint32_t)u * kSortSeg;
+    xv[u] = hxv[u] = hyv[u] = 0.0f;
+    if (e < m) {
+      xv[u] = R.x[s + e];
+      hxv[u] = R.hx[s + e];
+      hyv[u] = R.hy[s + e];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kSortSlots; ++u) {
+    const uint32_t e = sl + (uint32_t)u * kSortSeg;
+    if (e < m) {
+      R.id[s + rk[u]] = idv[u];
+      R.x[s + rk[u]] = xv[u];
+      R.y[s + rk[u]] = yv[u];
+      R.hx[s + rk[u]] = hxv[u];
+      R.hy[s + rk[u]] = hyv[u];
+    }
+  }
+}
+
+// One wave ranks bin [s, s + m) (kSortSegMax < m <= kSortWaveMax): entry
 // lane + 64 u in slot u of the lane's registers.
 __device__ __forceinline__ void sort_wave_bin(const Geom& g, Buf R, uint32_t s, uint32_t m,
                                               uint32_t lane) {
@@ -452,9 +536,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
     const uint32_t* __restrict__ offset, Buf T) {
   __shared__ uint64_t s_key[kSortChunk];
+  __shared__ uint32_t s_seg[kSortThreads];
   __shared__ uint32_t s_mid[kSortThreads];
   __shared__ uint32_t s_big[kSortThreads];
-  __shared__ uint32_t s_nmid, s_nbig;
+  __shared__ uint32_t s_nseg, s_nmid, s_nbig;
   const int t = threadIdx.x;
   const uint32_t lane = (uint32_t)t & (kWave - 1);
   const uint32_t wid = (uint32_t)t / kWave;
@@ -466,6 +551,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
 
   for (uint32_t k0 = 0; k0 < nb; k0 += kSortThreads) {
     if (t == 0) {
+      s_nseg = 0u;
       s_nmid = 0u;
       s_nbig = 0u;
     }
@@ -474,10 +560,23 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
       const uint32_t b = b0 + k0 + (uint32_t)t;
       const uint32_t s = offset[b], m = offset[b + 1] - s;
       if (m >= 2u && m <= kSortSmall) sort_small_bin(g, R, s, m);
-      else if (m > kSortSmall && m <= kSortWaveMax) s_mid[atomicAdd(&s_nmid, 1u)] = b;
+      else if (m > kSortSmall && m <= kSortSegMax) s_seg[atomicAdd(&s_nseg, 1u)] = b;
+      else if (m > kSortSegMax && m <= kSortWaveMax) s_mid[atomicAdd(&s_nmid, 1u)] = b;
       else if (m > kSortWaveMax) s_big[atomicAdd(&s_nbig, 1u)] = b;
     }
     __syncthreads();
+    // bins of up to 64: a 16-lane segment each, four per wave
+    const uint32_t nseg = s_nseg;
+    constexpr uint32_t kSegsPerWave = kWave / kSortSeg;
+    for (uint32_t i0 = wid * kSegsPerWave; i0 < nseg; i0 += kSortThreads / kSortSeg) {
+      const uint32_t i = i0 + lane / kSortSeg;
+      uint32_t s = 0, m = 0;
+      if (i < nseg) {
+        s = offset[s_seg[i]];
+        m = offset[s_seg[i] + 1] - s;
+      }
+      sort_seg_bins(g, R, s, m, lane);
+    }
     // middle bins: one wave each
     const uint32_t nmid = s_nmid;
     for (uint32_t i = wid; i < nmid; i += kSortThreads / kWave) {
