@@ -254,8 +254,10 @@ int kg_field_stats(kg_field *f, uint64_t out[16]);
  *
  * A kg_ensemble advances many independent flockers runs -- each its own
  * toroidal Field2D, parameters, Philox key and schedule step -- in one
- * tile-resident layout, one launch per schedule step for all of them.  A
- * run's results are bit-identical to the same run on its own kg_field
+ * device layout, the same launches per schedule step for all of them: a
+ * tile-resident layout (one launch per tile class), or the bin-grouped read
+ * buffer stepped by the three-kernel path over every run at once.  A run's
+ * results are bit-identical to the same run on its own kg_field
  * (kg_schedule_run), whatever else shares the ensemble.  Runs can join and
  * leave between kg_ensemble_run calls; the layout is rebuilt then. */
 typedef struct kg_ensemble kg_ensemble;
@@ -297,7 +299,10 @@ int kg_ensemble_admit(kg_ensemble *e, const kg_run_desc *runs, uint32_t n, uint3
 int kg_ensemble_retire(kg_ensemble *e, uint32_t id);
 
 /* Every run advances `nsteps` schedule steps (simulate!'s loop,
- * src/lib.rs:1163-1173).  flags: KG_RUN_GRAPH.  Asynchronous. */
+ * src/lib.rs:1163-1173).  flags: KG_RUN_GRAPH; KG_RUN_BINNED or
+ * KG_RUN_TILE_RESIDENT choose the layout's form (the layout is rebuilt when
+ * the form changes); without either, ensembles whose live runs average
+ * 32,768 agents or more take the bin-grouped form.  Asynchronous. */
 int kg_ensemble_run(kg_ensemble *e, uint32_t nsteps, int flags);
 
 /* Schedule steps run so far by run `id`. */

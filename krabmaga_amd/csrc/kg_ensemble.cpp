@@ -13,6 +13,14 @@
 // layout: the surviving runs are unloaded into their slots of an append
 // buffer, new runs are initialised into theirs, everything is binned
 // (count / scan / scatter) and loaded into a fresh tile-resident plan.
+//
+// The layout has two forms with identical results, as a single field has:
+// the tile-resident plan above (one k_rt_step launch per tile class and
+// step), and the bin-grouped read buffer itself, stepped by the
+// three-kernel path over every replica at once (one k_flock launch over all
+// tiles, one scan over all bins, one scatter and one bin sort).  By default
+// ensembles whose runs average 32,768 agents or more take the bin-grouped
+// form; KG_RUN_BINNED and KG_RUN_TILE_RESIDENT choose either.
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -97,6 +105,20 @@ struct kg_ensemble {
   size_t cap_tiles = 0;
   uint32_t* err = nullptr;
 
+  // bin-grouped form: the second read buffer, the write buffer, the next
+  // bin offsets, the per-tile global-path flags and the step counter
+  bool binned = false;  // the layout's form
+  int want = -1;        // requested form: -1 by run size, 0 tile-resident, 1 bin-grouped
+  DevBuf R2, W;
+  size_t cap_r2 = 0, cap_w = 0;
+  uint32_t* offset2 = nullptr;  // with the bin arrays (cap_bins + 1)
+  uint32_t* gflag = nullptr;
+  size_t cap_gflag = 0;
+  uint64_t* gstep = nullptr;
+  int lds_cap = 1024;
+  int parity = 0;  // bin-grouped steps taken since the build, mod 2
+  std::vector<std::pair<int, hipGraphExec_t>> bgraphs;  // two-step graphs by parity
+
   // current layout
   std::vector<RepDev> h_reps;
   std::vector<uint32_t> h_prefix;
@@ -118,8 +140,14 @@ struct kg_ensemble {
   }
 
   size_t bytes_held() const {
-    return (cap_a + cap_a2 + cap_r) * 20 + cap_bins * 12 + rt.stay_alloc * 40 + rt.inbox_alloc * 40 +
-           rt.spill_alloc * 44 + rt.boff_alloc * 8 + cap_tiles * 48;
+    return (cap_a + cap_a2 + cap_r + cap_r2 + cap_w) * 20 + cap_bins * 16 + cap_gflag * 4 +
+           rt.stay_alloc * 40 + rt.inbox_alloc * 40 + rt.spill_alloc * 44 +
+           rt.boff_alloc * 8 + cap_tiles * 48;
+  }
+
+  void drop_bgraphs() {
+    for (auto& g : bgraphs) (void)hipGraphExecDestroy(g.second);
+    bgraphs.clear();
   }
 };
 
@@ -160,12 +188,15 @@ int reserve(kg_ensemble* e, size_t bins, size_t reps, size_t tiles) {
     const size_t n = std::max(bins, e->cap_bins + e->cap_bins / 2);
     (void)hipFree(e->count);
     (void)hipFree(e->offset);
+    (void)hipFree(e->offset2);
     (void)hipFree(e->cursor);
     (void)hipFree(e->ss.partial);
-    e->count = e->offset = e->cursor = e->ss.partial = nullptr;
+    e->count = e->offset = e->offset2 = e->cursor = e->ss.partial = nullptr;
     e->cap_bins = 0;
     const size_t nscan = (n + kScanTile - 1) / kScanTile + 1;
+    // offset and offset2 have the same size: the bin-grouped steps swap them
     if ((err = dmalloc(&e->count, n)) != hipSuccess || (err = dmalloc(&e->offset, n + 1)) != hipSuccess ||
+        (err = dmalloc(&e->offset2, n + 1)) != hipSuccess ||
         (err = dmalloc(&e->cursor, n)) != hipSuccess ||
         (err = dmalloc(&e->ss.partial, nscan)) != hipSuccess)
       return set_error(KG_E_NOMEM, "ensemble bin arrays: %s", hipGetErrorString(err));
@@ -242,8 +273,37 @@ bool shrink_crowded(kg_ensemble* e, const std::vector<uint32_t>& counts) {
   return any;
 }
 
+// The form of the next layout: the requested one, or by the runs' mean size
+// (the single-field crossover of the two paths lies between 16k and 64k).
+constexpr uint64_t kBinnedMeanAgents = 32768;
+
+bool want_binned(const kg_ensemble* e) {
+  if (e->want >= 0) return e->want == 1;
+  uint64_t agents = 0, live = 0;
+  for (const EnsRun& r : e->runs)
+    if (!r.dead) {
+      agents += r.desc.agents;
+      ++live;
+    }
+  return live && agents >= kBinnedMeanAgents * live;
+}
+
+// Buffers of the bin-grouped form for `agents` and `tiles` (the second
+// offset table comes with the bin arrays).
+int reserve_binned(kg_ensemble* e, size_t agents, size_t tiles) {
+  if (int rc = reserve_buf(e, &e->R2, &e->cap_r2, agents)) return rc;
+  if (int rc = reserve_buf(e, &e->W, &e->cap_w, agents)) return rc;
+  hipError_t err;
+  if ((err = regrow(&e->gflag, &e->cap_gflag, tiles)) != hipSuccess)
+    return set_error(KG_E_NOMEM, "ensemble bin-grouped buffers: %s", hipGetErrorString(err));
+  if (!e->gstep && (err = dmalloc(&e->gstep, 1)) != hipSuccess)
+    return set_error(KG_E_NOMEM, "ensemble step counter: %s", hipGetErrorString(err));
+  return KG_OK;
+}
+
 // Builds the layout for the current run list: every run's agents go through
-// the append buffer, the bins and a fresh tile-resident plan.
+// the append buffer and the bins, then into a fresh tile-resident plan or,
+// for the bin-grouped form, through the bin sort.
 int rebuild(kg_ensemble* e) {
   KG_HIP(hipStreamSynchronize(e->stream));
   const uint64_t G = e->rt.G;
@@ -267,11 +327,17 @@ int rebuild(kg_ensemble* e) {
   // the old layout (replica tables, A) stays intact until its survivors
   // are unloaded into A and copied into the build buffer A2
   if (int rc = reserve_buf(e, &e->A2, &e->cap_a2, std::max<uint64_t>(agents, 1))) return rc;
-  if (int rc = reserve_buf(e, &e->R, &e->cap_r, std::max<uint64_t>(agents, 1))) return rc;
   const bool have_old = e->built && e->nreps > 0;
-  if (have_old) {
+  const bool old_binned = have_old && e->binned;
+  if (!old_binned)
+    if (int rc = reserve_buf(e, &e->R, &e->cap_r, std::max<uint64_t>(agents, 1))) return rc;
+  // survivors of a bin-grouped layout are already together in the read
+  // buffer, at their append slots
+  if (have_old && !e->binned) {
     if (int rc = unload(e, e->A.b, 0, e->ntiles)) return rc;
   }
+  const Buf old_src = e->binned ? e->R.b : e->A.b;
+  const bool binned = want_binned(e);
   uint32_t base = 0;
   for (size_t i = 0; i < e->runs.size(); ++i) {
     EnsRun& r = e->runs[i];
@@ -279,11 +345,15 @@ int rebuild(kg_ensemble* e) {
       KG_HIP(launch_init_uniform(e->A2.b, base, r.desc.agents, r.desc.width, r.desc.height,
                                  r.desc.init_seed, r.desc.heading_mode, e->stream));
     } else {
-      KG_HIP(copy_buf(e->A2.b, base, e->A.b, old_base[i], r.desc.agents, e->stream));
+      KG_HIP(copy_buf(e->A2.b, base, old_src, old_base[i], r.desc.agents, e->stream));
     }
     r.app_base = base;
     base += r.desc.agents;
   }
+  // the survivors have left the old read buffer: it may grow now
+  if (old_binned)
+    if (int rc = reserve_buf(e, &e->R, &e->cap_r, std::max<uint64_t>(agents, 1))) return rc;
+  e->drop_bgraphs();
   RtPlan plan;
   RtClassPlan cls;
   WorkList w{};
@@ -315,9 +385,10 @@ int rebuild(kg_ensemble* e) {
       rp.app_base = r.app_base;
       rp.app_n = r.desc.agents;
       rp.boff_base = boff_base;
-      // the first tile-resident launch after this layout rebuild runs step G; a run's
-      // next schedule step is steps_done
-      rp.step_base = r.steps_done - G;
+      // a run's next schedule step is steps_done: the first tile-resident
+      // launch after this rebuild runs step G, the bin-grouped step
+      // counter restarts at 0
+      rp.step_base = binned ? r.steps_done : r.steps_done - G;
       r.tile0 = tile0;
       r.ntiles = r.g.ntiles;
       infos[i] = RtRepInfo{r.g, rp.disc_dist, tile0};
@@ -357,6 +428,7 @@ int rebuild(kg_ensemble* e) {
     KG_HIP(hipStreamSynchronize(e->stream));
     if (errflag)
       return set_error(KG_E_RANGE, "an agent discretized outside its run's bag grid");
+    if (binned) break;
     std::vector<double> expect(e->ntiles);
     for (uint32_t i = 0; i < nreps; ++i) {
       const EnsRun& r = e->runs[i];
@@ -377,14 +449,86 @@ int rebuild(kg_ensemble* e) {
     if (attempt >= kShrinkAttempts || !shrink_crowded(e, counts))
       return set_error(KG_E_ADMIT, "the runs' tiles do not fit the LDS budget of the tile-resident step");
   }
-  if (int rc = e->rt.setup(plan, cls, e->nagents, e->stream)) return rc;
-  if (int rc = e->rt.reset(G, e->stream)) return rc;
-  const RtDev D = e->rt.dev();
-  for (int c = 0; c < e->rt.ncls; ++c)
-    KG_HIP(launch_rt_load(w, D, e->rt.caps[c], c, e->rt.dense + e->rt.off[c],
-                          e->rt.off[c + 1] - e->rt.off[c], e->R.b, e->offset, G - 1, e->stream));
+  e->binned = binned;
+  if (binned) {
+    // every bin into (cell y, id) order, the consumed build buffer as scratch
+    if (int rc = reserve_binned(e, std::max<uint64_t>(agents, 1), e->ntiles)) return rc;
+    KG_HIP(launch_sort_bins(w, e->R.b, e->offset, e->A2.b, e->stream));
+    KG_HIP(hipMemsetAsync(e->gstep, 0, sizeof(uint64_t), e->stream));
+    int cap = 256;
+    for (const EnsRun& r : e->runs) cap = std::max(cap, pick_lds_cap(r.g, r.desc.agents));
+    e->lds_cap = clamp_lds_cap(cap);
+    e->parity = 0;
+  } else {
+    if (int rc = e->rt.setup(plan, cls, e->nagents, e->stream)) return rc;
+    if (int rc = e->rt.reset(G, e->stream)) return rc;
+    const RtDev D = e->rt.dev();
+    for (int c = 0; c < e->rt.ncls; ++c)
+      KG_HIP(launch_rt_load(w, D, e->rt.caps[c], c, e->rt.dense + e->rt.off[c],
+                            e->rt.off[c + 1] - e->rt.off[c], e->R.b, e->offset, G - 1,
+                            e->stream));
+  }
   for (EnsRun& r : e->runs) r.fresh = false;
   ++e->builds;
+  return KG_OK;
+}
+
+// One bin-grouped schedule step of every replica: k_flock over all tiles,
+// the scan over all bins (it advances the step counter), the scatter and the
+// bin sort, then the buffers swap.
+hipError_t binned_step(kg_ensemble* e, const WorkList& w) {
+  hipError_t err;
+  if ((err = launch_flock(w, e->R.b, e->offset, e->W.b, e->count, e->gstep, e->err, e->lds_cap,
+                          e->gflag, e->nagents, e->stream)) != hipSuccess)
+    return err;
+  if ((err = launch_scan(e->count, e->offset2, e->cursor, e->nbins, e->ss, e->gstep,
+                         e->stream)) != hipSuccess)
+    return err;
+  if ((err = launch_scatter_aligned(w, e->R.b, e->W.b, e->offset, e->cursor, e->R2.b,
+                                    e->stream)) != hipSuccess)
+    return err;
+  // the write buffer has been consumed: it is the sort's scratch
+  if ((err = launch_sort_bins(w, e->R2.b, e->offset2, e->W.b, e->stream)) != hipSuccess)
+    return err;
+  std::swap(e->R, e->R2);
+  std::swap(e->cap_r, e->cap_r2);
+  std::swap(e->offset, e->offset2);
+  e->parity ^= 1;
+  return hipSuccess;
+}
+
+// `n` bin-grouped steps (graph replays of two steps when allowed: the
+// buffers are back in place after two).
+int binned_steps(kg_ensemble* e, uint32_t n, bool use_graph) {
+  const WorkList w = e->wl();
+  uint32_t left = n;
+  if (use_graph && left >= 2) {
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : e->bgraphs)
+      if (g.first == e->parity) exec = g.second;
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      KG_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t err = binned_step(e, w);
+      if (err == hipSuccess) err = binned_step(e, w);
+      const hipError_t e2 = hipStreamEndCapture(e->stream, &graph);
+      if (err != hipSuccess || e2 != hipSuccess)
+        return set_error(KG_E_HIP, "graph capture: %s",
+                         hipGetErrorString(err != hipSuccess ? err : e2));
+      err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (err != hipSuccess)
+        return set_error(KG_E_HIP, "graph instantiate: %s", hipGetErrorString(err));
+      e->bgraphs.push_back({e->parity, exec});
+    }
+    while (left >= 2) {
+      KG_HIP(hipGraphLaunch(exec, e->stream));
+      left -= 2;
+    }
+  }
+  for (; left > 0; --left) KG_HIP(binned_step(e, w));
+  e->steps_run += n;
+  for (EnsRun& r : e->runs) r.steps_done += n;
   return KG_OK;
 }
 
@@ -435,6 +579,12 @@ int steps(kg_ensemble* e, uint32_t n, bool use_graph) {
 
 int check_flags(kg_ensemble* e) {
   KG_HIP(hipStreamSynchronize(e->stream));
+  if (e->binned) {
+    uint32_t errflag = 0;
+    KG_HIP(hipMemcpy(&errflag, e->err, sizeof(errflag), hipMemcpyDeviceToHost));
+    if (errflag) return set_error(KG_E_RANGE, "an agent left its run's bag grid");
+    return KG_OK;
+  }
   uint32_t fl2[2] = {0, 0};
   KG_HIP(hipMemcpy(fl2, e->rt.flags, sizeof(fl2), hipMemcpyDeviceToHost));
   const uint32_t fl = fl2[0];
@@ -498,13 +648,17 @@ int kg_ensemble_create(const kg_tuning* tuning, kg_ensemble** out) {
 void kg_ensemble_destroy(kg_ensemble* e) {
   if (!e) return;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  e->drop_bgraphs();
   e->rt.release();
   e->A.release();
   e->A2.release();
   e->R.release();
-  for (void* p : {(void*)e->count, (void*)e->offset, (void*)e->cursor, (void*)e->ss.partial,
-                  (void*)e->d_reps, (void*)e->d_prefix, (void*)e->rep_cursor, (void*)e->tcount,
-                  (void*)e->err})
+  e->R2.release();
+  e->W.release();
+  for (void* p : {(void*)e->count, (void*)e->offset, (void*)e->offset2, (void*)e->cursor,
+                  (void*)e->ss.partial, (void*)e->d_reps, (void*)e->d_prefix,
+                  (void*)e->rep_cursor, (void*)e->tcount, (void*)e->err, (void*)e->gflag,
+                  (void*)e->gstep})
     (void)hipFree(p);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
@@ -554,13 +708,18 @@ int kg_ensemble_retire(kg_ensemble* e, uint32_t id) {
 int kg_ensemble_run(kg_ensemble* e, uint32_t nsteps, int flags) {
   clear_error();
   if (!e) return set_error(KG_E_INVALID, "ensemble is NULL");
+  if ((flags & KG_RUN_BINNED) && (flags & KG_RUN_TILE_RESIDENT))
+    return set_error(KG_E_INVALID, "KG_RUN_BINNED and KG_RUN_TILE_RESIDENT exclude each other");
   if (nsteps == 0) return KG_OK;
+  e->want = (flags & KG_RUN_BINNED) ? 1 : ((flags & KG_RUN_TILE_RESIDENT) ? 0 : -1);
+  if (e->built && !e->dirty && e->nreps > 0 && want_binned(e) != e->binned) e->dirty = true;
   if (e->dirty || !e->built)
     if (int rc = rebuild(e)) return rc;
   if (e->nreps == 0) return KG_OK;
+  const bool graph = (flags & KG_RUN_GRAPH) != 0;
   while (nsteps > 0) {
     const uint32_t chunk = std::min(nsteps, kRtCheckEvery);
-    if (int rc = steps(e, chunk, (flags & KG_RUN_GRAPH) != 0)) return rc;
+    if (int rc = e->binned ? binned_steps(e, chunk, graph) : steps(e, chunk, graph)) return rc;
     nsteps -= chunk;
     if (int rc = check_flags(e)) return rc;
     if (e->replan)
@@ -595,16 +754,22 @@ int kg_ensemble_read(kg_ensemble* e, uint32_t id, uint32_t* ids, float* x, float
   size_t idx = 0;
   EnsRun* r = find_run(e, id, &idx);
   if (!r) return set_error(KG_E_INVALID, "no run %u", id);
-  if (int rc = unload(e, e->A.b, r->tile0, r->ntiles)) return rc;
+  // the bin-grouped read buffer holds every run at its append slots; the
+  // tile-resident layout is unloaded there first
+  if (!e->binned)
+    if (int rc = unload(e, e->A.b, r->tile0, r->ntiles)) return rc;
+  const Buf src = e->binned ? e->R.b : e->A.b;
   const size_t m = r->desc.agents;
   const uint32_t b = r->app_base;
-  if (ids) KG_HIP(hipMemcpyAsync(ids, e->A.b.id + b, m * 4, hipMemcpyDeviceToHost, e->stream));
-  if (x) KG_HIP(hipMemcpyAsync(x, e->A.b.x + b, m * 4, hipMemcpyDeviceToHost, e->stream));
-  if (y) KG_HIP(hipMemcpyAsync(y, e->A.b.y + b, m * 4, hipMemcpyDeviceToHost, e->stream));
-  if (hx) KG_HIP(hipMemcpyAsync(hx, e->A.b.hx + b, m * 4, hipMemcpyDeviceToHost, e->stream));
-  if (hy) KG_HIP(hipMemcpyAsync(hy, e->A.b.hy + b, m * 4, hipMemcpyDeviceToHost, e->stream));
-  uint32_t got = 0;
-  KG_HIP(hipMemcpyAsync(&got, e->rep_cursor + idx, sizeof(got), hipMemcpyDeviceToHost, e->stream));
+  if (ids) KG_HIP(hipMemcpyAsync(ids, src.id + b, m * 4, hipMemcpyDeviceToHost, e->stream));
+  if (x) KG_HIP(hipMemcpyAsync(x, src.x + b, m * 4, hipMemcpyDeviceToHost, e->stream));
+  if (y) KG_HIP(hipMemcpyAsync(y, src.y + b, m * 4, hipMemcpyDeviceToHost, e->stream));
+  if (hx) KG_HIP(hipMemcpyAsync(hx, src.hx + b, m * 4, hipMemcpyDeviceToHost, e->stream));
+  if (hy) KG_HIP(hipMemcpyAsync(hy, src.hy + b, m * 4, hipMemcpyDeviceToHost, e->stream));
+  uint32_t got = (uint32_t)m;
+  if (!e->binned)
+    KG_HIP(hipMemcpyAsync(&got, e->rep_cursor + idx, sizeof(got), hipMemcpyDeviceToHost,
+                          e->stream));
   KG_HIP(hipStreamSynchronize(e->stream));
   if (got != r->desc.agents)
     return set_error(KG_E_STATE, "run %u holds %u agents, expected %u", id, got, r->desc.agents);

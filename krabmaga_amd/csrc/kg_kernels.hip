@@ -310,38 +310,43 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
   }
 }
 
-// The birds of flagged tiles, one thread per read-buffer slot (one replica).
+// The birds of flagged tiles, one thread per read-buffer slot.  The read
+// buffer holds the replicas one after another (replica r at
+// app_prefix[r], its tiles one after another), each tile's bins in order.
 __global__ __launch_bounds__(kFlockThreads) void k_flock_global(
-    const RepDev* __restrict__ rep, Buf R, const uint32_t* __restrict__ offset, Buf W,
+    const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix,
+    const uint32_t* __restrict__ app_prefix, Buf R, const uint32_t* __restrict__ offset, Buf W,
     uint32_t* __restrict__ count_next, const uint64_t* __restrict__ gstep,
     uint32_t* __restrict__ err, const uint32_t* __restrict__ gflag, uint32_t n) {
   __shared__ uint32_t s_any;
-  const RepDev& rp = *rep;
   const uint32_t g0 = blockIdx.x * kFlockThreads;
   const uint32_t gi = g0 + threadIdx.x;
-  // the read buffer holds the tiles one after another, so the block's birds
-  // belong to the tiles from its first bird's to its last bird's: skip the
-  // block unless one of those took the global path
+  // global tile of read-buffer slot q, or false for a bird outside its grid
+  auto tile_at = [&](uint32_t q, uint32_t* t) {
+    const int r = nreps == 1 ? 0 : find_rep(app_prefix, nreps, q);
+    const Geom& g = reps[r].g;
+    int32_t cx, cy;
+    discretize(g, R.x[q], R.y[q], &cx, &cy);
+    if (!cell_valid(g, cx, cy)) return false;
+    *t = tile_prefix[r] + tile_of(g, cx, cy);
+    return true;
+  };
+  // the block's birds belong to the tiles from its first bird's to its last
+  // bird's: skip the block unless one of those took the global path
   if (threadIdx.x == 0) {
     const uint32_t g1 = min(n, g0 + kFlockThreads) - 1u;
-    int32_t ax, ay, bx, by;
-    discretize(rp.g, R.x[g0], R.y[g0], &ax, &ay);
-    discretize(rp.g, R.x[g1], R.y[g1], &bx, &by);
-    uint32_t any = 1u;  // an out-of-grid bird: let every thread look
-    if (cell_valid(rp.g, ax, ay) && cell_valid(rp.g, bx, by)) {
-      const uint32_t t0 = tile_of(rp.g, ax, ay), t1 = tile_of(rp.g, bx, by);
-      if (t1 - t0 < kFlockThreads) {  // else (a sparse field) every thread looks
-        any = 0u;
-        for (uint32_t t = t0; t <= t1; ++t) any |= gflag[t];
-      }
+    uint32_t t0 = 0, t1 = 0, any = 1u;  // an out-of-grid bird: let every thread look
+    if (tile_at(g0, &t0) && tile_at(g1, &t1) && t1 - t0 < kFlockThreads) {
+      any = 0u;
+      for (uint32_t t = t0; t <= t1; ++t) any |= gflag[t];
     }
     s_any = any;
   }
   __syncthreads();
   if (!s_any || gi >= n) return;
-  int32_t cx, cy;
-  discretize(rp.g, R.x[gi], R.y[gi], &cx, &cy);
-  if (!cell_valid(rp.g, cx, cy) || !gflag[tile_of(rp.g, cx, cy)]) return;
+  uint32_t t;
+  if (!tile_at(gi, &t) || !gflag[t]) return;
+  const RepDev& rp = reps[nreps == 1 ? 0 : find_rep(app_prefix, nreps, gi)];
   flock_agent_global(rp, rp.step_base + *gstep, gi, R, offset, W, count_next, err);
 }
 
@@ -349,14 +354,13 @@ hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W
                         uint32_t* count_next, const uint64_t* gstep, uint32_t* err, int lds_cap,
                         uint32_t* gflag, uint32_t nagents, hipStream_t s) {
   if (wl.ntiles == 0) return hipSuccess;
-  if (wl.nreps != 1) return hipErrorInvalidValue;  // k_flock_global steps one replica
   hipLaunchKernelGGL(k_flock, dim3(wl.ntiles), dim3(kFlockThreads), flock_lds_bytes(lds_cap), s,
                      wl.reps, wl.nreps, wl.tile_prefix, R, offset, W, count_next, gstep, err,
                      lds_cap, gflag);
   if (nagents)
     hipLaunchKernelGGL(k_flock_global, dim3((nagents + kFlockThreads - 1) / kFlockThreads),
-                       dim3(kFlockThreads), 0, s, wl.reps, R, offset, W, count_next, gstep, err,
-                       gflag, nagents);
+                       dim3(kFlockThreads), 0, s, wl.reps, wl.nreps, wl.tile_prefix,
+                       wl.app_prefix, R, offset, W, count_next, gstep, err, gflag, nagents);
   return hipGetLastError();
 }
 

@@ -95,9 +95,15 @@ class Ensemble:
         check(self._L.kg_ensemble_retire(self._h, run_id))
         self._agents.pop(run_id, None)
 
-    def run(self, nsteps, use_graph=True):
-        """Every run advances nsteps schedule steps (asynchronous)."""
-        check(self._L.kg_ensemble_run(self._h, nsteps, 1 if use_graph else 0))
+    def run(self, nsteps, use_graph=True, binned=False, tile_resident=False):
+        """Every run advances nsteps schedule steps (asynchronous).
+
+        By default ensembles whose runs average 32,768 agents or more step
+        the bin-grouped layout with the three-kernel path, smaller ones the
+        tile-resident layout; binned / tile_resident choose either (same
+        results)."""
+        flags = (1 if use_graph else 0) | (2 if binned else 0) | (4 if tile_resident else 0)
+        check(self._L.kg_ensemble_run(self._h, nsteps, flags))
 
     def steps(self, run_id):
         s = ctypes.c_uint64()

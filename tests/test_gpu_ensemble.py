@@ -52,11 +52,15 @@ def uneven_specs(k, base_seed=11):
     return out
 
 
-def test_ensemble_matches_runs_alone():
+PATHS = [{"tile_resident": True}, {"binned": True}]
+
+
+@pytest.mark.parametrize("path", PATHS, ids=["tile_resident", "binned"])
+def test_ensemble_matches_runs_alone(path):
     specs = uneven_specs(6)
     with Ensemble() as ens:
         ids = ens.admit(specs)
-        ens.run(40)
+        ens.run(40, **path)
         ens.sync()
         st = ens.stats()
         assert st["runs"] == 6 and st["agents"] == sum(s.agents for s in specs)
@@ -65,31 +69,33 @@ def test_ensemble_matches_runs_alone():
             assert_same(ens, rid, s, 40, f"run {rid} (n={s.agents})")
 
 
-def test_ensemble_direct_launches_match_graphs():
+@pytest.mark.parametrize("path", PATHS, ids=["tile_resident", "binned"])
+def test_ensemble_direct_launches_match_graphs(path):
     specs = uneven_specs(3, base_seed=40)
     with Ensemble() as a, Ensemble() as b:
         ia = a.admit(specs)
         ib = b.admit(specs)
-        a.run(27, use_graph=True)
-        b.run(27, use_graph=False)
+        a.run(27, use_graph=True, **path)
+        b.run(27, use_graph=False, **path)
         for x, y in zip(ia, ib):
             sa, _ = a.state_by_id(x)
             sb, _ = b.state_by_id(y)
             assert np.array_equal(sa.view(np.uint32), sb.view(np.uint32))
 
 
-def test_ensemble_runs_join_and_leave():
+@pytest.mark.parametrize("path", PATHS, ids=["tile_resident", "binned"])
+def test_ensemble_runs_join_and_leave(path):
     s_a, s_b, s_c, s_d = uneven_specs(4, base_seed=70)
     with Ensemble() as ens:
         a, b = ens.admit([s_a, s_b])
-        ens.run(20)
+        ens.run(20, **path)
         (c,) = ens.admit([s_c])
-        ens.run(15)
+        ens.run(15, **path)
         assert ens.steps(a) == 35 and ens.steps(c) == 15
         assert_same(ens, a, s_a, 35, "a before leaving")
         ens.retire(a)
         (d,) = ens.admit([s_d])
-        ens.run(10)
+        ens.run(10, **path)
         assert_same(ens, b, s_b, 45, "b")
         assert_same(ens, c, s_c, 25, "c")
         assert_same(ens, d, s_d, 10, "d")
@@ -99,7 +105,7 @@ def test_ensemble_runs_join_and_leave():
         live = sorted([(s_b.agents, b, s_b, 45), (s_c.agents, c, s_c, 25), (s_d.agents, d, s_d, 10)])
         builds = ens.stats()["builds"]
         ens.retire(live[0][1])
-        ens.run(9)
+        ens.run(9, **path)
         assert ens.stats()["builds"] == builds and ens.stats()["runs"] == 2
         for _, rid, s, k in live[1:]:
             assert_same(ens, rid, s, k + 9, f"run {rid} after the smallest left")
@@ -107,25 +113,26 @@ def test_ensemble_runs_join_and_leave():
         # the layout: the next run compacts it
         tiles = ens.stats()["tiles"]
         ens.retire(live[2][1])
-        ens.run(5)
+        ens.run(5, **path)
         st = ens.stats()
         assert st["builds"] == builds + 1 and st["runs"] == 1 and st["tiles"] < tiles
         _, rid, s, k = live[1]
         assert_same(ens, rid, s, k + 14, "last run after compaction")
 
 
-def test_ensemble_read_after_retire_and_admit():
+@pytest.mark.parametrize("path", PATHS, ids=["tile_resident", "binned"])
+def test_ensemble_read_after_retire_and_admit(path):
     # a read that rebuilds the layout (an admission is pending) while a
     # retired run still sits in it: the read must address the run by id
     # after the rebuild has dropped the retired one
     s_a, s_b, s_c, s_d = uneven_specs(4, base_seed=130)
     with Ensemble() as ens:
         a, b, c = ens.admit([s_a, s_b, s_c])
-        ens.run(10)
+        ens.run(10, **path)
         ens.retire(a)
         (d,) = ens.admit([s_d])
         assert_same(ens, c, s_c, 10, "c read during the rebuild")
-        ens.run(5)
+        ens.run(5, **path)
         assert_same(ens, b, s_b, 15, "b")
         assert_same(ens, d, s_d, 5, "d")
         with pytest.raises(Exception):
@@ -163,3 +170,18 @@ def test_ensemble_rejects_non_toroidal_runs():
     with Ensemble() as ens:
         rc = _abi.lib().kg_ensemble_admit(ens._h, ctypes.byref(c), 1, None)
         assert rc == _abi.KG_E_INVALID
+
+
+def test_ensemble_switches_between_layouts():
+    # steps on one form, then the other, then back: every run keeps its bits
+    specs = uneven_specs(4, base_seed=160)
+    with Ensemble() as ens:
+        ids = ens.admit(specs)
+        ens.run(7, tile_resident=True)
+        ens.run(6, binned=True)
+        ens.run(5, tile_resident=True)
+        ens.run(3, binned=True)
+        assert ens.stats()["builds"] >= 4
+        for rid, s in zip(ids, specs):
+            assert ens.steps(rid) == 21
+            assert_same(ens, rid, s, 21, f"run {rid} (n={s.agents})")
