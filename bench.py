@@ -225,21 +225,22 @@ def _one_by_one(hip, jobs, warmup, whole_run=False):
 
 
 def run_batched(args, hip):
-    """512 replicas of uneven size in one tile-resident layout."""
+    """512 replicas of uneven size in one layout, its form and tile shape
+    chosen by the autotuner."""
     from krabmaga_amd import Ensemble, autotune, workloads
     specs = workloads.batched(args.replicas)
     agents = sum(s.agents for s in specs)
     steps = args.batched_steps
     log(f"batched: {len(specs)} replicas, {agents} agents")
     table = []
-    tuning = None
+    tuning, path = None, {}
     if not args.no_tune:
-        tuning, table = autotune.tune(specs, steps=16, warmup=8, log=log)
+        tuning, path, table = autotune.tune(specs, steps=16, warmup=8, log=log)
     ens = Ensemble(tuning)
     ens.admit(specs)
-    ens.run(args.warmup)
+    ens.run(args.warmup, **path)
     ens.sync()
-    ms = _timed(hip, ens.stream, lambda: ens.run(steps))
+    ms = _timed(hip, ens.stream, lambda: ens.run(steps, **path))
     st = ens.stats()
     ens.close()
     value = agents * steps / (ms * 1e-3)
@@ -261,7 +262,9 @@ def run_batched(args, hip):
                                  max(1, len(specs) // k)},
         "vs_one_by_one": value / obo_value,
         "tuning": None if tuning is None else tuning.__dict__,
-        "autotune": [{"tuning": t.__dict__, "ms_per_step": m} for t, m in table],
+        "form": "bin-grouped (three-kernel path)" if path.get("binned") else
+                ("tile-resident" if path.get("tile_resident") else "default"),
+        "autotune": [{"tuning": t.__dict__, "path": pth, "ms_per_step": m} for t, pth, m in table],
         "layout": st,
     }
     if not args.no_cpu:

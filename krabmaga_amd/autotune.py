@@ -1,12 +1,14 @@
-"""Autotuner of the tile-resident layout for a workload.
+"""Autotuner of an ensemble's layout for a workload.
 
 The reference fixes the cell size (Field2D::new's discretization) and it is
 part of the semantics (the query window is floor(radius / d) cells), so the
-tunable knobs are the device-side ones: tile width and height in cells, bin
-height, and the LDS budget of a normal-class work-group.  The tuner builds an
-ensemble of the workload for each candidate, times a few schedule steps with
-HIP events on the ensemble's stream after a warm-up, and keeps the fastest.
-Results do not depend on the choice (the tests check this), only speed.
+tunable knobs are the device-side ones: the layout's form (tile-resident or
+bin-grouped), tile width and height in cells, bin height, and the LDS budget
+of a normal-class work-group of the tile-resident form.  The tuner builds an
+ensemble of the workload for each candidate, times a few schedule steps
+with HIP events on the ensemble's stream after a warm-up, and keeps the
+fastest.  Results do not depend on the choice (the tests check this), only
+speed.
 """
 import ctypes
 import itertools
@@ -54,9 +56,15 @@ def candidates(mean_agents_per_cell):
     return out
 
 
+PATHS = ({"binned": True}, {"tile_resident": True})
+
+
 def tune(specs, steps=16, warmup=8, max_candidates=12, log=None):
-    """Picks the fastest Tuning for the runs `specs`; returns (tuning, table)
-    with table rows (tuning, ms per step)."""
+    """Picks the fastest Tuning and form for the runs `specs`; returns
+    (tuning, path, table): path holds the Ensemble.run keywords of the form,
+    table rows are (tuning, path, ms per step).  The LDS budget only shapes
+    the tile-resident form, so the bin-grouped one is timed once per tile
+    shape."""
     hip = _Hip()
     cells = sum(s.width * s.height / (s.discretization ** 2) for s in specs)
     agents = sum(s.agents for s in specs)
@@ -66,19 +74,26 @@ def tune(specs, steps=16, warmup=8, max_candidates=12, log=None):
         idx = np.linspace(1, len(cands) - 1, max_candidates - 1).round().astype(int)
         cands = [cands[0]] + [cands[i] for i in sorted(set(idx))]
     table = []
+    seen = set()
     for t in cands:
-        try:
-            with Ensemble(t) as ens:
-                ens.admit(specs)
-                ens.run(warmup)
-                ens.sync()
-                ms = hip.elapsed_ms(ens.stream, lambda: ens.run(steps))
-        except Exception as exc:  # a candidate the geometry rejects
+        for path in PATHS:
+            if path.get("binned"):
+                shape = (t.tile_cols, t.tile_rows, t.bin_rows)
+                if shape in seen:
+                    continue
+                seen.add(shape)
+            try:
+                with Ensemble(t) as ens:
+                    ens.admit(specs)
+                    ens.run(warmup, **path)
+                    ens.sync()
+                    ms = hip.elapsed_ms(ens.stream, lambda: ens.run(steps, **path))
+            except Exception as exc:  # a candidate the geometry rejects
+                if log:
+                    log(f"autotune: {t} {path} rejected ({exc})")
+                continue
+            table.append((t, path, ms / steps))
             if log:
-                log(f"autotune: {t} rejected ({exc})")
-            continue
-        table.append((t, ms / steps))
-        if log:
-            log(f"autotune: {t} -> {ms / steps:.3f} ms/step")
-    best = min(table, key=lambda r: r[1])[0]
-    return best, table
+                log(f"autotune: {t} {path} -> {ms / steps:.3f} ms/step")
+    best = min(table, key=lambda r: r[2])
+    return best[0], best[1], table
