@@ -164,11 +164,15 @@ hipError_t launch_materialize(Buf R, Buf W, uint32_t n, Buf A, hipStream_t s) {
 
 // ---------------------------------------------------------------------------
 // scan: exclusive prefix over the bin counts in two launches (tile sums,
-// then tile scans, each tile adding up the sums of the tiles before it).
+// then tile scans, each tile adding up the sums of the tiles before it), or,
+// past kScanInlineTiles tiles (where those sums would grow with the square
+// of the tile count), three: a one-block scan of the tile sums between them.
 // No work-group waits on another, so nothing depends on inter-work-group
-// ordering.  The second launch also zeroes the counts (for the next
+// ordering.  The last launch also zeroes the counts (for the next
 // histogram), fills the scatter cursors and advances the schedule-step
 // counter.
+
+constexpr uint32_t kScanInlineTiles = 2048;
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ count,
                                                               uint32_t n,
@@ -184,16 +188,40 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __
   if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
+// One block: partial[0..m) becomes its exclusive prefix.
+__global__ __launch_bounds__(1024) void k_scan_partials(uint32_t* __restrict__ partial,
+                                                        uint32_t m) {
+  __shared__ uint32_t s_warp[1024 / kWave];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < m; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < m ? partial[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan<1024>(v, s_warp, &total);
+    const uint32_t carry = s_carry;
+    if (i < m) partial[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry = carry + total;
+    __syncthreads();
+  }
+}
+
+// prefixed: partial[] already holds each tile's exclusive prefix.
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict__ count,
                                                             uint32_t* __restrict__ offset,
                                                             uint32_t* __restrict__ cursor, uint32_t n,
                                                             const uint32_t* __restrict__ partial,
-                                                            uint64_t* gstep) {
+                                                            int prefixed, uint64_t* gstep) {
   __shared__ uint32_t s_warp[kScanThreads / kWave];
   const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-  // the tiles before this one (a few thousand sums at most, cached)
+  // the tiles before this one (at most kScanInlineTiles sums, cached)
   uint32_t before = 0;
-  for (uint32_t q = threadIdx.x; q < blockIdx.x; q += kScanThreads) before += partial[q];
+  if (!prefixed)
+    for (uint32_t q = threadIdx.x; q < blockIdx.x; q += kScanThreads) before += partial[q];
+  else if (threadIdx.x == 0)
+    before = partial[blockIdx.x];
   uint32_t v[kScanItems];
   uint32_t sum = 0;
 #pragma unroll
@@ -225,8 +253,11 @@ hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint
   const uint32_t ntiles = (nbins + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kScanThreads), 0, s, count, nbins,
                      ss.partial);
+  const int prefixed = ntiles > kScanInlineTiles ? 1 : 0;
+  if (prefixed)
+    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, ss.partial, ntiles);
   hipLaunchKernelGGL(k_scan_down, dim3(ntiles), dim3(kScanThreads), 0, s, count, offset, cursor,
-                     nbins, ss.partial, gstep);
+                     nbins, ss.partial, prefixed, gstep);
   return hipGetLastError();
 }
 
