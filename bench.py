@@ -139,8 +139,10 @@ PLAIN_KERNELS = {"default": ("flock", "scan", "scatter", "sort_bins"),
 def run_plain(args, hip):
     """The 1M-boid field on its default path (the three-kernel path at this
     size) and, from the same initial state, on the tile-resident step: each
-    warmed up, then timed over the same window; per-kernel durations over a
-    few steps launched one by one after it."""
+    warmed up, then timed over the same window.  Per-kernel durations come
+    from the same run made again from the same initial state with the timed
+    window's steps launched one by one (HIP events around every launch); the
+    run is deterministic, so they cover the very steps the timed region ran."""
     from krabmaga_amd import Field2D, FlockParams
     w = plain_workload()
     n, side = w["n"], w["side"]
@@ -157,9 +159,14 @@ def run_plain(args, hip):
         layout = f.stats()
         log(f"{path} path: {ms / args.steps:.4f} ms/step over {args.steps} steps; "
             f"layout {layout}")
-        kprof = min(args.steps, args.profile_steps)
+        f.close()
+        kprof = args.steps if args.profile_steps <= 0 else min(args.steps, args.profile_steps)
+        f = Field2D(side, side, w["d"], True, n)
+        f.init_uniform(n, w["seed"], 1)
+        f.schedule_run(p, 0, args.warmup, **kw)
+        f.sync()
         f.profile(True)
-        f.schedule_run(p, args.warmup + args.steps, kprof, use_graph=False, **kw)
+        f.schedule_run(p, args.warmup, kprof, use_graph=False, **kw)
         kern = {}
         for k in PLAIN_KERNELS[path]:
             tot, cnt = f.profile_read(k)
@@ -444,7 +451,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="steps of the timed window given per-kernel durations (0: all)")
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -529,7 +537,9 @@ def main():
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            "dominant_kernel": "k_" + dom,
+            # the flock phase is k_flock plus its k_flock_global launch (tiles
+            # that do not fit LDS), timed as one
+            "dominant_kernel": "k_flock + k_flock_global" if three else "k_rt_step",
             "dominant_kernel_us": dom_us,
             "dominant_kernel_bytes_per_agent": dom_bytes,
             "dominant_kernel_frac": dom_bytes * n / (dom_us * 1e-6) / HBM_PEAK if dom_us else 0.0,

@@ -38,6 +38,9 @@ FLOCK = "kg::k_flock"
 # the kernels of one three-kernel schedule step
 STEP_KERNELS = ("kg::k_flock", "kg::k_flock_global", "kg::k_scan_reduce", "kg::k_scan_down",
                 "kg::k_scatter_aligned", "kg::k_sort_bins")
+# launched within a three-kernel step on some layouts: the tile-sum prefix
+# pass of large scans, the device step counter
+STEP_AUX_KERNELS = ("kg::k_scan_partials", "kg::k_add_u64")
 COPY = "__amd_rocclr_copyBuffer"
 BYTES_PER_AGENT = 76  # per agent-update of a whole schedule step
 KERNEL_BYTES_PER_AGENT = 40  # the tile-resident kernel: read 20 bytes, write 20 back
@@ -103,18 +106,20 @@ def pmc_totals(path, counter):
 
 def three_kernel_steps(trace, step_kernel, grid=0):
     """Launch groups of the three-kernel path: each launch of `step_kernel`
-    (of `grid` threads, if given) starts a step that runs to the next one."""
+    (of `grid` threads, if given) starts a step, which runs through the step's
+    kernels that follow it and ends at the first launch of any other kernel
+    (a flag read's copy, another field's work) or at the next step."""
     starts = [i for i, r in enumerate(trace)
               if r["name"] == step_kernel and (not grid or r["grid"] == grid)]
     groups = []
     for a, b in zip(starts, starts[1:] + [len(trace)]):
-        groups.append([r for r in trace[a:b] if _of_three_kernel_step(r["name"])])
+        g = [trace[a]]
+        for r in trace[a + 1:b]:
+            if r["name"] not in STEP_KERNELS + STEP_AUX_KERNELS:
+                break
+            g.append(r)
+        groups.append(g)
     return groups
-
-
-def _of_three_kernel_step(name):
-    # the field's own kernels, not those of a tile-resident field in the same run
-    return name.startswith("kg::") and not name.startswith("kg::k_rt_")
 
 
 def summarize(a):
@@ -225,11 +230,11 @@ def summarize(a):
                     write += w[k] / wn[k] * 1024 * kw
             what = "three-kernel step (one dispatch of each of its kernels)"
         else:
-            # per step: all k_rt_step dispatches over the steps of the trace
-            # (the counter passes run the same command)
+            # per step: all k_rt_step dispatches of a counter pass over the
+            # steps that pass ran (its own launches, grouped as the trace's)
             ktot_f = sum(v for k, v in f.items() if k[0] == KERNEL)
             ktot_w = sum(v for k, v in w.items() if k[0] == KERNEL)
-            nsteps_f = len(groups)
+            nsteps_f = len(steps_of(load_trace(a.fetch)))
             fetch = ktot_f / nsteps_f * 1024 * kf
             write = ktot_w / nsteps_f * 1024 * kw
             what = f"k_rt_step, all launches of the step, {nsteps_f} steps"
@@ -242,25 +247,23 @@ def summarize(a):
         lines.extend(counter_table(a.counters, kname))
     if three:
         lines.extend(per_step_table(tr, a.step_kernel, a.skip, a.agents,
-                                    getattr(a, "step_grid", 0) or 0))
+                                    getattr(a, "step_grid", 0) or 0, a.count))
     return "\n".join(lines) + "\n"
 
 
-def per_step_table(tr, step_kernel, skip, agents, grid=0):
-    """For the three-kernel path: every launch of `step_kernel` (of `grid`
-    threads, if given) starts a schedule step; each kernel's time summed per
-    step over the steps after the first `skip`."""
-    starts = [r["start"] for r in tr
-              if r["name"] == step_kernel and (not grid or r["grid"] == grid)]
-    if len(starts) <= skip:
+def per_step_table(tr, step_kernel, skip, agents, grid=0, count=0):
+    """For the three-kernel path: each kernel's time summed per step over the
+    steps (three_kernel_steps) from number `skip`, `count` of them (0: all)."""
+    groups = three_kernel_steps(tr, step_kernel, grid)[skip:]
+    if count:
+        groups = groups[:count]
+    if not groups:
         return []
-    starts = starts[skip:]
-    lo = starts[0]
     tot = collections.defaultdict(float)
-    for r in tr:
-        if r["start"] >= lo and _of_three_kernel_step(r["name"]):
+    for g in groups:
+        for r in g:
             tot[r["name"]] += (r["end"] - r["start"]) / 1000.0
-    n = len(starts)
+    n = len(groups)
     out = [f"\nKernel time per schedule step ({step_kernel} launches from number {skip}: "
            f"{n} steps; the kernels run one after another, so the times add up):\n",
            "| kernel | us per step |", "|---|---|"]

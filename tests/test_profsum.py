@@ -5,6 +5,9 @@ from krabmaga_amd import profsum
 
 TRACE_COLS = ["Kernel_Name", "Grid_Size", "Workgroup_Size", "Start_Timestamp", "End_Timestamp"]
 PMC_COLS = ["Kernel_Name", "Grid_Size", "Counter_Name", "Counter_Value"]
+# a counter pass's own launch columns, as rocprofv3 writes them beside the counter
+PMC_TIMED_COLS = ["Kernel_Name", "Grid_Size", "Workgroup_Size", "Start_Timestamp", "End_Timestamp",
+                  "Counter_Name", "Counter_Value"]
 RT = "void kg::k_rt_step<256>(kg::RepDev const*, int)"
 RT512 = "void kg::k_rt_step<512>(kg::RepDev const*, int)"
 
@@ -44,13 +47,18 @@ def test_summary_reports_span_and_traffic(tmp_path):
     t = tmp_path / "t.csv"
     write(t, TRACE_COLS, trace_rows())
     copy = "__amd_rocclr_copyBuffer"
-    fetch = [[copy, 131072, "FETCH_SIZE", 524288.0]]  # a 1 GiB copy reads half in this counter
+    # a 1 GiB copy reads half in this counter
+    fetch = [[copy, 131072, 512, 9_000_000, 9_100_000, "FETCH_SIZE", 524288.0]]
     write_ = [[copy, 131072, "WRITE_SIZE", 1048576.0]]
-    for _ in range(3):
-        fetch += [[RT, 1235200, "FETCH_SIZE", 1000.0], [RT512, 4096, "FETCH_SIZE", 24.0]]
+    # the counter pass ran two steps, the traced run three: traffic is per
+    # step of the counter pass
+    for k in range(2):
+        t0 = 1_000_000 * (k + 1)
+        fetch += [[RT, 1235200, 256, t0, t0 + 120_000, "FETCH_SIZE", 1000.0],
+                  [RT512, 4096, 512, t0 - 2_000, t0 + 150_000, "FETCH_SIZE", 24.0]]
         write_ += [[RT, 1235200, "WRITE_SIZE", 800.0], [RT512, 4096, "WRITE_SIZE", 0.0]]
     fp, wp = tmp_path / "f.csv", tmp_path / "w.csv"
-    write(fp, PMC_COLS, fetch)
+    write(fp, PMC_TIMED_COLS, fetch)
     write(wp, PMC_COLS, write_)
 
     class A:
@@ -99,6 +107,8 @@ def three_kernel_rows():
             rows.append([name, 642304, 256, t, t + dur])
             t += dur
     rows.append(["kg::k_rt_step<512>(kg::RepDev const*)", 4096, 512, 2_500_000, 2_600_000])
+    # another field's work after the last step ends that step
+    rows.append(["kg::k_count_append(kg::Buf)", 1048576, 256, 3_500_000, 3_900_000])
     return rows
 
 
@@ -106,7 +116,9 @@ def test_three_kernel_steps_span_and_traffic(tmp_path):
     t = tmp_path / "t.csv"
     write(t, TRACE_COLS, three_kernel_rows())
     groups = profsum.three_kernel_steps(profsum.load_trace(str(t)), "kg::k_flock")
-    assert [len(g) for g in groups] == [5, 5, 5]  # the k_rt_step launch is not a step kernel
+    # neither the k_rt_step launch nor the k_count_append one is a step kernel
+    assert [len(g) for g in groups] == [5, 5, 5]
+    assert all(g[-1]["end"] - g[0]["start"] == 160_000 for g in groups)
     copy = "__amd_rocclr_copyBuffer"
     fetch = [[copy, 131072, "FETCH_SIZE", 524288.0]]
     write_ = [[copy, 131072, "WRITE_SIZE", 1048576.0]]
