@@ -52,7 +52,7 @@ struct kg_field {
   RepDev* d_rep = nullptr;
   uint32_t* scratch = nullptr;  // small query outputs
   uint32_t* tcount = nullptr;   // per-tile agent counts (tile-resident planning)
-  uint32_t* gflag = nullptr;    // per tile: k_flock left its birds to k_flock_global
+  uint32_t* gflag = nullptr;    // per tile: k_flock left its birds to k_flock_global; + step stamp
   RepDev h_rep{};
 
   uint32_t nR = 0, nW = 0, nA = 0;
@@ -436,7 +436,7 @@ int kg_field_create(const kg_field_desc* desc, kg_field** out) {
   if ((e = f->R2.alloc(f->cap)) != hipSuccess) return fail(e);
   if ((e = f->W.alloc(f->cap)) != hipSuccess) return fail(e);
   if ((e = f->A.alloc(f->cap)) != hipSuccess) return fail(e);
-  if ((e = dmalloc(&f->gflag, std::max<uint32_t>(f->g.ntiles, 1u))) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->gflag, f->g.ntiles + 1u)) != hipSuccess) return fail(e);
   const uint32_t nb = f->g.nbins;
   const uint32_t nscan = (nb + kScanTile - 1) / kScanTile;
   if ((e = dmalloc(&f->count, nb)) != hipSuccess) return fail(e);

@@ -271,7 +271,9 @@ size_t flock_lds_bytes(int cap) { return flock_smem_bytes(cap); }
 // columns (each with its halo) one after another; a tile with an own column
 // whose staged items alone exceed it only sets its flag in `gflag`, and
 // k_flock_global then steps its birds one thread per bird, so a crowded tile
-// is spread over the whole GPU instead of one work-group.
+// is spread over the whole GPU instead of one work-group.  Such a tile also
+// stamps gflag[ntiles] with the step (step + 1, so a zeroed word matches no
+// step), which lets k_flock_global return at once in a step without one.
 __global__ __launch_bounds__(kFlockThreads) void k_flock(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
     const uint32_t* __restrict__ offset, Buf W, uint32_t* __restrict__ count_next,
@@ -312,7 +314,10 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
     for (int32_t c = own_lo; use_lds && c < own_hi; ++c)
       if (flock_chunk_size(rp, ft, s_col_base, c, c + 1) > (uint32_t)lds_cap) use_lds = false;
   }
-  if (t == 0) gflag[item] = use_lds ? 0u : 1u;
+  if (t == 0) {
+    gflag[item] = use_lds ? 0u : 1u;
+    if (!use_lds) gflag[gridDim.x] = (uint32_t)*gstep + 1u;
+  }
   if (use_lds) {
     const FlockSmem S = flock_carve(smem_raw, lds_cap);
     for (int32_t oa = own_lo; oa < own_hi;) {
@@ -348,8 +353,11 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock_global(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix,
     const uint32_t* __restrict__ app_prefix, Buf R, const uint32_t* __restrict__ offset, Buf W,
     uint32_t* __restrict__ count_next, const uint64_t* __restrict__ gstep,
-    uint32_t* __restrict__ err, const uint32_t* __restrict__ gflag, uint32_t n) {
+    uint32_t* __restrict__ err, const uint32_t* __restrict__ gflag, uint32_t ntiles,
+    uint32_t n) {
   __shared__ uint32_t s_any;
+  // no tile took the global path in this step
+  if (gflag[ntiles] != (uint32_t)*gstep + 1u) return;
   const uint32_t g0 = blockIdx.x * kFlockThreads;
   const uint32_t gi = g0 + threadIdx.x;
   // global tile of read-buffer slot q, or false for a bird outside its grid
@@ -391,7 +399,8 @@ hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W
   if (nagents)
     hipLaunchKernelGGL(k_flock_global, dim3((nagents + kFlockThreads - 1) / kFlockThreads),
                        dim3(kFlockThreads), 0, s, wl.reps, wl.nreps, wl.tile_prefix,
-                       wl.app_prefix, R, offset, W, count_next, gstep, err, gflag, nagents);
+                       wl.app_prefix, R, offset, W, count_next, gstep, err, gflag, wl.ntiles,
+                       nagents);
   return hipGetLastError();
 }
 

@@ -25,7 +25,8 @@ hipError_t launch_scatter_append(const WorkList& wl, Buf A, uint32_t* cursor, Bu
 // R's capacity whose contents are lost.
 hipError_t launch_sort_bins(const WorkList& wl, Buf R, const uint32_t* offset, Buf T,
                             hipStream_t s);
-// One replica; gflag: one word per tile (scratch), nagents: R's count.
+// One replica; gflag: one word per tile plus one (scratch, no set-up needed),
+// nagents: R's count.
 hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W,
                         uint32_t* count_next, const uint64_t* gstep, uint32_t* err, int lds_cap,
                         uint32_t* gflag, uint32_t nagents, hipStream_t s);
