@@ -294,7 +294,7 @@ int reserve_binned(kg_ensemble* e, size_t agents, size_t tiles) {
   if (int rc = reserve_buf(e, &e->R2, &e->cap_r2, agents)) return rc;
   if (int rc = reserve_buf(e, &e->W, &e->cap_w, agents)) return rc;
   hipError_t err;
-  if ((err = regrow(&e->gflag, &e->cap_gflag, tiles + 1)) != hipSuccess)
+  if ((err = regrow(&e->gflag, &e->cap_gflag, tiles + 2)) != hipSuccess)
     return set_error(KG_E_NOMEM, "ensemble bin-grouped buffers: %s", hipGetErrorString(err));
   if (!e->gstep && (err = dmalloc(&e->gstep, 1)) != hipSuccess)
     return set_error(KG_E_NOMEM, "ensemble step counter: %s", hipGetErrorString(err));

@@ -26,8 +26,12 @@ namespace {
 struct GraphKey {
   uint8_t params[sizeof(kg_flock_params)];
   int parity;
+  int gather;         // the steps rebin by gather
+  int inline_global;  // global-path tiles stepped inside k_flock
   bool operator<(const GraphKey& o) const {
     if (parity != o.parity) return parity < o.parity;
+    if (gather != o.gather) return gather < o.gather;
+    if (inline_global != o.inline_global) return inline_global < o.inline_global;
     return std::memcmp(params, o.params, sizeof(params)) < 0;
   }
 };
@@ -52,7 +56,25 @@ struct kg_field {
   RepDev* d_rep = nullptr;
   uint32_t* scratch = nullptr;  // small query outputs
   uint32_t* tcount = nullptr;   // per-tile agent counts (tile-resident planning)
-  uint32_t* gflag = nullptr;    // per tile: k_flock left its birds to k_flock_global; + step stamp
+  // per tile: k_flock left its birds to the global path; then the step
+  // stamp and the count of global-path tiles so far
+  uint32_t* gflag = nullptr;
+  // gather rebinning (tiles of at most kGatherMaxBins bins): the flock
+  // kernel's lists of leavers, the gate words (flag, steps redone) and
+  // whether the last flock step filled the lists
+  bool gather_ok = false;
+  Outbox outbox{};
+  uint32_t* gate = nullptr;
+  bool outbox_filled = false;
+  // steps run with the gather; the counts the host last saw (pinned, copied
+  // after every run): [0] steps the gather could not rebin, [1] global-path
+  // tiles; the first gather step of a field is checked on its own
+  uint64_t gather_steps = 0, gather_steps_copied = 0;
+  uint32_t* seen = nullptr;
+  bool gather_probed = false;
+  // no tile has taken the global path since the probe: k_flock steps such a
+  // tile itself and k_flock_global is not launched
+  bool global_inline = false;
   RepDev h_rep{};
 
   uint32_t nR = 0, nW = 0, nA = 0;
@@ -182,6 +204,12 @@ int upload_rep(kg_field* f) {
 // fresh read buffer.  `step_gstep` advances the schedule-step counter.
 int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
   WorkList wl = f->worklist();
+  // a flock step's output with its lists of leavers and nothing appended:
+  // the gather rebins it, the scatter and the bin sort run only if it
+  // sets the gate
+  const bool gather = f->w_aligned && f->outbox_filled && f->nA == 0;
+  uint32_t* gate = gather ? f->gate : nullptr;
+  f->outbox_filled = false;
   // one replica: the append work list needs no device prefix
   if (f->nA) {
     ProfScope ps(f, KG_KERNEL_COUNT_APPEND);
@@ -189,11 +217,16 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
   }
   {
     ProfScope ps(f, KG_KERNEL_SCAN);
-    KG_HIP(launch_scan(f->count, f->offset2, f->cursor, f->g.nbins, f->ss, step_gstep, f->stream));
+    KG_HIP(launch_scan(f->count, f->offset2, f->cursor, f->g.nbins, f->ss, step_gstep, f->stream,
+                       gate));
   }
   if (f->w_aligned) {
     ProfScope ps(f, KG_KERNEL_SCATTER);
-    KG_HIP(launch_scatter_aligned(wl, f->R.b, f->W.b, f->offset, f->cursor, f->R2.b, f->stream));
+    if (gather)
+      KG_HIP(launch_rebin_gather(wl, f->R.b, f->W.b, f->offset, f->offset2, f->outbox,
+                                 f->R2.b, gate, f->stream));
+    KG_HIP(launch_scatter_aligned(wl, f->R.b, f->W.b, f->offset, f->cursor, f->R2.b, f->stream,
+                                  gate));
   }
   if (f->nA) {
     ProfScope ps(f, KG_KERNEL_SCATTER_APPEND);
@@ -202,7 +235,7 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
   {
     // the write buffer has been consumed: it is the sort's scratch
     ProfScope ps(f, KG_KERNEL_SORT_BINS);
-    KG_HIP(launch_sort_bins(wl, f->R2.b, f->offset2, f->W.b, f->stream));
+    KG_HIP(launch_sort_bins(wl, f->R2.b, f->offset2, f->W.b, f->stream, gate));
   }
   std::swap(f->R, f->R2);
   std::swap(f->offset, f->offset2);
@@ -221,7 +254,9 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
 int flock_impl(kg_field* f, const uint64_t* gstep) {
   ProfScope ps(f, KG_KERNEL_FLOCK);
   KG_HIP(launch_flock(f->worklist(), f->R.b, f->offset, f->W.b, f->count, gstep, f->err,
-                      f->lds_cap, f->gflag, (uint32_t)f->nR, f->stream));
+                      f->lds_cap, f->gflag, (uint32_t)f->nR, f->stream,
+                      f->gather_ok ? f->outbox : Outbox{}, f->global_inline));
+  f->outbox_filled = f->gather_ok;
   f->nW = f->nR;
   f->w_aligned = true;
   return KG_OK;
@@ -237,6 +272,16 @@ int set_params(kg_field* f, const kg_flock_params* p, uint64_t step_base) {
 }
 
 // ---- tile-resident layout -------------------------------------------------
+
+// The gather rebinning after flock steps; KG_REBIN_GATHER=0 turns it off
+// (scatter and bin sort every step), for comparison.
+bool rebin_gather_enabled() {
+  static const bool v = [] {
+    const char* e = std::getenv("KG_REBIN_GATHER");
+    return !(e && e[0] == '0');
+  }();
+  return v;
+}
 
 // Deferred tiles in one step that ask for a new plan; KG_RT_REPLAN_DEFERRED
 // overrides it for tuning.
@@ -436,7 +481,7 @@ int kg_field_create(const kg_field_desc* desc, kg_field** out) {
   if ((e = f->R2.alloc(f->cap)) != hipSuccess) return fail(e);
   if ((e = f->W.alloc(f->cap)) != hipSuccess) return fail(e);
   if ((e = f->A.alloc(f->cap)) != hipSuccess) return fail(e);
-  if ((e = dmalloc(&f->gflag, f->g.ntiles + 1u)) != hipSuccess) return fail(e);
+  if ((e = dmalloc(&f->gflag, f->g.ntiles + 2u)) != hipSuccess) return fail(e);
   const uint32_t nb = f->g.nbins;
   const uint32_t nscan = (nb + kScanTile - 1) / kScanTile;
   if ((e = dmalloc(&f->count, nb)) != hipSuccess) return fail(e);
@@ -450,11 +495,23 @@ int kg_field_create(const kg_field_desc* desc, kg_field** out) {
   if ((e = dmalloc(&f->prefix, 4)) != hipSuccess) return fail(e);
   if ((e = dmalloc(&f->d_rep, 1)) != hipSuccess) return fail(e);
   if ((e = dmalloc(&f->scratch, 4)) != hipSuccess) return fail(e);
+  f->gather_ok = rebin_gather_enabled() && (uint32_t)(f->g.tx * f->g.tr) <= kGatherMaxBins;
+  if (f->gather_ok) {
+    if ((e = dmalloc(&f->outbox.n, f->g.ntiles)) != hipSuccess) return fail(e);
+    if ((e = dmalloc(&f->outbox.slot, (size_t)f->g.ntiles * kOutboxCap)) != hipSuccess)
+      return fail(e);
+    if ((e = dmalloc(&f->gate, 2)) != hipSuccess) return fail(e);
+    if ((e = hipMemset(f->gate, 0, 2 * sizeof(uint32_t))) != hipSuccess) return fail(e);
+    if ((e = hipHostMalloc((void**)&f->seen, 2 * sizeof(uint32_t))) != hipSuccess)
+      return fail(e);
+    f->seen[0] = f->seen[1] = 0;
+  }
   // one-time clears (never inside a captured graph)
   const std::pair<void*, size_t> clears[] = {
       {f->count, sizeof(uint32_t) * nb},          {f->offset, sizeof(uint32_t) * (nb + 1)},
       {f->offset2, sizeof(uint32_t) * (nb + 1)},  {f->ss.partial, sizeof(uint32_t) * (nscan + 1)},
       {f->err, sizeof(uint32_t)},                 {f->gstep, sizeof(uint64_t)},
+      {f->gflag, sizeof(uint32_t) * (f->g.ntiles + 2u)},
       {f->zero64, sizeof(uint64_t)}};
   for (const auto& c : clears)
     if ((e = hipMemset(c.first, 0, c.second)) != hipSuccess) return fail(e);
@@ -494,6 +551,10 @@ void kg_field_destroy(kg_field* f) {
   (void)hipFree(f->scratch);
   (void)hipFree(f->tcount);
   (void)hipFree(f->gflag);
+  (void)hipFree(f->outbox.n);
+  (void)hipFree(f->outbox.slot);
+  (void)hipFree(f->gate);
+  if (f->seen) (void)hipHostFree(f->seen);
   if (f->stream) (void)hipStreamDestroy(f->stream);
   delete f;
 }
@@ -936,12 +997,39 @@ int kg_schedule_run(kg_field* f, const kg_flock_params* p, uint64_t first_step, 
   }
   if (int rc = ensure_atomic(f)) return rc;
   if (int rc = set_params(f, p, first_step)) return rc;
+  // a field whose steps fail the gather (tiles too full, long jumps) pays
+  // for it and for the scatter and the sort: it rebins by scatter and sort
+  // from then on.  The first gather step is checked at once (one
+  // synchronisation per field), later ones from the count copied after each
+  // run, which may lag a run behind: it is only a hint.
+  // Likewise a field whose tiles took the global path (counted by k_flock)
+  // goes back to launching k_flock_global.
+  const volatile uint32_t* seen = f->seen;
+  if (f->gather_ok && f->gather_steps_copied >= 16 &&
+      16ull * seen[0] >= f->gather_steps_copied)
+    f->gather_ok = false;
+  if (f->global_inline && seen[1] != 0) f->global_inline = false;
   KG_HIP(hipMemsetAsync(f->gstep, 0, sizeof(uint64_t), f->stream));
   uint32_t left = nsteps;
+  if (f->gather_ok && !f->gather_probed) {
+    if (int rc = run_direct_step(f)) return rc;
+    --left;
+    uint32_t counts[2] = {0, 0};
+    KG_HIP(hipMemcpyAsync(&counts[0], f->gate + 1, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          f->stream));
+    KG_HIP(hipMemcpyAsync(&counts[1], f->gflag + f->g.ntiles + 1, sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, f->stream));
+    KG_HIP(hipStreamSynchronize(f->stream));
+    f->gather_probed = true;
+    if (counts[0]) f->gather_ok = false;
+    f->global_inline = counts[1] == 0;
+  }
   if ((use_graph & KG_RUN_GRAPH) && left >= 2 && !f->prof_on) {
     GraphKey key;
     std::memcpy(key.params, p, sizeof(kg_flock_params));
     key.parity = f->parity;
+    key.gather = f->gather_ok ? 1 : 0;
+    key.inline_global = f->global_inline ? 1 : 0;
     auto it = f->graphs.find(key);
     hipGraphExec_t exec = nullptr;
     if (it == f->graphs.end()) {
@@ -971,6 +1059,15 @@ int kg_schedule_run(kg_field* f, const kg_flock_params* p, uint64_t first_step, 
     if (int rc = run_direct_step(f)) return rc;
     --left;
   }
+  if (f->gather_ok) {
+    f->gather_steps += nsteps;
+    KG_HIP(hipMemcpyAsync(f->seen, f->gate + 1, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          f->stream));
+    f->gather_steps_copied = f->gather_steps;
+  }
+  if (f->global_inline)
+    KG_HIP(hipMemcpyAsync(f->seen + 1, f->gflag + f->g.ntiles + 1, sizeof(uint32_t),
+                          hipMemcpyDeviceToHost, f->stream));
   return KG_OK;
 }
 

@@ -88,61 +88,174 @@ __global__ __launch_bounds__(256) void k_scatter_aligned(const RepDev* __restric
                                                          const uint32_t* __restrict__ tile_prefix,
                                                          Buf R, Buf W,
                                                          const uint32_t* __restrict__ offset_cur,
-                                                         uint32_t* __restrict__ cursor, Buf R2) {
-  const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
-  const TileRange tr = tile_range(reps, nreps, tile_prefix, item, offset_cur);
-  const RepDev& rp = reps[tr.r];
-  const Geom& g = rp.g;
-  const uint32_t lane = threadIdx.x & (kWave - 1);
-  // The read buffer is in bin order and birds move little, so neighbouring
-  // lanes mostly share a destination bin: one cursor atomic per run of
-  // equal bins in a wave (the bin sort that follows fixes the order inside
-  // a bin).  Every lane takes part in every round for the wave operations.
-  for (uint32_t i0 = tr.lo; i0 < tr.hi; i0 += blockDim.x) {
-    const uint32_t i = i0 + threadIdx.x;
-    bool valid = i < tr.hi;
-    float x = 0.0f, y = 0.0f;
-    uint32_t bin = 0xFFFFFFFFu;
-    if (valid) {
-      x = W.x[i];
-      y = W.y[i];
-      int32_t cx, cy;
-      discretize(g, x, y, &cx, &cy);
-      valid = cell_valid(g, cx, cy);  // an invalid cell was reported by the flock kernel
-      if (valid) bin = rp.bin_base + bin_of(g, cx, cy);
-    }
-    const uint32_t prev = __shfl_up(bin, 1, kWave);
-    const bool head = valid && (lane == 0 || prev != bin);
-    const uint64_t heads = __ballot(head);
-    const uint64_t vmask = __ballot(valid);
-    const uint64_t upto = lane == kWave - 1 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-    const uint64_t below = heads & upto;
-    const int h = below ? 63 - __clzll((long long)below) : 0;
-    // the run of head h ends at the next head
-    const uint64_t after = heads & ~upto;
-    const int nh = after ? __ffsll((long long)after) - 1 : kWave;
-    const uint64_t from_h = ~((1ull << h) - 1ull);
-    const uint64_t to_nh = nh == kWave ? ~0ull : ((1ull << nh) - 1ull);
-    uint32_t base = 0;
-    if (head) base = atomicAdd(&cursor[bin], (uint32_t)__popcll(vmask & from_h & to_nh));
-    base = __shfl(base, h, kWave);
-    if (valid) {
-      // base plus the valid lanes of the run before this one
-      const uint32_t pos = base + (uint32_t)__popcll(vmask & from_h & (upto >> 1));
-      R2.x[pos] = x;
-      R2.y[pos] = y;
-      R2.hx[pos] = W.hx[i];
-      R2.hy[pos] = W.hy[i];
-      R2.id[pos] = R.id[i];
+                                                         uint32_t* __restrict__ cursor, Buf R2,
+                                                         const uint32_t* __restrict__ gate,
+                                                         uint32_t nitems) {
+  if (gate && gate[0] == 0u) return;  // the gather rebinned this step
+  for (uint32_t bi = blockIdx.x; bi < nitems; bi += gridDim.x) {
+    const uint32_t item = xcd_remap(bi, nitems);
+    const TileRange tr = tile_range(reps, nreps, tile_prefix, item, offset_cur);
+    const RepDev& rp = reps[tr.r];
+    const Geom& g = rp.g;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    // The read buffer is in bin order and birds move little, so neighbouring
+    // lanes mostly share a destination bin: one cursor atomic per run of
+    // equal bins in a wave (the bin sort that follows fixes the order inside
+    // a bin).  Every lane takes part in every round for the wave operations.
+    for (uint32_t i0 = tr.lo; i0 < tr.hi; i0 += blockDim.x) {
+      const uint32_t i = i0 + threadIdx.x;
+      bool valid = i < tr.hi;
+      float x = 0.0f, y = 0.0f;
+      uint32_t bin = 0xFFFFFFFFu;
+      if (valid) {
+        x = W.x[i];
+        y = W.y[i];
+        int32_t cx, cy;
+        discretize(g, x, y, &cx, &cy);
+        valid = cell_valid(g, cx, cy);  // an invalid cell was reported by the flock kernel
+        if (valid) bin = rp.bin_base + bin_of(g, cx, cy);
+      }
+      const uint32_t prev = __shfl_up(bin, 1, kWave);
+      const bool head = valid && (lane == 0 || prev != bin);
+      const uint64_t heads = __ballot(head);
+      const uint64_t vmask = __ballot(valid);
+      const uint64_t upto = lane == kWave - 1 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+      const uint64_t below = heads & upto;
+      const int h = below ? 63 - __clzll((long long)below) : 0;
+      // the run of head h ends at the next head
+      const uint64_t after = heads & ~upto;
+      const int nh = after ? __ffsll((long long)after) - 1 : kWave;
+      const uint64_t from_h = ~((1ull << h) - 1ull);
+      const uint64_t to_nh = nh == kWave ? ~0ull : ((1ull << nh) - 1ull);
+      uint32_t base = 0;
+      if (head) base = atomicAdd(&cursor[bin], (uint32_t)__popcll(vmask & from_h & to_nh));
+      base = __shfl(base, h, kWave);
+      if (valid) {
+        // base plus the valid lanes of the run before this one
+        const uint32_t pos = base + (uint32_t)__popcll(vmask & from_h & (upto >> 1));
+        R2.x[pos] = x;
+        R2.y[pos] = y;
+        R2.hx[pos] = W.hx[i];
+        R2.hy[pos] = W.hy[i];
+        R2.id[pos] = R.id[i];
+      }
     }
   }
 }
 
+// Work-groups of a gated scatter or bin sort: in most steps they only read
+// the gate (a launch costs in proportion to its work-groups), and in a step
+// they redo they loop over the tiles.
+constexpr uint32_t kGatedGrid = 256;
+
 hipError_t launch_scatter_aligned(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
-                                  uint32_t* cursor, Buf R2, hipStream_t s) {
+                                  uint32_t* cursor, Buf R2, hipStream_t s, const uint32_t* gate) {
   if (wl.ntiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_scatter_aligned, dim3(wl.ntiles), dim3(256), 0, s, wl.reps, wl.nreps,
-                     wl.tile_prefix, R, W, offset_cur, cursor, R2);
+  const uint32_t grid = gate && wl.ntiles > kGatedGrid ? kGatedGrid : wl.ntiles;
+  hipLaunchKernelGGL(k_scatter_aligned, dim3(grid), dim3(256), 0, s, wl.reps, wl.nreps,
+                     wl.tile_prefix, R, W, offset_cur, cursor, R2, gate, wl.ntiles);
+  return hipGetLastError();
+}
+
+// The gather rebinning: one work-group per tile, phases in kg_phases.hpp.
+// Each thread keeps its candidates in registers from load to store, so a
+// bird is read once and written once; a tile with more candidates (or new
+// birds) than kGatherCap sends the step to the scatter and the bin sort.
+__global__ __launch_bounds__(kGatherThreads) void k_rebin_gather(
+    const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
+    Buf W, const uint32_t* __restrict__ offset_cur, const uint32_t* __restrict__ offset_new,
+    Outbox ob, Buf R2, uint32_t* __restrict__ gate) {
+  __shared__ uint32_t s_bstart[kGatherMaxBins + 1];
+  __shared__ uint32_t s_fill[kGatherMaxBins];
+  __shared__ uint64_t s_key[kGatherCap];
+  __shared__ uint32_t s_sitem[kGatherSources], s_scnt[kGatherSources];
+  __shared__ uint32_t s_fail;
+  const uint32_t t = threadIdx.x;
+  const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
+  const int r = find_rep(tile_prefix, nreps, item);
+  const RepDev& rp = reps[r];
+  const Geom& g = rp.g;
+  const uint32_t tile = item - tile_prefix[r];
+  const uint32_t nbt = 1u << (g.tx_log2 + g.tr_log2);
+  const uint32_t tb0 = rp.bin_base + tile * nbt;
+  const uint32_t lo = offset_cur[tb0], nown = offset_cur[tb0 + nbt] - lo;
+  const uint32_t nlo = offset_new[tb0];
+  for (uint32_t b = t; b <= nbt; b += kGatherThreads) s_bstart[b] = offset_new[tb0 + b] - nlo;
+  if (t < kGatherSources) {
+    const int32_t sl = gather_source(g, tile, t);
+    const uint32_t it = sl >= 0 ? tile_prefix[r] + (uint32_t)sl : 0u;
+    s_sitem[t] = it;
+    s_scnt[t] = sl >= 0 ? ob.n[it] : 0u;
+  }
+  if (t == 0) s_fail = 0u;
+  __syncthreads();
+  uint32_t ncand = nown;
+  for (uint32_t q = 0; q < kGatherSources; ++q) ncand += s_scnt[q];
+  if (t == 0 && (ncand > kGatherCap || s_bstart[nbt] > kGatherCap)) s_fail = 1u;
+  if (t < kGatherSources && s_scnt[t] > kOutboxCap) s_fail = 1u;
+  for (uint32_t b = t; b < nbt; b += kGatherThreads) {
+    if (s_bstart[b + 1] - s_bstart[b] > kGatherBinMax) s_fail = 1u;
+    s_fill[b] = 0u;
+  }
+  __syncthreads();
+  // s_fail is read only between barriers, so every thread takes the same path
+  if (!s_fail) {
+    // per candidate: (bin << 16 | slot in the bin) or ~0, and the bird
+    uint32_t cbs[kGatherRegs], cid[kGatherRegs];
+    float cx[kGatherRegs], cy[kGatherRegs], chx[kGatherRegs], chy[kGatherRegs];
+#pragma unroll
+    for (uint32_t u = 0; u < kGatherRegs; ++u) {
+      cbs[u] = ~0u;
+      const uint32_t k = t + u * kGatherThreads;
+      if (k < ncand) {
+        const uint32_t src = gather_src(ob, s_sitem, s_scnt, lo, nown, k);
+        cx[u] = W.x[src];
+        cy[u] = W.y[src];
+        chx[u] = W.hx[src];
+        chy[u] = W.hy[src];
+        cid[u] = R.id[src];
+        int32_t row;
+        const uint32_t b = gather_bin(g, tile, cx[u], cy[u], &row);
+        if (b < nbt) {
+          const uint32_t slot = atom_add(&s_fill[b], 1u);
+          if (slot < s_bstart[b + 1] - s_bstart[b]) {
+            s_key[s_bstart[b] + slot] = ((uint64_t)(uint32_t)row << 32) | cid[u];
+            cbs[u] = (b << 16) | slot;
+          } else {
+            s_fail = 1u;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t b = t; b < nbt; b += kGatherThreads)
+      if (s_fill[b] != s_bstart[b + 1] - s_bstart[b]) s_fail = 1u;
+    __syncthreads();
+    if (!s_fail) {
+#pragma unroll
+      for (uint32_t u = 0; u < kGatherRegs; ++u) {
+        if (cbs[u] == ~0u) continue;
+        const uint32_t b = cbs[u] >> 16;
+        const uint32_t e = s_bstart[b] + (cbs[u] & 0xFFFFu);
+        const uint32_t pos = nlo + s_bstart[b] + gather_rank(s_key, s_bstart[b], s_bstart[b + 1], e,
+                                                             s_key[e]);
+        R2.x[pos] = cx[u];
+        R2.y[pos] = cy[u];
+        R2.hx[pos] = chx[u];
+        R2.hy[pos] = chy[u];
+        R2.id[pos] = cid[u];
+      }
+    }
+  }
+  if (t == 0 && s_fail) gate[0] = 1u;
+}
+
+hipError_t launch_rebin_gather(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
+                               const uint32_t* offset_new, const Outbox& ob, Buf R2,
+                               uint32_t* gate, hipStream_t s) {
+  if (wl.ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rebin_gather, dim3(wl.ntiles), dim3(kGatherThreads), 0, s, wl.reps,
+                     wl.nreps, wl.tile_prefix, R, W, offset_cur, offset_new, ob, R2, gate);
   return hipGetLastError();
 }
 
@@ -213,7 +326,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
                                                             uint32_t* __restrict__ offset,
                                                             uint32_t* __restrict__ cursor, uint32_t n,
                                                             const uint32_t* __restrict__ partial,
-                                                            int prefixed, uint64_t* gstep) {
+                                                            int prefixed, uint64_t* gstep,
+                                                            uint32_t* gate) {
   __shared__ uint32_t s_warp[kScanThreads / kWave];
   const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   // the tiles before this one (at most kScanInlineTiles sums, cached)
@@ -245,11 +359,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
     run += v[k];
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) offset[n] = tile_before + total;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && gstep) *gstep += 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (gstep) *gstep += 1;
+    if (gate) *gate = 0u;
+  }
 }
 
 hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint32_t nbins,
-                       const ScanState& ss, uint64_t* gstep, hipStream_t s) {
+                       const ScanState& ss, uint64_t* gstep, hipStream_t s, uint32_t* gate) {
   const uint32_t ntiles = (nbins + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(kScanThreads), 0, s, count, nbins,
                      ss.partial);
@@ -257,7 +374,7 @@ hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint
   if (prefixed)
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, ss.partial, ntiles);
   hipLaunchKernelGGL(k_scan_down, dim3(ntiles), dim3(kScanThreads), 0, s, count, offset, cursor,
-                     nbins, ss.partial, prefixed, gstep);
+                     nbins, ss.partial, prefixed, gstep, gate);
   return hipGetLastError();
 }
 
@@ -269,22 +386,25 @@ size_t flock_lds_bytes(int cap) { return flock_smem_bytes(cap); }
 
 // A tile whose staged set exceeds the LDS capacity runs in chunks of own
 // columns (each with its halo) one after another; a tile with an own column
-// whose staged items alone exceed it only sets its flag in `gflag`, and
-// k_flock_global then steps its birds one thread per bird, so a crowded tile
-// is spread over the whole GPU instead of one work-group.  Such a tile also
-// stamps gflag[ntiles] with the step (step + 1, so a zeroed word matches no
-// step), which lets k_flock_global return at once in a step without one.
+// whose staged items alone exceed it takes the global path: it sets its flag
+// in `gflag`, and k_flock_global then steps its birds one thread per bird, so
+// a crowded tile is spread over the whole GPU instead of one work-group.
+// Such a tile also stamps gflag[ntiles] with the step (step + 1, so a zeroed
+// word matches no step), which lets k_flock_global return at once in a step
+// without one, and counts itself in gflag[ntiles + 1].  With inline_global
+// (no k_flock_global launch) the work-group steps its tile's birds itself.
 __global__ __launch_bounds__(kFlockThreads) void k_flock(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
     const uint32_t* __restrict__ offset, Buf W, uint32_t* __restrict__ count_next,
     const uint64_t* __restrict__ gstep, uint32_t* __restrict__ err, int lds_cap,
-    uint32_t* __restrict__ gflag) {
+    uint32_t* __restrict__ gflag, Outbox ob, int inline_global) {
   extern __shared__ __align__(16) uint8_t smem_raw[];
   __shared__ uint32_t s_seg_start[3 * kMaxStageCols];
   __shared__ uint32_t s_seg_len[3 * kMaxStageCols];
   __shared__ uint32_t s_col_base[kMaxStageCols + 1];
   __shared__ uint32_t s_warp[kFlockThreads / kWave];
   __shared__ uint32_t s_nown;
+  __shared__ uint32_t s_outn;
 
   const int t = threadIdx.x;
   const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
@@ -302,6 +422,7 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
     if (t == 0) {
       s_col_base[ft.ncols] = total;
       s_nown = 0;
+      s_outn = 0;
     }
     L = total;
     __syncthreads();
@@ -316,9 +437,14 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
   }
   if (t == 0) {
     gflag[item] = use_lds ? 0u : 1u;
-    if (!use_lds) gflag[gridDim.x] = (uint32_t)*gstep + 1u;
+    if (!use_lds) {
+      gflag[gridDim.x] = (uint32_t)*gstep + 1u;
+      atomicAdd(&gflag[gridDim.x + 1], 1u);
+      if (ob.n) ob.n[item] = 0u;  // the global path lists the leavers
+    }
   }
   if (use_lds) {
+    const Leavers lv{ob, item - ft.tile, &s_outn};
     const FlockSmem S = flock_carve(smem_raw, lds_cap);
     for (int32_t oa = own_lo; oa < own_hi;) {
       const int32_t ob = L <= (uint32_t)lds_cap
@@ -334,7 +460,7 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
       __syncthreads();
       const uint32_t nown = s_nown;
       for (uint32_t o = t; o < nown; o += kFlockThreads)
-        flock_agent_lds(rp, ft, S.own[o], s_col_base, base, S, W, count_next, err);
+        flock_agent_lds(rp, ft, S.own[o], s_col_base, base, S, W, count_next, err, lv);
       oa = ob;
       if (oa < own_hi) {
         // the next chunk reuses the LDS
@@ -343,18 +469,31 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
         __syncthreads();
       }
     }
+    if (ob.n) {
+      __syncthreads();
+      if (t == 0) ob.n[item] = s_outn;
+    }
+  } else if (inline_global) {
+    __syncthreads();  // the outbox count is cleared
+    const uint32_t lo = offset[ft.tb0], hi = offset[ft.tb1];
+    const Leavers lv{ob, item - ft.tile, nullptr};
+    for (uint32_t gi = lo + (uint32_t)t; gi < hi; gi += kFlockThreads)
+      flock_agent_global(rp, ft.step, gi, R, offset, W, count_next, err, lv);
   }
 }
 
 // The birds of flagged tiles, one thread per read-buffer slot.  The read
 // buffer holds the replicas one after another (replica r at
 // app_prefix[r], its tiles one after another), each tile's bins in order.
+// kLeavers: outboxes are filled (compiled apart: the listing costs the walk
+// registers).
+template <bool kLeavers>
 __global__ __launch_bounds__(kFlockThreads) void k_flock_global(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix,
     const uint32_t* __restrict__ app_prefix, Buf R, const uint32_t* __restrict__ offset, Buf W,
     uint32_t* __restrict__ count_next, const uint64_t* __restrict__ gstep,
     uint32_t* __restrict__ err, const uint32_t* __restrict__ gflag, uint32_t ntiles,
-    uint32_t n) {
+    uint32_t n, Outbox ob) {
   __shared__ uint32_t s_any;
   // no tile took the global path in this step
   if (gflag[ntiles] != (uint32_t)*gstep + 1u) return;
@@ -385,22 +524,26 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock_global(
   if (!s_any || gi >= n) return;
   uint32_t t;
   if (!tile_at(gi, &t) || !gflag[t]) return;
-  const RepDev& rp = reps[nreps == 1 ? 0 : find_rep(app_prefix, nreps, gi)];
-  flock_agent_global(rp, rp.step_base + *gstep, gi, R, offset, W, count_next, err);
+  const int r = nreps == 1 ? 0 : find_rep(app_prefix, nreps, gi);
+  const RepDev& rp = reps[r];
+  flock_agent_global(rp, rp.step_base + *gstep, gi, R, offset, W, count_next, err,
+                     Leavers{kLeavers ? ob : Outbox{}, tile_prefix[r], nullptr});
 }
 
 hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W,
                         uint32_t* count_next, const uint64_t* gstep, uint32_t* err, int lds_cap,
-                        uint32_t* gflag, uint32_t nagents, hipStream_t s) {
+                        uint32_t* gflag, uint32_t nagents, hipStream_t s, const Outbox& ob,
+                        bool inline_global) {
   if (wl.ntiles == 0) return hipSuccess;
   hipLaunchKernelGGL(k_flock, dim3(wl.ntiles), dim3(kFlockThreads), flock_lds_bytes(lds_cap), s,
                      wl.reps, wl.nreps, wl.tile_prefix, R, offset, W, count_next, gstep, err,
-                     lds_cap, gflag);
-  if (nagents)
-    hipLaunchKernelGGL(k_flock_global, dim3((nagents + kFlockThreads - 1) / kFlockThreads),
+                     lds_cap, gflag, ob, inline_global ? 1 : 0);
+  if (nagents && !inline_global)
+    hipLaunchKernelGGL(ob.n ? k_flock_global<true> : k_flock_global<false>,
+                       dim3((nagents + kFlockThreads - 1) / kFlockThreads),
                        dim3(kFlockThreads), 0, s, wl.reps, wl.nreps, wl.tile_prefix,
                        wl.app_prefix, R, offset, W, count_next, gstep, err, gflag, wl.ntiles,
-                       nagents);
+                       nagents, ob);
   return hipGetLastError();
 }
 
@@ -578,7 +721,9 @@ __device__ __forceinline__ void sort_wave_bin(const Geom& g, Buf R, uint32_t s, 
 
 __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
-    const uint32_t* __restrict__ offset, Buf T) {
+    const uint32_t* __restrict__ offset, Buf T, uint32_t* __restrict__ gate, uint32_t nitems) {
+  if (gate && gate[0] == 0u) return;  // the gather rebinned this step
+  if (gate && blockIdx.x == 0 && threadIdx.x == 0) gate[1] += 1u;  // steps redone
   __shared__ uint64_t s_key[kSortChunk];
   __shared__ uint32_t s_seg[kSortThreads];
   __shared__ uint32_t s_mid[kSortThreads];
@@ -587,102 +732,105 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
   const int t = threadIdx.x;
   const uint32_t lane = (uint32_t)t & (kWave - 1);
   const uint32_t wid = (uint32_t)t / kWave;
-  const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
-  const int r = find_rep(tile_prefix, nreps, item);
-  const Geom g = reps[r].g;
-  const uint32_t nb = 1u << (g.tx_log2 + g.tr_log2);
-  const uint32_t b0 = reps[r].bin_base + ((item - tile_prefix[r]) << (g.tx_log2 + g.tr_log2));
+  for (uint32_t bi = blockIdx.x; bi < nitems; bi += gridDim.x) {
+    const uint32_t item = xcd_remap(bi, nitems);
+    const int r = find_rep(tile_prefix, nreps, item);
+    const Geom g = reps[r].g;
+    const uint32_t nb = 1u << (g.tx_log2 + g.tr_log2);
+    const uint32_t b0 = reps[r].bin_base + ((item - tile_prefix[r]) << (g.tx_log2 + g.tr_log2));
 
-  for (uint32_t k0 = 0; k0 < nb; k0 += kSortThreads) {
-    if (t == 0) {
-      s_nseg = 0u;
-      s_nmid = 0u;
-      s_nbig = 0u;
-    }
-    __syncthreads();
-    if (k0 + (uint32_t)t < nb) {
-      const uint32_t b = b0 + k0 + (uint32_t)t;
-      const uint32_t s = offset[b], m = offset[b + 1] - s;
-      if (m >= 2u && m <= kSortSmall) sort_small_bin(g, R, s, m);
-      else if (m > kSortSmall && m <= kSortSegMax) s_seg[atomicAdd(&s_nseg, 1u)] = b;
-      else if (m > kSortSegMax && m <= kSortWaveMax) s_mid[atomicAdd(&s_nmid, 1u)] = b;
-      else if (m > kSortWaveMax) s_big[atomicAdd(&s_nbig, 1u)] = b;
-    }
-    __syncthreads();
-    // bins of up to 64: a 16-lane segment each, four per wave
-    const uint32_t nseg = s_nseg;
-    constexpr uint32_t kSegsPerWave = kWave / kSortSeg;
-    for (uint32_t i0 = wid * kSegsPerWave; i0 < nseg; i0 += kSortThreads / kSortSeg) {
-      const uint32_t i = i0 + lane / kSortSeg;
-      uint32_t s = 0, m = 0;
-      if (i < nseg) {
-        s = offset[s_seg[i]];
-        m = offset[s_seg[i] + 1] - s;
+    for (uint32_t k0 = 0; k0 < nb; k0 += kSortThreads) {
+      if (t == 0) {
+        s_nseg = 0u;
+        s_nmid = 0u;
+        s_nbig = 0u;
       }
-      sort_seg_bins(g, R, s, m, lane);
-    }
-    // middle bins: one wave each
-    const uint32_t nmid = s_nmid;
-    for (uint32_t i = wid; i < nmid; i += kSortThreads / kWave) {
-      const uint32_t s = offset[s_mid[i]];
-      sort_wave_bin(g, R, s, offset[s_mid[i] + 1] - s, lane);
-    }
-    // larger bins: the whole block
-    const uint32_t nbig = s_nbig;
-    for (uint32_t i = 0; i < nbig; ++i) {
-      const uint32_t s = offset[s_big[i]];
-      const uint32_t m = offset[s_big[i] + 1] - s;
-      const bool staged_once = m <= kSortChunk;
-      if (staged_once) {
-        for (uint32_t j = t; j < m; j += kSortThreads) s_key[j] = bin_key(g, R.y[s + j], R.id[s + j]);
+      __syncthreads();
+      if (k0 + (uint32_t)t < nb) {
+        const uint32_t b = b0 + k0 + (uint32_t)t;
+        const uint32_t s = offset[b], m = offset[b + 1] - s;
+        if (m >= 2u && m <= kSortSmall) sort_small_bin(g, R, s, m);
+        else if (m > kSortSmall && m <= kSortSegMax) s_seg[atomicAdd(&s_nseg, 1u)] = b;
+        else if (m > kSortSegMax && m <= kSortWaveMax) s_mid[atomicAdd(&s_nmid, 1u)] = b;
+        else if (m > kSortWaveMax) s_big[atomicAdd(&s_nbig, 1u)] = b;
+      }
+      __syncthreads();
+      // bins of up to 64: a 16-lane segment each, four per wave
+      const uint32_t nseg = s_nseg;
+      constexpr uint32_t kSegsPerWave = kWave / kSortSeg;
+      for (uint32_t i0 = wid * kSegsPerWave; i0 < nseg; i0 += kSortThreads / kSortSeg) {
+        const uint32_t i = i0 + lane / kSortSeg;
+        uint32_t s = 0, m = 0;
+        if (i < nseg) {
+          s = offset[s_seg[i]];
+          m = offset[s_seg[i] + 1] - s;
+        }
+        sort_seg_bins(g, R, s, m, lane);
+      }
+      // middle bins: one wave each
+      const uint32_t nmid = s_nmid;
+      for (uint32_t i = wid; i < nmid; i += kSortThreads / kWave) {
+        const uint32_t s = offset[s_mid[i]];
+        sort_wave_bin(g, R, s, offset[s_mid[i] + 1] - s, lane);
+      }
+      // larger bins: the whole block
+      const uint32_t nbig = s_nbig;
+      for (uint32_t i = 0; i < nbig; ++i) {
+        const uint32_t s = offset[s_big[i]];
+        const uint32_t m = offset[s_big[i] + 1] - s;
+        const bool staged_once = m <= kSortChunk;
+        if (staged_once) {
+          for (uint32_t j = t; j < m; j += kSortThreads) s_key[j] = bin_key(g, R.y[s + j], R.id[s + j]);
+          __syncthreads();
+        }
+        for (uint32_t e0 = 0; e0 < m; e0 += kSortThreads) {
+          const uint32_t e = e0 + (uint32_t)t;
+          const bool act = e < m;
+          const uint64_t key = act ? bin_key(g, R.y[s + e], R.id[s + e]) : 0ull;
+          uint32_t rank = 0;
+          for (uint32_t c0 = 0; c0 < m; c0 += kSortChunk) {
+            const uint32_t n = m - c0 < kSortChunk ? m - c0 : kSortChunk;
+            if (!staged_once) {
+              __syncthreads();
+              for (uint32_t j = t; j < n; j += kSortThreads)
+                s_key[j] = bin_key(g, R.y[s + c0 + j], R.id[s + c0 + j]);
+              __syncthreads();
+            }
+            if (act)
+              for (uint32_t j = 0; j < n; ++j) {
+                const uint64_t kj = s_key[j];
+                rank += (kj < key) || (kj == key && c0 + j < e);
+              }
+          }
+          if (act) {
+            T.id[s + rank] = R.id[s + e];
+            T.x[s + rank] = R.x[s + e];
+            T.y[s + rank] = R.y[s + e];
+            T.hx[s + rank] = R.hx[s + e];
+            T.hy[s + rank] = R.hy[s + e];
+          }
+        }
+        __syncthreads();
+        for (uint32_t j = t; j < m; j += kSortThreads) {
+          R.id[s + j] = T.id[s + j];
+          R.x[s + j] = T.x[s + j];
+          R.y[s + j] = T.y[s + j];
+          R.hx[s + j] = T.hx[s + j];
+          R.hy[s + j] = T.hy[s + j];
+        }
         __syncthreads();
       }
-      for (uint32_t e0 = 0; e0 < m; e0 += kSortThreads) {
-        const uint32_t e = e0 + (uint32_t)t;
-        const bool act = e < m;
-        const uint64_t key = act ? bin_key(g, R.y[s + e], R.id[s + e]) : 0ull;
-        uint32_t rank = 0;
-        for (uint32_t c0 = 0; c0 < m; c0 += kSortChunk) {
-          const uint32_t n = m - c0 < kSortChunk ? m - c0 : kSortChunk;
-          if (!staged_once) {
-            __syncthreads();
-            for (uint32_t j = t; j < n; j += kSortThreads)
-              s_key[j] = bin_key(g, R.y[s + c0 + j], R.id[s + c0 + j]);
-            __syncthreads();
-          }
-          if (act)
-            for (uint32_t j = 0; j < n; ++j) {
-              const uint64_t kj = s_key[j];
-              rank += (kj < key) || (kj == key && c0 + j < e);
-            }
-        }
-        if (act) {
-          T.id[s + rank] = R.id[s + e];
-          T.x[s + rank] = R.x[s + e];
-          T.y[s + rank] = R.y[s + e];
-          T.hx[s + rank] = R.hx[s + e];
-          T.hy[s + rank] = R.hy[s + e];
-        }
-      }
-      __syncthreads();
-      for (uint32_t j = t; j < m; j += kSortThreads) {
-        R.id[s + j] = T.id[s + j];
-        R.x[s + j] = T.x[s + j];
-        R.y[s + j] = T.y[s + j];
-        R.hx[s + j] = T.hx[s + j];
-        R.hy[s + j] = T.hy[s + j];
-      }
       __syncthreads();
     }
-    __syncthreads();
   }
 }
 
 hipError_t launch_sort_bins(const WorkList& wl, Buf R, const uint32_t* offset, Buf T,
-                            hipStream_t s) {
+                            hipStream_t s, uint32_t* gate) {
   if (wl.ntiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sort_bins, dim3(wl.ntiles), dim3(kSortThreads), 0, s, wl.reps, wl.nreps,
-                     wl.tile_prefix, R, offset, T);
+  const uint32_t grid = gate && wl.ntiles > kGatedGrid ? kGatedGrid : wl.ntiles;
+  hipLaunchKernelGGL(k_sort_bins, dim3(grid), dim3(kSortThreads), 0, s, wl.reps, wl.nreps,
+                     wl.tile_prefix, R, offset, T, gate, wl.ntiles);
   return hipGetLastError();
 }
 

@@ -15,21 +15,37 @@ hipError_t launch_init_uniform(Buf A, uint32_t base, uint32_t n, float w, float 
                                int heading_mode, hipStream_t s);
 hipError_t launch_count_append(const WorkList& wl, Buf A, uint32_t* count, uint32_t* err,
                                hipStream_t s);
+// gate (or null): the rebinning gate word, cleared for the step that follows
 hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint32_t nbins,
-                       const ScanState& ss, uint64_t* gstep, hipStream_t s);
+                       const ScanState& ss, uint64_t* gstep, hipStream_t s,
+                       uint32_t* gate = nullptr);
+// gate (or null): with a gate the kernel runs only if the gather set gate[0]
 hipError_t launch_scatter_aligned(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
-                                  uint32_t* cursor, Buf R2, hipStream_t s);
+                                  uint32_t* cursor, Buf R2, hipStream_t s,
+                                  const uint32_t* gate = nullptr);
+// The gather rebinning (one work-group per tile; tiles of at most
+// kGatherMaxBins bins) of the flock step that filled the outboxes: R2 in
+// bin order with every bin sorted, or gate[0] set for the scatter and the
+// bin sort to redo it.
+hipError_t launch_rebin_gather(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
+                               const uint32_t* offset_new, const Outbox& ob, Buf R2,
+                               uint32_t* gate, hipStream_t s);
 hipError_t launch_scatter_append(const WorkList& wl, Buf A, uint32_t* cursor, Buf R2,
                                  hipStream_t s);
 // Sorts every bin of R (offsets `offset`) by (cell y, id); T is scratch of
 // R's capacity whose contents are lost.
+// gate (or null): as for the scatter; a sort that runs also counts the step
+// in gate[1]
 hipError_t launch_sort_bins(const WorkList& wl, Buf R, const uint32_t* offset, Buf T,
-                            hipStream_t s);
-// One replica; gflag: one word per tile plus one (scratch, no set-up needed),
-// nagents: R's count.
+                            hipStream_t s, uint32_t* gate = nullptr);
+// gflag: one word per tile plus two (the last counts global-path tiles and
+// starts at zero), nagents: R's count; ob: the outboxes for the gather
+// rebinning that follows, or none; inline_global: each global-path tile is
+// stepped by its own work-group, and k_flock_global is not launched.
 hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W,
                         uint32_t* count_next, const uint64_t* gstep, uint32_t* err, int lds_cap,
-                        uint32_t* gflag, uint32_t nagents, hipStream_t s);
+                        uint32_t* gflag, uint32_t nagents, hipStream_t s,
+                        const Outbox& ob = Outbox{}, bool inline_global = false);
 hipError_t launch_materialize(Buf R, Buf W, uint32_t n, Buf A, hipStream_t s);
 
 // Tile-resident step (one launch per schedule step); with load != 0 the

@@ -77,6 +77,88 @@ KG_HD void scatter_append_one(const RepDev* reps, int nreps, const uint32_t* app
   R2.id[pos] = A.id[i];
 }
 
+// ---- rebinning by gather ----------------------------------------------------
+// After a flock step the birds of a tile's new bins are the tile's old birds
+// that stayed in it plus birds that left neighbouring tiles: the flock
+// kernel lists every bird that leaves its tile in that tile's outbox.  A
+// work-group per tile collects its candidates, ranks each inside its new
+// bin by (cell row, id) -- the order the bin sort gives -- and writes it to
+// its final slot, so the scatter and the bin sort need not run.  A tile that
+// cannot (more than kGatherCap candidates or new birds, an outbox past
+// kOutboxCap, a bin past kGatherBinMax birds, or collected counts that
+// differ from the histogram's, as when a bird jumps past the neighbouring
+// tiles) sets the gate word, and then the scatter and the bin sort redo the
+// whole rebinning of that step.
+
+constexpr uint32_t kGatherThreads = 256;
+constexpr uint32_t kGatherCap = 2048;     // candidates and new birds of a tile, at most
+constexpr uint32_t kGatherRegs = kGatherCap / kGatherThreads;  // candidates per thread
+constexpr uint32_t kGatherBinMax = 512;   // a fuller bin sends the step to the bin sort
+constexpr uint32_t kGatherMaxBins = 256;  // bins per tile the gather handles
+constexpr uint32_t kOutboxCap = 256;      // birds listed per tile and step
+constexpr uint32_t kGatherSources = 16;   // neighbouring tiles read, at most
+
+struct Outbox {
+  uint32_t* n = nullptr;     // per tile (work item): birds that left it (> kOutboxCap: overflow)
+  uint32_t* slot = nullptr;  // per tile: kOutboxCap read-buffer indices
+};
+
+// Entry i (0..3) of the tile columns (or rows) of n whose birds can enter
+// column c in one step, or -1: c - 1, c, c + 1 cyclically and, across the
+// seam of a torus whose last tile column may be narrower than a step, the
+// partner at distance two (n - 2 for 0, 0 for n - 2); each value once.
+KG_HD int32_t gather_axis_at(int32_t c, int32_t n, int i) {
+  const int32_t a0 = (c + n - 1) % n, a1 = c, a2 = (c + 1) % n;
+  const int32_t p = n >= 3 ? (c == 0 ? n - 2 : (c == n - 2 ? 0 : -1)) : -1;
+  if (i == 0) return a0;
+  if (i == 1) return a1 == a0 ? -1 : a1;
+  if (i == 2) return (a2 == a0 || a2 == a1) ? -1 : a2;
+  return (p < 0 || p == a0 || p == a1 || p == a2) ? -1 : p;
+}
+
+// Source q (< kGatherSources) of local tile `tile`: the local tile index, or
+// -1 (not a neighbour, or the tile itself).
+KG_HD int32_t gather_source(const Geom& g, uint32_t tile, uint32_t q) {
+  const int32_t tcx = (int32_t)(tile % (uint32_t)g.ntx), tcy = (int32_t)(tile / (uint32_t)g.ntx);
+  const int32_t sx = gather_axis_at(tcx, g.ntx, (int)(q & 3u));
+  const int32_t sy = gather_axis_at(tcy, g.nty, (int)(q >> 2));
+  if (sx < 0 || sy < 0 || (sx == tcx && sy == tcy)) return -1;
+  return sy * g.ntx + sx;
+}
+
+// Candidate k of a tile: its old birds [lo, lo + nown), then the outbox
+// entries of its sources (work items sitem, counts scnt, kGatherSources).
+KG_HD uint32_t gather_src(const Outbox& ob, const uint32_t* sitem, const uint32_t* scnt,
+                          uint32_t lo, uint32_t nown, uint32_t k) {
+  if (k < nown) return lo + k;
+  uint32_t m = k - nown, q = 0;
+  while (m >= scnt[q]) m -= scnt[q++];
+  return ob.slot[(size_t)sitem[q] * kOutboxCap + m];
+}
+
+// The tile-local bin of a bird at (x, y) if it lies in local tile `tile`,
+// else kGatherMaxBins; *cy: its cell row.
+KG_HD uint32_t gather_bin(const Geom& g, uint32_t tile, float x, float y, int32_t* cy) {
+  int32_t cx;
+  discretize(g, x, y, &cx, cy);
+  if (!cell_valid(g, cx, *cy)) return kGatherMaxBins;
+  const uint32_t nb = bin_of(g, cx, *cy);
+  if ((nb >> (g.tx_log2 + g.tr_log2)) != tile) return kGatherMaxBins;
+  return nb & ((1u << (g.tx_log2 + g.tr_log2)) - 1u);
+}
+
+// Rank of entry e among keys[lo, hi) of its bin by (cell row, id), ties by
+// entry.
+KG_HD uint32_t gather_rank(const uint64_t* key, uint32_t lo, uint32_t hi, uint32_t e,
+                           uint64_t k) {
+  uint32_t rank = 0;
+  for (uint32_t j = lo; j < hi; ++j) {
+    const uint64_t kj = key[j];
+    rank += (kj < k || (kj == k && j < e)) ? 1u : 0u;
+  }
+  return rank;
+}
+
 // ---- rebinning of the flock kernel's output (aligned with R) ---------------
 
 struct TileRange {
@@ -349,24 +431,47 @@ KG_HD void flock_own(const FlockTile& ft, uint32_t k, int32_t oa, int32_t ob, Fl
   }
 }
 
-// Writes the agent's new state into the write buffer and counts its new bin.
+// Where a flock kernel lists the birds that leave their tile: the outboxes
+// (or none), the work item of the replica's first tile (a bird's old tile
+// is item_base + its local tile), and the count to take slots from (a
+// work-group's LDS counter, or null for the outbox's own count).
+struct Leavers {
+  Outbox ob;
+  uint32_t item_base;
+  uint32_t* lds_n;
+};
+
+// Writes the agent's new state into the write buffer and counts its new bin;
+// an agent that leaves its tile (old_tile, local to the replica) is listed
+// in the tile's outbox, if there are outboxes.
 KG_HD void flock_emit(const Geom& g, uint32_t bin_base, Buf W, uint32_t gi, float nx, float ny,
-                      float nhx, float nhy, uint32_t* count_next, uint32_t* err) {
+                      float nhx, float nhy, uint32_t* count_next, uint32_t* err,
+                      uint32_t old_tile, const Leavers& lv) {
   W.x[gi] = nx;
   W.y[gi] = ny;
   W.hx[gi] = nhx;
   W.hy[gi] = nhy;
   int32_t cx, cy;
   discretize(g, nx, ny, &cx, &cy);
-  if (cell_valid(g, cx, cy)) atom_add(&count_next[bin_base + bin_of(g, cx, cy)], 1u);
-  else atom_or(err, 1u);
+  if (!cell_valid(g, cx, cy)) {
+    atom_or(err, 1u);
+    return;
+  }
+  const uint32_t nb = bin_of(g, cx, cy);
+  atom_add(&count_next[bin_base + nb], 1u);
+  if (lv.ob.n && (nb >> (g.tx_log2 + g.tr_log2)) != old_tile) {
+    const uint32_t item = lv.item_base + old_tile;
+    const uint32_t o = atom_add(lv.lds_n ? lv.lds_n : &lv.ob.n[item], 1u);
+    // past the capacity the count alone tells the gather to give up
+    if (o < kOutboxCap) lv.ob.slot[(size_t)item * kOutboxCap + o] = gi;
+  }
 }
 
 // Bird::step for own agent k of the staged tile, neighbours from LDS; LDS
 // slot j holds staged item j + base.
 KG_HD void flock_agent_lds(const RepDev& rp, const FlockTile& ft, uint32_t k,
                            const uint32_t* col_base, uint32_t base, FlockSmem S, Buf W,
-                           uint32_t* count_next, uint32_t* err) {
+                           uint32_t* count_next, uint32_t* err, const Leavers& lv) {
   const Geom& g = rp.g;
   const float T = rp.sq_thresh;
   const int32_t dd = rp.disc_dist;
@@ -415,13 +520,13 @@ KG_HD void flock_agent_lds(const RepDev& rp, const FlockTile& ft, uint32_t k,
   }
   float nx, ny, nhx, nhy;
   flock_finish(acc, rp.p, qid, ft.step, qx, qy, S.hx[k], S.hy[k], g.w, &nx, &ny, &nhx, &nhy);
-  flock_emit(g, rp.bin_base, W, S.gi[k], nx, ny, nhx, nhy, count_next, err);
+  flock_emit(g, rp.bin_base, W, S.gi[k], nx, ny, nhx, nhy, count_next, err, ft.tile, lv);
 }
 
 // Bird::step for read-buffer agent gi, neighbours straight from HBM.
 KG_HD void flock_agent_global(const RepDev& rp, uint64_t step, uint32_t gi, Buf R,
-                              const uint32_t* offset, Buf W, uint32_t* count_next,
-                              uint32_t* err) {
+                              const uint32_t* offset, Buf W, uint32_t* count_next, uint32_t* err,
+                              const Leavers& lv) {
   const Geom& g = rp.g;
   const float qx = R.x[gi], qy = R.y[gi];
   const uint32_t qid = R.id[gi];
@@ -434,7 +539,10 @@ KG_HD void flock_agent_global(const RepDev& rp, uint64_t step, uint32_t gi, Buf 
                });
   float nx, ny, nhx, nhy;
   flock_finish(acc, rp.p, qid, step, qx, qy, R.hx[gi], R.hy[gi], g.w, &nx, &ny, &nhx, &nhy);
-  flock_emit(g, rp.bin_base, W, gi, nx, ny, nhx, nhy, count_next, err);
+  // the old tile from the old position (fewer registers held over the walk)
+  int32_t ocx, ocy;
+  discretize(g, qx, qy, &ocx, &ocy);
+  flock_emit(g, rp.bin_base, W, gi, nx, ny, nhx, nhy, count_next, err, tile_of(g, ocx, ocy), lv);
 }
 
 }  // namespace kg

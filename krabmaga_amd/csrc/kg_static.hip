@@ -94,7 +94,7 @@ size_t carve(unsigned char* base, uint32_t n, const Geom& g, Carve* c) {
   c->offset2 = k.take<uint32_t>(nb + 1);
   c->cursor = k.take<uint32_t>(nb);
   c->partial = k.take<uint32_t>(nscan + 1);
-  c->gflag = k.take<uint32_t>(g.ntiles + 1);
+  c->gflag = k.take<uint32_t>(g.ntiles + 2);
   c->err = k.take<uint32_t>(1);
   c->prefix = k.take<uint32_t>(4);
   c->gstep = k.take<uint64_t>(1);

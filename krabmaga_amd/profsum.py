@@ -37,7 +37,7 @@ KERNEL = "kg::k_rt_step"
 FLOCK = "kg::k_flock"
 # the kernels of one three-kernel schedule step
 STEP_KERNELS = ("kg::k_flock", "kg::k_flock_global", "kg::k_scan_reduce", "kg::k_scan_down",
-                "kg::k_scatter_aligned", "kg::k_sort_bins")
+                "kg::k_rebin_gather", "kg::k_scatter_aligned", "kg::k_sort_bins")
 # launched within a three-kernel step on some layouts: the tile-sum prefix
 # pass of large scans, the device step counter
 STEP_AUX_KERNELS = ("kg::k_scan_partials", "kg::k_add_u64")

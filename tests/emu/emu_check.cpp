@@ -16,6 +16,8 @@ emu_field* emu_create(float w, float h, float d, int tor, uint32_t cap, uint32_t
                       int lds_cap);
 void emu_destroy(emu_field* f);
 void emu_options(emu_field* f, int force_global, uint64_t shuffle_seed);
+void emu_set_gather(emu_field* f, int on);
+void emu_gather_stats(emu_field* f, uint64_t* steps, uint64_t* fallbacks);
 void emu_stats(emu_field* f, uint64_t* lds_tiles, uint64_t* global_tiles,
                uint64_t* chunked_tiles);
 int emu_set(emu_field* f, const uint32_t* ids, const float* x, const float* y, const float* hx,
@@ -60,8 +62,12 @@ struct Scenario {
   int force_global;
   uint64_t shuffle;
   int layout;  // 0 uniform, 1 clustered, 2 hugging the far edges (padding cells),
-               // 3 stacked birds and cell-boundary positions
+               // 3 stacked birds and cell-boundary positions, 4 bands one
+               // jump wide at both sides of the seams, heading across them
   float radius;
+  float jump = 0.7f;
+  int gather = 1;  // the gather rebinning after flock steps (the device default)
+  int all_gathered = 0;  // every flock step must rebin by gather
 };
 
 static void make_positions(const Scenario& s, std::vector<float>& x, std::vector<float>& y,
@@ -85,6 +91,22 @@ static void make_positions(const Scenario& s, std::vector<float>& x, std::vector
       const int k = (int)(i % 6);
       x[i] = or_toroidal_transform((float)(cx[k] + nd(rng)), s.side);
       y[i] = or_toroidal_transform((float)(cy[k] + nd(rng)), s.side);
+    }
+  } else if (s.layout == 4) {
+    std::uniform_real_distribution<double> ud(0.05, 1.0);
+    for (uint32_t i = 0; i < s.n; ++i) {
+      const float off = (float)(ud(rng) * s.jump);
+      const float sgn = (i & 2u) ? 1.0f : -1.0f;  // toward the seam
+      const float at = sgn > 0 ? s.side - off : off;
+      if (i & 1u) {
+        x[i] = at;
+        hx[i] = sgn;
+        hy[i] = 0.0f;
+      } else {
+        y[i] = at;
+        hx[i] = 0.0f;
+        hy[i] = sgn;
+      }
     }
   } else if (s.layout == 3) {
     for (uint32_t i = 0; i < s.n; ++i) {
@@ -120,7 +142,7 @@ static void make_positions(const Scenario& s, std::vector<float>& x, std::vector
 static int run(const Scenario& s) {
   std::vector<float> x, y, hx, hy;
   make_positions(s, x, y, hx, hy);
-  or_flock_params op{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.7f, s.radius, s.seed};
+  or_flock_params op{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, s.jump, s.radius, s.seed};
   or_flocker* fl = or_flocker_new(s.n, s.side, s.side, s.d, s.tor, x.data(), y.data(), hx.data(),
                                   hy.data(), &op);
   emu_field* e = emu_create(s.side, s.side, s.d, s.tor, s.n, s.bin_rows, s.lds_cap);
@@ -129,11 +151,12 @@ static int run(const Scenario& s) {
     return 1;
   }
   emu_options(e, s.force_global, s.shuffle);
+  emu_set_gather(e, s.gather);
   std::vector<uint32_t> ids(s.n);
   for (uint32_t i = 0; i < s.n; ++i) ids[i] = i;
   emu_set(e, ids.data(), x.data(), y.data(), hx.data(), hy.data(), s.n);
   int rc = emu_lazy_update(e);
-  kg_flock_params kp{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 0.7f, s.radius, 0u, s.seed};
+  kg_flock_params kp{1.0f, 1.0f, 1.0f, 1.0f, 1.0f, s.jump, s.radius, 0u, s.seed};
   for (uint32_t k = 0; k < s.steps && rc == 0; ++k) {
     rc = emu_flock_step(e, &kp, k);
     if (rc == 0) rc = emu_lazy_update(e);
@@ -144,8 +167,9 @@ static int run(const Scenario& s) {
   const uint32_t m = emu_read(e, gid.data(), gx.data(), gy.data(), ghx.data(), ghy.data());
   std::vector<float> rx(s.n), ry(s.n), rhx(s.n), rhy(s.n);
   or_flocker_get(fl, rx.data(), ry.data(), rhx.data(), rhy.data());
-  uint64_t lds = 0, glob = 0, chunked = 0;
+  uint64_t lds = 0, glob = 0, chunked = 0, gsteps = 0, gfalls = 0;
   emu_stats(e, &lds, &glob, &chunked);
+  emu_gather_stats(e, &gsteps, &gfalls);
   uint32_t bad = 0;
   std::vector<int> seen(s.n, 0);
   for (uint32_t k = 0; k < m; ++k) {
@@ -159,10 +183,12 @@ static int run(const Scenario& s) {
       ++bad;
   }
   if (m != s.n) bad += s.n;
+  if (s.all_gathered && (gfalls != 0 || gsteps != s.steps)) bad += 1;
   std::printf("%-28s rc=%d agents=%u lds_tiles=%llu chunked=%llu global_tiles=%llu "
-              "mismatches=%u\n",
+              "gathered=%llu/%llu mismatches=%u\n",
               s.name, rc, m, (unsigned long long)lds, (unsigned long long)chunked,
-              (unsigned long long)glob, bad);
+              (unsigned long long)glob, (unsigned long long)(gsteps - gfalls),
+              (unsigned long long)gsteps, bad);
   emu_destroy(e);
   or_flocker_free(fl);
   return (rc != 0 || bad != 0) ? 1 : 0;
@@ -234,7 +260,7 @@ int main() {
   std::setvbuf(stdout, nullptr, _IOLBF, 0);
   const float D = 10.0f / 1.5f;
   const Scenario sc[] = {
-      {"flockers-200", 1000, 200.0f, D, 1, 40, 0, 0, 0, 0, 0, 0, 10.0f},
+      {"flockers-200", 1000, 200.0f, D, 1, 40, 0, 0, 0, 0, 0, 0, 10.0f, 0.7f, 1, 1},
       {"flockers-200-global", 1000, 200.0f, D, 1, 10, 0, 0, 0, 1, 0, 0, 10.0f},
       {"flockers-200-shuffled", 1000, 200.0f, D, 1, 20, 0, 0, 0, 0, 99, 0, 10.0f},
       {"flockers-200-hb16-cap64", 1000, 200.0f, D, 1, 10, 1, 16, 64, 0, 0, 0, 10.0f},
@@ -261,6 +287,26 @@ int main() {
       {"stacked-boundaries", 3000, 200.0f, D, 1, 6, 15, 0, 0, 0, 0, 3, 10.0f},
       {"stacked-boundaries-global", 3000, 200.0f, D, 1, 4, 15, 0, 0, 1, 0, 3, 10.0f},
       {"empty", 0, 200.0f, D, 1, 3, 16, 0, 0, 0, 0, 0, 10.0f},
+      // the scatter and the bin sort every step
+      {"flockers-200-no-gather", 1000, 200.0f, D, 1, 20, 0, 0, 0, 0, 0, 0, 10.0f, 0.7f, 0},
+      {"clustered-no-gather", 6000, 700.0f, D, 1, 4, 4, 0, 0, 0, 3, 1, 10.0f, 0.7f, 0},
+      // jumps of several cells: migrants from far tiles, through the inboxes
+      {"big-jumps", 3000, 600.0f, D, 1, 10, 17, 0, 0, 0, 5, 0, 10.0f, 45.0f},
+      // most birds change tile every step: outboxes past their capacity send
+      // steps to the scatter and the bin sort
+      {"outbox-overflow", 6000, 200.0f, D, 1, 6, 18, 0, 0, 0, 0, 0, 10.0f, 60.0f},
+      // jumps longer than a tile: birds from beyond the neighbouring tiles
+      // leave counts short, and those steps go to the scatter and the bin sort
+      {"far-jumps", 3000, 1200.0f, D, 1, 6, 20, 0, 0, 0, 0, 0, 10.0f, 150.0f},
+      {"non-toroidal-shuffled", 1500, 150.0f, D, 0, 6, 8, 0, 0, 0, 11, 0, 10.0f},
+      // tiles of more than 2,048 birds: every step goes to the scatter and the bin sort
+      {"dense-tiles", 40000, 700.0f, D, 1, 2, 7, 0, 0, 0, 3, 1, 10.0f},
+      // a narrow last tile column: birds cross the seam between tile
+      // columns 0 and n - 2 (20,480 / (10 / 1.5) cells: the 1M field's seam)
+      {"seam-sliver", 4000, 20480.0f / 16.0f, D / 16.0f, 1, 8, 19, 0, 0, 0, 0, 4, 10.0f / 16.0f,
+       0.7f / 16.0f, 1, 1},
+      {"seam-sliver-shuffled", 4000, 20480.0f / 16.0f, D / 16.0f, 1, 4, 21, 0, 0, 0, 9, 4,
+       10.0f / 16.0f, 0.7f / 16.0f, 1, 1},
   };
   int fails = 0;
   for (const Scenario& s : sc) fails += run(s);

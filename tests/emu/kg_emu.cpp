@@ -92,6 +92,12 @@ struct emu_field {
   bool force_global = false;
   uint64_t shuffle_seed = 0;  // 0 = scatter in index order
   uint64_t lds_tiles = 0, global_tiles = 0, deferred_tiles = 0, chunked_tiles = 0;
+  // gather rebinning, as the device field runs it by default
+  bool gather = true;
+  bool outbox_filled = false;
+  std::vector<uint32_t> outbox_n, outbox_slot;
+  uint64_t gather_steps = 0, gather_fallbacks = 0;
+  bool gather_ok() const { return gather && (uint32_t)(g.tx * g.tr) <= kGatherMaxBins; }
 };
 
 extern "C" {
@@ -136,6 +142,8 @@ emu_field* emu_reset_geom(emu_field* f, float w, float h, float d, int tor, uint
   f->offset.assign(f->g.nbins + 1, 0);
   f->offset2.assign(f->g.nbins + 1, 0);
   f->cursor.assign(f->g.nbins, 0);
+  f->outbox_n.assign(f->g.ntiles, 0xFFFFFFFFu);  // written by every flock step
+  f->outbox_slot.assign((size_t)f->g.ntiles * kOutboxCap, 0xFFFFFFFFu);
   return f;
 }
 
@@ -429,6 +437,14 @@ void emu_options(emu_field* f, int force_global, uint64_t shuffle_seed) {
   f->shuffle_seed = shuffle_seed;
 }
 
+// gather rebinning on or off (on by default, as on the device)
+void emu_set_gather(emu_field* f, int on) { f->gather = on != 0; }
+
+void emu_gather_stats(emu_field* f, uint64_t* steps, uint64_t* fallbacks) {
+  *steps = f->gather_steps;
+  *fallbacks = f->gather_fallbacks;
+}
+
 void emu_stats(emu_field* f, uint64_t* lds_tiles, uint64_t* global_tiles,
                uint64_t* chunked_tiles) {
   *lds_tiles = f->lds_tiles;
@@ -460,9 +476,84 @@ static std::vector<uint32_t> order(uint32_t lo, uint32_t hi, uint64_t seed) {
   return o;
 }
 
+// k_rebin_gather for work item `item`, phase by phase (candidates taken in
+// a shuffled order, as the device's atomics may take them); true if the
+// tile could not rebin (the gate word).
+static bool emu_gather_tile(emu_field* f, uint32_t item, Buf R, Buf W, Buf R2) {
+  const Geom& g = f->rep.g;
+  const uint32_t tile = item;  // one replica
+  const uint32_t nbt = 1u << (g.tx_log2 + g.tr_log2);
+  const uint32_t tb0 = tile * nbt;
+  const uint32_t lo = f->offset[tb0], nown = f->offset[tb0 + nbt] - lo;
+  const uint32_t nlo = f->offset2[tb0];
+  std::vector<uint32_t> bstart(kGatherMaxBins + 1), fill(kGatherMaxBins, 0u);
+  std::vector<uint64_t> key(kGatherCap);
+  uint32_t sitem[kGatherSources], scnt[kGatherSources];
+  for (uint32_t b = 0; b <= nbt; ++b) bstart[b] = f->offset2[tb0 + b] - nlo;
+  for (uint32_t q = 0; q < kGatherSources; ++q) {
+    const int32_t sl = gather_source(g, tile, q);
+    sitem[q] = sl >= 0 ? (uint32_t)sl : 0u;
+    scnt[q] = sl >= 0 ? f->outbox_n[(uint32_t)sl] : 0u;
+  }
+  uint32_t fail = 0, ncand = nown;
+  for (uint32_t q = 0; q < kGatherSources; ++q) {
+    ncand += scnt[q];
+    if (scnt[q] > kOutboxCap) fail = 1u;
+  }
+  if (ncand > kGatherCap || bstart[nbt] > kGatherCap) fail = 1u;
+  for (uint32_t b = 0; b < nbt; ++b)
+    if (bstart[b + 1] - bstart[b] > kGatherBinMax) fail = 1u;
+  if (fail) return true;
+  // each candidate stays with its thread from load to store
+  struct Held {
+    uint32_t b = kGatherMaxBins, slot = 0;
+    float x = 0, y = 0, hx = 0, hy = 0;
+    uint32_t id = 0;
+  };
+  const Outbox ob{f->outbox_n.data(), f->outbox_slot.data()};
+  std::vector<Held> held(ncand);
+  for (uint32_t k : order(0, ncand, f->shuffle_seed)) {
+    const uint32_t src = gather_src(ob, sitem, scnt, lo, nown, k);
+    int32_t row;
+    const uint32_t b = gather_bin(g, tile, W.x[src], W.y[src], &row);
+    if (b >= nbt) continue;
+    const uint32_t slot = fill[b]++;
+    if (slot >= bstart[b + 1] - bstart[b]) {
+      fail = 1u;
+      continue;
+    }
+    Held& h = held[k];
+    h.b = b;
+    h.slot = slot;
+    h.x = W.x[src];
+    h.y = W.y[src];
+    h.hx = W.hx[src];
+    h.hy = W.hy[src];
+    h.id = R.id[src];
+    key[bstart[b] + slot] = ((uint64_t)(uint32_t)row << 32) | h.id;
+  }
+  for (uint32_t b = 0; b < nbt; ++b)
+    if (fill[b] != bstart[b + 1] - bstart[b]) fail = 1u;
+  if (fail) return true;
+  for (const Held& h : held) {
+    if (h.b >= nbt) continue;
+    const uint32_t e = bstart[h.b] + h.slot;
+    const uint32_t pos =
+        nlo + bstart[h.b] + gather_rank(key.data(), bstart[h.b], bstart[h.b + 1], e, key[e]);
+    R2.x[pos] = h.x;
+    R2.y[pos] = h.y;
+    R2.hx[pos] = h.hx;
+    R2.hy[pos] = h.hy;
+    R2.id[pos] = h.id;
+  }
+  return false;
+}
+
 int emu_lazy_update(emu_field* f) {
   const uint32_t prefix[2] = {0u, f->g.ntiles};
   const uint32_t app_prefix[2] = {0u, f->nA};
+  const bool gather = f->w_aligned && f->outbox_filled && f->nA == 0;
+  f->outbox_filled = false;
   Buf A = f->A.view();
   for (uint32_t i = 0; i < f->nA; ++i)
     count_append_one(&f->rep, 1, app_prefix, A, i, f->count.data(), &f->err);
@@ -476,7 +567,14 @@ int emu_lazy_update(emu_field* f) {
   }
   f->offset2[f->g.nbins] = run;
   Buf R = f->R.view(), W = f->W.view(), R2 = f->R2.view();
-  if (f->w_aligned) {
+  bool redo = !gather;
+  if (gather) {
+    ++f->gather_steps;
+    for (uint32_t item : order(0, f->g.ntiles, f->shuffle_seed))
+      if (emu_gather_tile(f, item, R, W, R2)) redo = true;
+    if (redo) ++f->gather_fallbacks;
+  }
+  if (f->w_aligned && redo) {
     std::vector<uint32_t> items = order(0, f->g.ntiles, f->shuffle_seed);
     for (uint32_t item : items) {
       const TileRange tr = tile_range(&f->rep, 1, prefix, item, f->offset.data());
@@ -489,7 +587,7 @@ int emu_lazy_update(emu_field* f) {
   // k_sort_bins: each bin into (cell y, id) order, ties by position
   std::vector<uint32_t> idx;
   std::vector<uint64_t> key;
-  for (uint32_t b = 0; b < f->g.nbins; ++b) {
+  for (uint32_t b = 0; b < f->g.nbins && redo; ++b) {
     const uint32_t s = f->offset2[b], m = f->offset2[b + 1] - s;
     if (m <= 1) continue;
     if (m <= kSortSmall) {  // one lane on the device
@@ -537,6 +635,8 @@ int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step) {
       col_base(kMaxStageCols + 1);
   std::vector<uint8_t> smem(flock_smem_bytes(f->lds_cap));
   std::vector<uint8_t> gflag(f->g.ntiles, 0);
+  const bool ob_on = f->gather_ok();
+  f->outbox_filled = ob_on;
   for (uint32_t item = 0; item < f->g.ntiles; ++item) {
     const FlockTile ft = flock_tile(&f->rep, 1, prefix, item, 0);
     const RepDev& rp = f->rep;
@@ -564,6 +664,9 @@ int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step) {
       ++f->lds_tiles;
       if (L > cap) ++f->chunked_tiles;
       const FlockSmem S = flock_carve(smem.data(), f->lds_cap);
+      uint32_t outn = 0;  // the tile's leavers (an LDS counter on the device)
+      const Leavers lv{ob_on ? Outbox{f->outbox_n.data(), f->outbox_slot.data()} : Outbox{}, 0u,
+                       &outn};
       for (int32_t oa = own_lo; oa < own_hi;) {
         const int32_t ob =
             L <= cap ? own_hi : flock_chunk_end(rp, ft, col_base.data(), oa, own_hi, cap);
@@ -579,12 +682,14 @@ int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step) {
         }
         for (uint32_t o = 0; o < nown; ++o)
           flock_agent_lds(rp, ft, S.own[o], col_base.data(), base, S, W, f->count.data(),
-                          &f->err);
+                          &f->err, lv);
         oa = ob;
       }
+      if (ob_on) f->outbox_n[item] = outn;
     } else {
       ++f->global_tiles;
       gflag[item] = 1;
+      if (ob_on) f->outbox_n[item] = 0u;
     }
   }
   // k_flock_global: the flagged tiles' birds, one per thread, in any order
@@ -592,8 +697,11 @@ int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step) {
     int32_t cx, cy;
     discretize(f->rep.g, R.x[gi], R.y[gi], &cx, &cy);
     if (!cell_valid(f->rep.g, cx, cy) || !gflag[tile_of(f->rep.g, cx, cy)]) continue;
-    flock_agent_global(f->rep, f->rep.step_base, gi, R, f->offset.data(), W, f->count.data(),
-                       &f->err);
+    const uint32_t tile = tile_of(f->rep.g, cx, cy);
+    flock_agent_global(
+        f->rep, f->rep.step_base, gi, R, f->offset.data(), W, f->count.data(), &f->err,
+        Leavers{ob_on ? Outbox{f->outbox_n.data(), f->outbox_slot.data()} : Outbox{}, 0u,
+                nullptr});
   }
   f->w_aligned = true;
   return f->err ? -3 : 0;

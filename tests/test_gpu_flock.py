@@ -185,6 +185,73 @@ def test_flock_step_then_lazy_update_equals_schedule_run():
     assert np.array_equal(bits(sa), bits(sb))
 
 
+@pytest.mark.parametrize("n,side,jump,seed", [
+    (3000, 1200.0, 150.0, 20),  # jumps past the neighbouring tiles: counts come up short
+    (6000, 200.0, 60.0, 18),    # most birds leave their tile: outboxes overflow
+    (20000, 900.0, 0.7, 1),     # the gather rebins every step
+])
+def test_gather_rebinning_and_its_fallback_bit_exact(oracle, n, side, jump, seed):
+    """Flock steps followed by lazy_update rebin by gather; the steps it cannot
+    do go to the scatter and the bin sort.  Either way the run equals the oracle."""
+    from krabmaga_amd import FlockParams
+    p = FlockParams(seed=seed, jump=jump)
+    f = gpu_field(n, side, D, seed=seed)
+    f.lazy_update()  # schedule.rs:349-351 at step 0
+    for s in range(6):
+        f.flock_step(p, s)
+        f.lazy_update()
+    got, ids = f.state_by_id(0)
+    assert np.array_equal(ids, np.arange(n))
+    fl = oracle_flocker(oracle, n, side, D, seed, jump=jump)
+    fl.step(6)
+    assert np.array_equal(bits(got), bits(fl.state()))
+
+
+def test_inline_global_path_equals_the_global_launch():
+    """A uniform field's first step shows no tile on the global path, so later
+    steps of schedule_run step such tiles inside k_flock; a dense cluster
+    added afterwards sends tiles there.  kg_flock_step keeps launching
+    k_flock_global: both runs agree bit for bit."""
+    from krabmaga_amd import Field2D, FlockParams
+    n, m, side, seed = 20000, 6000, 900.0, 12
+    p = FlockParams(seed=seed)
+    rng = np.random.default_rng(seed)
+    cx = (400.0 + 20.0 * rng.random(m)).astype(np.float32)
+    cy = (300.0 + 20.0 * rng.random(m)).astype(np.float32)
+    ang = rng.random(m) * 2 * np.pi
+    hx, hy = np.cos(ang).astype(np.float32), np.sin(ang).astype(np.float32)
+    ids = np.arange(n, n + m, dtype=np.uint32)
+    runs = []
+    for by_step in (False, True):
+        f = Field2D(side, side, D, True, n + m)
+        f.init_uniform(n, seed, 1)
+        if by_step:
+            f.lazy_update()
+            for s in range(4):
+                f.flock_step(p, s)
+                f.lazy_update()
+        else:
+            f.schedule_run(p, 0, 4, binned=True)
+        # the write buffer holds what is set for the next step: every bird
+        # so far, then the cluster
+        st0, ids0 = f.state_by_id(0)
+        f.set_object_locations(np.concatenate([ids0, ids]), np.concatenate([st0[0], cx]),
+                               np.concatenate([st0[1], cy]), np.concatenate([st0[2], hx]),
+                               np.concatenate([st0[3], hy]))
+        f.lazy_update()
+        if by_step:
+            for s in range(4, 10):
+                f.flock_step(p, s)
+                f.lazy_update()
+        else:
+            f.schedule_run(p, 4, 6, binned=True)
+        st, got_ids = f.state_by_id(0)
+        assert np.array_equal(got_ids, np.arange(n + m))
+        runs.append(st)
+        f.close()
+    assert np.array_equal(bits(runs[0]), bits(runs[1]))
+
+
 @pytest.mark.parametrize("side", [20480.0, 6476.0])  # the 1M configuration; at the 1,000-boid density
 def test_one_million_boids_invariants(side):
     """1,048,576 boids on a torus: properties independent of size, and the
