@@ -1,6 +1,9 @@
 // kg_common.cpp -- host-side helpers shared by the C ABI implementations:
-// per-thread error messages and the flock kernel's LDS capacity clamp.
+// per-thread error messages, the gather switch and the flock kernel's LDS
+// capacity clamp.
 #include "kg_common.hpp"
+
+#include <cstdlib>
 
 
 namespace kg {
@@ -18,6 +21,14 @@ int set_error(int code, const char* fmt, ...) {
 }
 
 void clear_error() { t_last_error.clear(); }
+
+bool rebin_gather_enabled() {
+  static const bool v = [] {
+    const char* e = std::getenv("KG_REBIN_GATHER");
+    return !(e && e[0] == '0');
+  }();
+  return v;
+}
 
 // Keeps the flock kernel's LDS (dynamic staging + about 4 KB of static
 // tables) within the device's per-block limit.

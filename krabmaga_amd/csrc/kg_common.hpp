@@ -23,6 +23,10 @@ void clear_error();
 // dynamic plus static LDS fits the device's per-block limit.
 int clamp_lds_cap(int cap);
 
+// The gather rebinning after flock steps; KG_REBIN_GATHER=0 turns it off
+// (scatter and bin sort every step), for comparison.
+bool rebin_gather_enabled();
+
 #define KG_HIP(expr)                                                                   \
   do {                                                                                 \
     hipError_t e_ = (expr);                                                            \

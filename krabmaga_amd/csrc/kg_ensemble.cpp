@@ -117,7 +117,16 @@ struct kg_ensemble {
   uint64_t* gstep = nullptr;
   int lds_cap = 1024;
   int parity = 0;  // bin-grouped steps taken since the build, mod 2
-  std::vector<std::pair<int, hipGraphExec_t>> bgraphs;  // two-step graphs by parity
+  // two-step graphs by parity + 2 * gather
+  std::vector<std::pair<int, hipGraphExec_t>> bgraphs;
+  // gather rebinning (every run's tiles of at most kGatherMaxBins bins): the
+  // lists of leavers, the gate words (flag, steps redone) and the redone
+  // count at the last check; a layout whose steps fail it drops it
+  bool gather = false;
+  Outbox outbox{};
+  size_t cap_outbox = 0;
+  uint32_t* gate = nullptr;
+  uint32_t redone_seen = 0;
 
   // current layout
   std::vector<RepDev> h_reps;
@@ -141,6 +150,7 @@ struct kg_ensemble {
 
   size_t bytes_held() const {
     return (cap_a + cap_a2 + cap_r + cap_r2 + cap_w) * 20 + cap_bins * 16 + cap_gflag * 4 +
+           cap_outbox * (kOutboxCap + 1) * 4 +
            rt.stay_alloc * 40 + rt.inbox_alloc * 40 + rt.spill_alloc * 44 +
            rt.boff_alloc * 8 + cap_tiles * 48;
   }
@@ -298,6 +308,19 @@ int reserve_binned(kg_ensemble* e, size_t agents, size_t tiles) {
     return set_error(KG_E_NOMEM, "ensemble bin-grouped buffers: %s", hipGetErrorString(err));
   if (!e->gstep && (err = dmalloc(&e->gstep, 1)) != hipSuccess)
     return set_error(KG_E_NOMEM, "ensemble step counter: %s", hipGetErrorString(err));
+  if (tiles > e->cap_outbox) {
+    (void)hipFree(e->outbox.n);
+    (void)hipFree(e->outbox.slot);
+    e->outbox = Outbox{};
+    e->cap_outbox = 0;
+    const size_t n = std::max(tiles, e->cap_outbox + e->cap_outbox / 2);
+    if ((err = dmalloc(&e->outbox.n, n)) != hipSuccess ||
+        (err = dmalloc(&e->outbox.slot, n * kOutboxCap)) != hipSuccess)
+      return set_error(KG_E_NOMEM, "ensemble outboxes: %s", hipGetErrorString(err));
+    e->cap_outbox = n;
+  }
+  if (!e->gate && (err = dmalloc(&e->gate, 2)) != hipSuccess)
+    return set_error(KG_E_NOMEM, "ensemble gate: %s", hipGetErrorString(err));
   return KG_OK;
 }
 
@@ -459,6 +482,11 @@ int rebuild(kg_ensemble* e) {
     for (const EnsRun& r : e->runs) cap = std::max(cap, pick_lds_cap(r.g, r.desc.agents));
     e->lds_cap = clamp_lds_cap(cap);
     e->parity = 0;
+    e->gather = rebin_gather_enabled();
+    for (const RepDev& rp : e->h_reps)
+      if ((uint32_t)(rp.g.tx * rp.g.tr) > kGatherMaxBins) e->gather = false;
+    KG_HIP(hipMemsetAsync(e->gate, 0, 2 * sizeof(uint32_t), e->stream));
+    e->redone_seen = 0;
   } else {
     if (int rc = e->rt.setup(plan, cls, e->nagents, e->stream)) return rc;
     if (int rc = e->rt.reset(G, e->stream)) return rc;
@@ -474,21 +502,26 @@ int rebuild(kg_ensemble* e) {
 }
 
 // One bin-grouped schedule step of every replica: k_flock over all tiles,
-// the scan over all bins (it advances the step counter), the scatter and the
-// bin sort, then the buffers swap.
+// the scan over all bins (it advances the step counter), the gather (the
+// scatter and the bin sort only if it cannot), then the buffers swap.
 hipError_t binned_step(kg_ensemble* e, const WorkList& w) {
   hipError_t err;
+  uint32_t* gate = e->gather ? e->gate : nullptr;
   if ((err = launch_flock(w, e->R.b, e->offset, e->W.b, e->count, e->gstep, e->err, e->lds_cap,
-                          e->gflag, e->nagents, e->stream)) != hipSuccess)
+                          e->gflag, e->nagents, e->stream,
+                          e->gather ? e->outbox : Outbox{})) != hipSuccess)
     return err;
-  if ((err = launch_scan(e->count, e->offset2, e->cursor, e->nbins, e->ss, e->gstep,
-                         e->stream)) != hipSuccess)
+  if ((err = launch_scan(e->count, e->offset2, e->cursor, e->nbins, e->ss, e->gstep, e->stream,
+                         gate)) != hipSuccess)
     return err;
-  if ((err = launch_scatter_aligned(w, e->R.b, e->W.b, e->offset, e->cursor, e->R2.b,
-                                    e->stream)) != hipSuccess)
+  if (gate && (err = launch_rebin_gather(w, e->R.b, e->W.b, e->offset, e->offset2, e->outbox,
+                                         e->R2.b, gate, e->stream)) != hipSuccess)
+    return err;
+  if ((err = launch_scatter_aligned(w, e->R.b, e->W.b, e->offset, e->cursor, e->R2.b, e->stream,
+                                    gate)) != hipSuccess)
     return err;
   // the write buffer has been consumed: it is the sort's scratch
-  if ((err = launch_sort_bins(w, e->R2.b, e->offset2, e->W.b, e->stream)) != hipSuccess)
+  if ((err = launch_sort_bins(w, e->R2.b, e->offset2, e->W.b, e->stream, gate)) != hipSuccess)
     return err;
   std::swap(e->R, e->R2);
   std::swap(e->cap_r, e->cap_r2);
@@ -504,8 +537,9 @@ int binned_steps(kg_ensemble* e, uint32_t n, bool use_graph) {
   uint32_t left = n;
   if (use_graph && left >= 2) {
     hipGraphExec_t exec = nullptr;
+    const int key = e->parity + (e->gather ? 2 : 0);
     for (auto& g : e->bgraphs)
-      if (g.first == e->parity) exec = g.second;
+      if (g.first == key) exec = g.second;
     if (!exec) {
       hipGraph_t graph = nullptr;
       KG_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
@@ -519,7 +553,7 @@ int binned_steps(kg_ensemble* e, uint32_t n, bool use_graph) {
       (void)hipGraphDestroy(graph);
       if (err != hipSuccess)
         return set_error(KG_E_HIP, "graph instantiate: %s", hipGetErrorString(err));
-      e->bgraphs.push_back({e->parity, exec});
+      e->bgraphs.push_back({key, exec});
     }
     while (left >= 2) {
       KG_HIP(hipGraphLaunch(exec, e->stream));
@@ -583,6 +617,14 @@ int check_flags(kg_ensemble* e) {
     uint32_t errflag = 0;
     KG_HIP(hipMemcpy(&errflag, e->err, sizeof(errflag), hipMemcpyDeviceToHost));
     if (errflag) return set_error(KG_E_RANGE, "an agent left its run's bag grid");
+    if (e->gather) {
+      // steps the gather could not rebin since the last check: a layout
+      // that keeps failing it pays for it and for the scatter and the sort
+      uint32_t redone = 0;
+      KG_HIP(hipMemcpy(&redone, e->gate + 1, sizeof(redone), hipMemcpyDeviceToHost));
+      if (redone != e->redone_seen) e->gather = false;
+      e->redone_seen = redone;
+    }
     return KG_OK;
   }
   uint32_t fl2[2] = {0, 0};
@@ -621,7 +663,8 @@ int kg_run_footprint(const kg_run_desc* run, const kg_tuning* tuning, uint64_t* 
   // again that reserve_buf and the layout give them; spill, bins (count,
   // offset, cursor) and the per-tile tables
   *bytes = n * 20 * 3 * 3 / 2 + n * 40 * 3 * 3 / 2 + n * 44 + (uint64_t)g.nbins * 12 +
-           (uint64_t)g.ntiles * ((uint64_t)(g.tx * g.tr + 1) * 8 + 64) * 3 / 2;
+           (uint64_t)g.ntiles * ((uint64_t)(g.tx * g.tr + 1) * 8 + 64 + (kOutboxCap + 1) * 4) *
+               3 / 2;
   return KG_OK;
 }
 
@@ -658,7 +701,7 @@ void kg_ensemble_destroy(kg_ensemble* e) {
   for (void* p : {(void*)e->count, (void*)e->offset, (void*)e->offset2, (void*)e->cursor,
                   (void*)e->ss.partial, (void*)e->d_reps, (void*)e->d_prefix,
                   (void*)e->rep_cursor, (void*)e->tcount, (void*)e->err, (void*)e->gflag,
-                  (void*)e->gstep})
+                  (void*)e->gstep, (void*)e->outbox.n, (void*)e->outbox.slot, (void*)e->gate})
     (void)hipFree(p);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;

@@ -273,16 +273,6 @@ int set_params(kg_field* f, const kg_flock_params* p, uint64_t step_base) {
 
 // ---- tile-resident layout -------------------------------------------------
 
-// The gather rebinning after flock steps; KG_REBIN_GATHER=0 turns it off
-// (scatter and bin sort every step), for comparison.
-bool rebin_gather_enabled() {
-  static const bool v = [] {
-    const char* e = std::getenv("KG_REBIN_GATHER");
-    return !(e && e[0] == '0');
-  }();
-  return v;
-}
-
 // Deferred tiles in one step that ask for a new plan; KG_RT_REPLAN_DEFERRED
 // overrides it for tuning.
 uint32_t rt_replan_deferred() {
