@@ -391,13 +391,16 @@ size_t flock_lds_bytes(int cap) { return flock_smem_bytes(cap); }
 // a crowded tile is spread over the whole GPU instead of one work-group.
 // Such a tile also stamps gflag[ntiles] with the step (step + 1, so a zeroed
 // word matches no step), which lets k_flock_global return at once in a step
-// without one, and counts itself in gflag[ntiles + 1].  With inline_global
-// (no k_flock_global launch) the work-group steps its tile's birds itself.
+// without one, and counts itself in gflag[ntiles + 1].  With kInlineGlobal
+// (no k_flock_global launch) the work-group steps its tile's birds itself
+// (compiled apart: the global walk's registers would cost the LDS path
+// occupancy).
+template <bool kInlineGlobal>
 __global__ __launch_bounds__(kFlockThreads) void k_flock(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
     const uint32_t* __restrict__ offset, Buf W, uint32_t* __restrict__ count_next,
     const uint64_t* __restrict__ gstep, uint32_t* __restrict__ err, int lds_cap,
-    uint32_t* __restrict__ gflag, Outbox ob, int inline_global) {
+    uint32_t* __restrict__ gflag, Outbox ob) {
   extern __shared__ __align__(16) uint8_t smem_raw[];
   __shared__ uint32_t s_seg_start[3 * kMaxStageCols];
   __shared__ uint32_t s_seg_len[3 * kMaxStageCols];
@@ -473,7 +476,7 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
       __syncthreads();
       if (t == 0) ob.n[item] = s_outn;
     }
-  } else if (inline_global) {
+  } else if (kInlineGlobal) {
     __syncthreads();  // the outbox count is cleared
     const uint32_t lo = offset[ft.tb0], hi = offset[ft.tb1];
     const Leavers lv{ob, item - ft.tile, nullptr};
@@ -535,9 +538,9 @@ hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W
                         uint32_t* gflag, uint32_t nagents, hipStream_t s, const Outbox& ob,
                         bool inline_global) {
   if (wl.ntiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_flock, dim3(wl.ntiles), dim3(kFlockThreads), flock_lds_bytes(lds_cap), s,
-                     wl.reps, wl.nreps, wl.tile_prefix, R, offset, W, count_next, gstep, err,
-                     lds_cap, gflag, ob, inline_global ? 1 : 0);
+  hipLaunchKernelGGL(inline_global ? k_flock<true> : k_flock<false>, dim3(wl.ntiles),
+                     dim3(kFlockThreads), flock_lds_bytes(lds_cap), s, wl.reps, wl.nreps,
+                     wl.tile_prefix, R, offset, W, count_next, gstep, err, lds_cap, gflag, ob);
   if (nagents && !inline_global)
     hipLaunchKernelGGL(ob.n ? k_flock_global<true> : k_flock_global<false>,
                        dim3((nagents + kFlockThreads - 1) / kFlockThreads),
