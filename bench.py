@@ -494,9 +494,9 @@ def main():
     three = d["layout"]["rt_steps"] == 0
     # The roofline prices the whole schedule step at the 76 algorithmic bytes
     # per agent-update: a "launch" is one schedule step -- on the default
-    # (three-kernel) path its five kernels back to back on the field's
-    # stream -- and its duration is the timed region's time per step (HIP
-    # events on that stream).  The dominant kernel's own figure goes beside.
+    # (three-kernel) path its kernels back to back on the field's stream --
+    # and its duration is the timed region's time per step (HIP events on
+    # that stream).  The dominant kernel's own figure goes beside.
     step_us = 1e3 * ms / steps
     achieved = BYTES_STEP * n / (step_us * 1e-6)
     dom, dom_bytes = ("flock", BYTES_FLOCK) if three else ("rt_step", BYTES_RT)
@@ -520,14 +520,15 @@ def main():
             "workload": "flockers plain GPU drop-in: 1,048,576 boids, 20,480 x 20,480 torus, "
                         "discretization 10/1.5, radius 10, weights 1, jump 0.7",
             "agents": n, "side": w["side"], "seed": w["seed"],
-            "path": "three-kernel (k_flock, scan, k_scatter_aligned, k_sort_bins), the "
-                    "field's default at this size" if three else "tile-resident (k_rt_step)",
+            "path": "three-kernel (k_flock, scan, k_rebin_gather), the field's default at "
+                    "this size" if three else "tile-resident (k_rt_step)",
         },
         "roofline": {
             "bound": "hbm",
             "kernel": ("schedule step: k_flock + k_scan_reduce + k_scan_down + "
-                       "k_scatter_aligned + k_sort_bins" if three else
-                       "schedule step: all k_rt_step launches of the step"),
+                       "k_rebin_gather + k_scatter_aligned + k_sort_bins (the last two "
+                       "return at once unless the gather could not rebin the step)"
+                       if three else "schedule step: all k_rt_step launches of the step"),
             "launch_us": step_us,
             "unit_of_launch": "one schedule step",
             "bytes_per_agent": BYTES_STEP,
@@ -537,9 +538,9 @@ def main():
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
             "traffic_source": traffic_src,
-            # the flock phase is k_flock plus its k_flock_global launch (tiles
-            # that do not fit LDS), timed as one
-            "dominant_kernel": "k_flock + k_flock_global" if three else "k_rt_step",
+            # the flock phase: k_flock, plus its k_flock_global launch (tiles
+            # that do not fit LDS) on fields that need one, timed as one
+            "dominant_kernel": "k_flock (+ k_flock_global)" if three else "k_rt_step",
             "dominant_kernel_us": dom_us,
             "dominant_kernel_bytes_per_agent": dom_bytes,
             "dominant_kernel_frac": dom_bytes * n / (dom_us * 1e-6) / HBM_PEAK if dom_us else 0.0,
