@@ -75,6 +75,10 @@ struct kg_field {
   // no tile has taken the global path since the probe: k_flock steps such a
   // tile itself and k_flock_global is not launched
   bool global_inline = false;
+  // the steps of the current run add their counts to the scan's tile sums
+  // (by step parity, the other cleared by the scan): one scan launch
+  bool summed = false;
+  uint32_t* tsum[2] = {nullptr, nullptr};
   RepDev h_rep{};
 
   uint32_t nR = 0, nW = 0, nA = 0;
@@ -210,6 +214,7 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
   const bool gather = f->w_aligned && f->outbox_filled && f->nA == 0;
   uint32_t* gate = gather ? f->gate : nullptr;
   f->outbox_filled = false;
+  const bool summed = gather && f->summed;
   // one replica: the append work list needs no device prefix
   if (f->nA) {
     ProfScope ps(f, KG_KERNEL_COUNT_APPEND);
@@ -217,8 +222,12 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
   }
   {
     ProfScope ps(f, KG_KERNEL_SCAN);
-    KG_HIP(launch_scan(f->count, f->offset2, f->cursor, f->g.nbins, f->ss, step_gstep, f->stream,
-                       gate));
+    if (summed)
+      KG_HIP(launch_scan_summed(f->count, f->offset2, f->cursor, f->g.nbins, f->tsum[f->parity],
+                                f->tsum[f->parity ^ 1], step_gstep, gate, f->stream));
+    else
+      KG_HIP(launch_scan(f->count, f->offset2, f->cursor, f->g.nbins, f->ss, step_gstep,
+                         f->stream, gate));
   }
   if (f->w_aligned) {
     ProfScope ps(f, KG_KERNEL_SCATTER);
@@ -255,7 +264,8 @@ int flock_impl(kg_field* f, const uint64_t* gstep) {
   ProfScope ps(f, KG_KERNEL_FLOCK);
   KG_HIP(launch_flock(f->worklist(), f->R.b, f->offset, f->W.b, f->count, gstep, f->err,
                       f->lds_cap, f->gflag, (uint32_t)f->nR, f->stream,
-                      f->gather_ok ? f->outbox : Outbox{}, f->global_inline));
+                      f->gather_ok ? f->outbox : Outbox{}, f->global_inline,
+                      f->gather_ok && f->summed ? f->tsum[f->parity] : nullptr));
   f->outbox_filled = f->gather_ok;
   f->nW = f->nR;
   f->w_aligned = true;
@@ -495,6 +505,9 @@ int kg_field_create(const kg_field_desc* desc, kg_field** out) {
     if ((e = hipHostMalloc((void**)&f->seen, 2 * sizeof(uint32_t))) != hipSuccess)
       return fail(e);
     f->seen[0] = f->seen[1] = 0;
+    const uint32_t nscan = (f->g.nbins + kScanTile - 1) / kScanTile;
+    for (int k = 0; k < 2; ++k)
+      if ((e = dmalloc(&f->tsum[k], nscan)) != hipSuccess) return fail(e);
   }
   // one-time clears (never inside a captured graph)
   const std::pair<void*, size_t> clears[] = {
@@ -544,6 +557,8 @@ void kg_field_destroy(kg_field* f) {
   (void)hipFree(f->outbox.n);
   (void)hipFree(f->outbox.slot);
   (void)hipFree(f->gate);
+  (void)hipFree(f->tsum[0]);
+  (void)hipFree(f->tsum[1]);
   if (f->seen) (void)hipHostFree(f->seen);
   if (f->stream) (void)hipStreamDestroy(f->stream);
   delete f;
@@ -1014,6 +1029,16 @@ int kg_schedule_run(kg_field* f, const kg_flock_params* p, uint64_t first_step, 
     if (counts[0]) f->gather_ok = false;
     f->global_inline = counts[1] == 0;
   }
+  // steps whose counts go straight into the scan's tile sums: gathered
+  // steps on a field without global-path tiles (their atomics would crowd
+  // one sum), whose tiles fit the LDS histogram and whole scan tiles
+  const uint32_t nbt = (uint32_t)(f->g.tx * f->g.tr);
+  const uint32_t nscan = (f->g.nbins + kScanTile - 1) / kScanTile;
+  f->summed = f->gather_ok && f->global_inline && nscan <= kScanInlineTiles &&
+              nbt <= kGatherMaxBins && (uint32_t)kScanTile % nbt == 0;
+  if (f->summed)
+    for (int k = 0; k < 2; ++k)
+      KG_HIP(hipMemsetAsync(f->tsum[k], 0, nscan * sizeof(uint32_t), f->stream));
   if ((use_graph & KG_RUN_GRAPH) && left >= 2 && !f->prof_on) {
     GraphKey key;
     std::memcpy(key.params, p, sizeof(kg_flock_params));
@@ -1058,6 +1083,7 @@ int kg_schedule_run(kg_field* f, const kg_flock_params* p, uint64_t first_step, 
   if (f->global_inline)
     KG_HIP(hipMemcpyAsync(f->seen + 1, f->gflag + f->g.ntiles + 1, sizeof(uint32_t),
                           hipMemcpyDeviceToHost, f->stream));
+  f->summed = false;  // flock steps outside a run count the usual way
   return KG_OK;
 }
 

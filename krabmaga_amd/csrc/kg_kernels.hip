@@ -4,6 +4,7 @@
 //   count     bin histogram of appended agents       field_2d.rs:838-846 (set_object_location)
 //   scan      exclusive prefix over bin counts       (bag layout, field_2d.rs:307)
 //   scatter   agents to their bins (rebinning)       field_2d.rs:905-921 (lazy_update)
+//   gather    rebinning of a flock step by tile      field_2d.rs:905-921 (lazy_update)
 //   flock     fused neighbour query + Bird::step     field_2d.rs:386-440, bird.rs:38-154
 //             + bin histogram of the new positions
 //   query/... single-replica API queries             field_2d.rs:472-808
@@ -285,7 +286,6 @@ hipError_t launch_materialize(Buf R, Buf W, uint32_t n, Buf A, hipStream_t s) {
 // histogram), fills the scatter cursors and advances the schedule-step
 // counter.
 
-constexpr uint32_t kScanInlineTiles = 2048;
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ count,
                                                               uint32_t n,
@@ -327,7 +327,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
                                                             uint32_t* __restrict__ cursor, uint32_t n,
                                                             const uint32_t* __restrict__ partial,
                                                             int prefixed, uint64_t* gstep,
-                                                            uint32_t* gate) {
+                                                            uint32_t* gate,
+                                                            uint32_t* __restrict__ zero_next) {
   __shared__ uint32_t s_warp[kScanThreads / kWave];
   const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   // the tiles before this one (at most kScanInlineTiles sums, cached)
@@ -363,6 +364,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
     if (gstep) *gstep += 1;
     if (gate) *gate = 0u;
   }
+  if (zero_next && threadIdx.x == 0) zero_next[blockIdx.x] = 0u;
 }
 
 hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint32_t nbins,
@@ -374,7 +376,16 @@ hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint
   if (prefixed)
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, ss.partial, ntiles);
   hipLaunchKernelGGL(k_scan_down, dim3(ntiles), dim3(kScanThreads), 0, s, count, offset, cursor,
-                     nbins, ss.partial, prefixed, gstep, gate);
+                     nbins, ss.partial, prefixed, gstep, gate, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan_summed(uint32_t* count, uint32_t* offset, uint32_t* cursor,
+                              uint32_t nbins, const uint32_t* sums, uint32_t* sums_next,
+                              uint64_t* gstep, uint32_t* gate, hipStream_t s) {
+  const uint32_t ntiles = (nbins + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_down, dim3(ntiles), dim3(kScanThreads), 0, s, count, offset, cursor,
+                     nbins, sums, 0, gstep, gate, sums_next);
   return hipGetLastError();
 }
 
@@ -394,13 +405,15 @@ size_t flock_lds_bytes(int cap) { return flock_smem_bytes(cap); }
 // without one, and counts itself in gflag[ntiles + 1].  With kInlineGlobal
 // (no k_flock_global launch) the work-group steps its tile's birds itself
 // (compiled apart: the global walk's registers would cost the LDS path
-// occupancy).
+// occupancy).  With tile_sum (the scan's per-scan-tile sums, tiles of at
+// most kGatherMaxBins bins) the LDS path counts its stayers in LDS and adds
+// them to the bins and the sum once at its end.
 template <bool kInlineGlobal>
 __global__ __launch_bounds__(kFlockThreads) void k_flock(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
     const uint32_t* __restrict__ offset, Buf W, uint32_t* __restrict__ count_next,
     const uint64_t* __restrict__ gstep, uint32_t* __restrict__ err, int lds_cap,
-    uint32_t* __restrict__ gflag, Outbox ob) {
+    uint32_t* __restrict__ gflag, Outbox ob, uint32_t* __restrict__ tile_sum) {
   extern __shared__ __align__(16) uint8_t smem_raw[];
   __shared__ uint32_t s_seg_start[3 * kMaxStageCols];
   __shared__ uint32_t s_seg_len[3 * kMaxStageCols];
@@ -408,11 +421,15 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
   __shared__ uint32_t s_warp[kFlockThreads / kWave];
   __shared__ uint32_t s_nown;
   __shared__ uint32_t s_outn;
+  __shared__ uint32_t s_hist[kGatherMaxBins];
 
   const int t = threadIdx.x;
   const uint32_t item = xcd_remap(blockIdx.x, gridDim.x);
   const FlockTile ft = flock_tile(reps, nreps, tile_prefix, item, *gstep);
   const RepDev& rp = reps[ft.r];
+  const uint32_t nbt = ft.tb1 - ft.tb0;
+  if (tile_sum)
+    for (uint32_t b = t; b < nbt; b += kFlockThreads) s_hist[b] = 0u;
 
   bool use_lds = rp.lds_ok != 0 && ft.ncols <= kMaxStageCols;
   uint32_t L = 0;
@@ -447,7 +464,7 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
     }
   }
   if (use_lds) {
-    const Leavers lv{ob, item - ft.tile, &s_outn};
+    const FlockSinks lv{ob, item - ft.tile, &s_outn, tile_sum ? s_hist : nullptr, tile_sum};
     const FlockSmem S = flock_carve(smem_raw, lds_cap);
     for (int32_t oa = own_lo; oa < own_hi;) {
       const int32_t ob = L <= (uint32_t)lds_cap
@@ -472,14 +489,24 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
         __syncthreads();
       }
     }
-    if (ob.n) {
-      __syncthreads();
-      if (t == 0) ob.n[item] = s_outn;
+    if (ob.n || tile_sum) __syncthreads();
+    if (ob.n && t == 0) ob.n[item] = s_outn;
+    if (tile_sum) {
+      // the stayers into their bins; all of them into the one scan tile
+      // that holds the tile's bins, one atomic per wave
+      uint32_t h = 0;
+      for (uint32_t b = t; b < nbt; b += kFlockThreads) {
+        const uint32_t v = s_hist[b];
+        if (v) atomicAdd(&count_next[ft.tb0 + b], v);
+        h += v;
+      }
+      for (int off = kWave / 2; off > 0; off >>= 1) h += __shfl_xor(h, off, kWave);
+      if ((t & (kWave - 1)) == 0 && h) atomicAdd(&tile_sum[ft.tb0 / (uint32_t)kScanTile], h);
     }
   } else if (kInlineGlobal) {
     __syncthreads();  // the outbox count is cleared
     const uint32_t lo = offset[ft.tb0], hi = offset[ft.tb1];
-    const Leavers lv{ob, item - ft.tile, nullptr};
+    const FlockSinks lv{ob, item - ft.tile, nullptr, nullptr, tile_sum};
     for (uint32_t gi = lo + (uint32_t)t; gi < hi; gi += kFlockThreads)
       flock_agent_global(rp, ft.step, gi, R, offset, W, count_next, err, lv);
   }
@@ -496,7 +523,7 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock_global(
     const uint32_t* __restrict__ app_prefix, Buf R, const uint32_t* __restrict__ offset, Buf W,
     uint32_t* __restrict__ count_next, const uint64_t* __restrict__ gstep,
     uint32_t* __restrict__ err, const uint32_t* __restrict__ gflag, uint32_t ntiles,
-    uint32_t n, Outbox ob) {
+    uint32_t n, Outbox ob, uint32_t* __restrict__ tile_sum) {
   __shared__ uint32_t s_any;
   // no tile took the global path in this step
   if (gflag[ntiles] != (uint32_t)*gstep + 1u) return;
@@ -530,23 +557,24 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock_global(
   const int r = nreps == 1 ? 0 : find_rep(app_prefix, nreps, gi);
   const RepDev& rp = reps[r];
   flock_agent_global(rp, rp.step_base + *gstep, gi, R, offset, W, count_next, err,
-                     Leavers{kLeavers ? ob : Outbox{}, tile_prefix[r], nullptr});
+                     FlockSinks{kLeavers ? ob : Outbox{}, tile_prefix[r], nullptr, nullptr, tile_sum});
 }
 
 hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W,
                         uint32_t* count_next, const uint64_t* gstep, uint32_t* err, int lds_cap,
                         uint32_t* gflag, uint32_t nagents, hipStream_t s, const Outbox& ob,
-                        bool inline_global) {
+                        bool inline_global, uint32_t* tile_sum) {
   if (wl.ntiles == 0) return hipSuccess;
   hipLaunchKernelGGL(inline_global ? k_flock<true> : k_flock<false>, dim3(wl.ntiles),
                      dim3(kFlockThreads), flock_lds_bytes(lds_cap), s, wl.reps, wl.nreps,
-                     wl.tile_prefix, R, offset, W, count_next, gstep, err, lds_cap, gflag, ob);
+                     wl.tile_prefix, R, offset, W, count_next, gstep, err, lds_cap, gflag, ob,
+                     tile_sum);
   if (nagents && !inline_global)
     hipLaunchKernelGGL(ob.n ? k_flock_global<true> : k_flock_global<false>,
                        dim3((nagents + kFlockThreads - 1) / kFlockThreads),
                        dim3(kFlockThreads), 0, s, wl.reps, wl.nreps, wl.tile_prefix,
                        wl.app_prefix, R, offset, W, count_next, gstep, err, gflag, wl.ntiles,
-                       nagents, ob);
+                       nagents, ob, tile_sum);
   return hipGetLastError();
 }
 

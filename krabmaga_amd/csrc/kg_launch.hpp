@@ -19,6 +19,13 @@ hipError_t launch_count_append(const WorkList& wl, Buf A, uint32_t* count, uint3
 hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint32_t nbins,
                        const ScanState& ss, uint64_t* gstep, hipStream_t s,
                        uint32_t* gate = nullptr);
+// The scan in one launch, with the sums of its tiles (kScanTile bins each,
+// at most kScanInlineTiles of them) already in `sums`, as a flock step with
+// tile sums leaves them; sums_next, the other step's sums, is cleared.
+hipError_t launch_scan_summed(uint32_t* count, uint32_t* offset, uint32_t* cursor,
+                              uint32_t nbins, const uint32_t* sums, uint32_t* sums_next,
+                              uint64_t* gstep, uint32_t* gate, hipStream_t s);
+
 // gate (or null): with a gate the kernel runs only if the gather set gate[0]
 hipError_t launch_scatter_aligned(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
                                   uint32_t* cursor, Buf R2, hipStream_t s,
@@ -41,11 +48,14 @@ hipError_t launch_sort_bins(const WorkList& wl, Buf R, const uint32_t* offset, B
 // gflag: one word per tile plus two (the last counts global-path tiles and
 // starts at zero), nagents: R's count; ob: the outboxes for the gather
 // rebinning that follows, or none; inline_global: each global-path tile is
-// stepped by its own work-group, and k_flock_global is not launched.
+// stepped by its own work-group, and k_flock_global is not launched;
+// tile_sum: the scan's tile sums to add the counts to (zero on entry; tiles
+// of at most kGatherMaxBins bins, whole tiles per scan tile), or none.
 hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W,
                         uint32_t* count_next, const uint64_t* gstep, uint32_t* err, int lds_cap,
                         uint32_t* gflag, uint32_t nagents, hipStream_t s,
-                        const Outbox& ob = Outbox{}, bool inline_global = false);
+                        const Outbox& ob = Outbox{}, bool inline_global = false,
+                        uint32_t* tile_sum = nullptr);
 hipError_t launch_materialize(Buf R, Buf W, uint32_t n, Buf A, hipStream_t s);
 
 // Tile-resident step (one launch per schedule step); with load != 0 the

@@ -431,14 +431,22 @@ KG_HD void flock_own(const FlockTile& ft, uint32_t k, int32_t oa, int32_t ob, Fl
   }
 }
 
-// Where a flock kernel lists the birds that leave their tile: the outboxes
-// (or none), the work item of the replica's first tile (a bird's old tile
-// is item_base + its local tile), and the count to take slots from (a
-// work-group's LDS counter, or null for the outbox's own count).
-struct Leavers {
+// Where a flock kernel records a bird's new place besides the bin counts:
+// - the outboxes (or none) listing the birds that leave their tile, with
+//   the work item of the replica's first tile (a bird's old tile is
+//   item_base + its local tile) and the count to take slots from (a
+//   work-group's LDS counter, or null for the outbox's own count);
+// - the sums of the scan tiles (or none; each scan tile holds whole tiles)
+//   and a histogram of the work-group's own tile's bins in LDS (or none):
+//   with one, a bird that stays in its tile is counted there (the work-group
+//   adds it to the bins and the sum at its end), any other bird in its bin
+//   and its scan tile's sum at once.
+struct FlockSinks {
   Outbox ob;
   uint32_t item_base;
   uint32_t* lds_n;
+  uint32_t* lds_hist = nullptr;
+  uint32_t* tile_sum = nullptr;
 };
 
 // Writes the agent's new state into the write buffer and counts its new bin;
@@ -446,7 +454,7 @@ struct Leavers {
 // in the tile's outbox, if there are outboxes.
 KG_HD void flock_emit(const Geom& g, uint32_t bin_base, Buf W, uint32_t gi, float nx, float ny,
                       float nhx, float nhy, uint32_t* count_next, uint32_t* err,
-                      uint32_t old_tile, const Leavers& lv) {
+                      uint32_t old_tile, const FlockSinks& lv) {
   W.x[gi] = nx;
   W.y[gi] = ny;
   W.hx[gi] = nhx;
@@ -458,8 +466,14 @@ KG_HD void flock_emit(const Geom& g, uint32_t bin_base, Buf W, uint32_t gi, floa
     return;
   }
   const uint32_t nb = bin_of(g, cx, cy);
-  atom_add(&count_next[bin_base + nb], 1u);
-  if (lv.ob.n && (nb >> (g.tx_log2 + g.tr_log2)) != old_tile) {
+  const bool stays = (nb >> (g.tx_log2 + g.tr_log2)) == old_tile;
+  if (lv.lds_hist && stays) {
+    atom_add(&lv.lds_hist[nb & ((1u << (g.tx_log2 + g.tr_log2)) - 1u)], 1u);
+  } else {
+    atom_add(&count_next[bin_base + nb], 1u);
+    if (lv.tile_sum) atom_add(&lv.tile_sum[(bin_base + nb) / (uint32_t)kScanTile], 1u);
+  }
+  if (lv.ob.n && !stays) {
     const uint32_t item = lv.item_base + old_tile;
     const uint32_t o = atom_add(lv.lds_n ? lv.lds_n : &lv.ob.n[item], 1u);
     // past the capacity the count alone tells the gather to give up
@@ -471,7 +485,7 @@ KG_HD void flock_emit(const Geom& g, uint32_t bin_base, Buf W, uint32_t gi, floa
 // slot j holds staged item j + base.
 KG_HD void flock_agent_lds(const RepDev& rp, const FlockTile& ft, uint32_t k,
                            const uint32_t* col_base, uint32_t base, FlockSmem S, Buf W,
-                           uint32_t* count_next, uint32_t* err, const Leavers& lv) {
+                           uint32_t* count_next, uint32_t* err, const FlockSinks& lv) {
   const Geom& g = rp.g;
   const float T = rp.sq_thresh;
   const int32_t dd = rp.disc_dist;
@@ -526,7 +540,7 @@ KG_HD void flock_agent_lds(const RepDev& rp, const FlockTile& ft, uint32_t k,
 // Bird::step for read-buffer agent gi, neighbours straight from HBM.
 KG_HD void flock_agent_global(const RepDev& rp, uint64_t step, uint32_t gi, Buf R,
                               const uint32_t* offset, Buf W, uint32_t* count_next, uint32_t* err,
-                              const Leavers& lv) {
+                              const FlockSinks& lv) {
   const Geom& g = rp.g;
   const float qx = R.x[gi], qy = R.y[gi];
   const uint32_t qid = R.id[gi];

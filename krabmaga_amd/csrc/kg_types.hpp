@@ -43,6 +43,8 @@ struct ScanState {
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 2;
 constexpr int kScanTile = kScanThreads * kScanItems;
+// Scan tiles the two-launch scan sums inline (past them it adds a prefix pass).
+constexpr uint32_t kScanInlineTiles = 2048;
 constexpr int kFlockThreads = 256;
 constexpr int kMaxStageCols = 136;  // tx <= 128 plus a halo of up to 4 columns
 

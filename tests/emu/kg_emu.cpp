@@ -98,6 +98,15 @@ struct emu_field {
   std::vector<uint32_t> outbox_n, outbox_slot;
   uint64_t gather_steps = 0, gather_fallbacks = 0;
   bool gather_ok() const { return gather && (uint32_t)(g.tx * g.tr) <= kGatherMaxBins; }
+  // flock counts into the scan's tile sums (the LDS histogram of a tile's
+  // stayers), as the device field's runs take them
+  bool summed_step = false;
+  std::vector<uint32_t> tsum;
+  bool summed_ok() const {
+    const uint32_t nbt = (uint32_t)(g.tx * g.tr);
+    return gather_ok() && (g.nbins + kScanTile - 1) / kScanTile <= kScanInlineTiles &&
+           (uint32_t)kScanTile % nbt == 0;
+  }
 };
 
 extern "C" {
@@ -143,6 +152,7 @@ emu_field* emu_reset_geom(emu_field* f, float w, float h, float d, int tor, uint
   f->offset2.assign(f->g.nbins + 1, 0);
   f->cursor.assign(f->g.nbins, 0);
   f->outbox_n.assign(f->g.ntiles, 0xFFFFFFFFu);  // written by every flock step
+  f->tsum.assign((f->g.nbins + kScanTile - 1) / kScanTile, 0u);
   f->outbox_slot.assign((size_t)f->g.ntiles * kOutboxCap, 0xFFFFFFFFu);
   return f;
 }
@@ -554,6 +564,17 @@ int emu_lazy_update(emu_field* f) {
   const uint32_t app_prefix[2] = {0u, f->nA};
   const bool gather = f->w_aligned && f->outbox_filled && f->nA == 0;
   f->outbox_filled = false;
+  if (f->summed_step) {
+    // the tile sums the flock step left equal the scan's own
+    for (uint32_t q = 0; q < f->tsum.size(); ++q) {
+      uint32_t sum = 0;
+      for (uint32_t b = q * kScanTile; b < std::min<uint32_t>(f->g.nbins, (q + 1) * kScanTile); ++b)
+        sum += f->count[b];
+      if (sum != f->tsum[q]) return -8;
+      f->tsum[q] = 0;
+    }
+    f->summed_step = false;
+  }
   Buf A = f->A.view();
   for (uint32_t i = 0; i < f->nA; ++i)
     count_append_one(&f->rep, 1, app_prefix, A, i, f->count.data(), &f->err);
@@ -637,6 +658,10 @@ int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step) {
   std::vector<uint8_t> gflag(f->g.ntiles, 0);
   const bool ob_on = f->gather_ok();
   f->outbox_filled = ob_on;
+  const bool summed = f->summed_ok();
+  f->summed_step = summed;
+  uint32_t* const tsum = summed ? f->tsum.data() : nullptr;
+  std::vector<uint32_t> hist(kGatherMaxBins);
   for (uint32_t item = 0; item < f->g.ntiles; ++item) {
     const FlockTile ft = flock_tile(&f->rep, 1, prefix, item, 0);
     const RepDev& rp = f->rep;
@@ -665,8 +690,9 @@ int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step) {
       if (L > cap) ++f->chunked_tiles;
       const FlockSmem S = flock_carve(smem.data(), f->lds_cap);
       uint32_t outn = 0;  // the tile's leavers (an LDS counter on the device)
-      const Leavers lv{ob_on ? Outbox{f->outbox_n.data(), f->outbox_slot.data()} : Outbox{}, 0u,
-                       &outn};
+      std::fill(hist.begin(), hist.end(), 0u);
+      const FlockSinks lv{ob_on ? Outbox{f->outbox_n.data(), f->outbox_slot.data()} : Outbox{}, 0u,
+                          &outn, summed ? hist.data() : nullptr, tsum};
       for (int32_t oa = own_lo; oa < own_hi;) {
         const int32_t ob =
             L <= cap ? own_hi : flock_chunk_end(rp, ft, col_base.data(), oa, own_hi, cap);
@@ -686,6 +712,11 @@ int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step) {
         oa = ob;
       }
       if (ob_on) f->outbox_n[item] = outn;
+      if (summed)
+        for (uint32_t b = 0; b < ft.tb1 - ft.tb0; ++b) {
+          f->count[ft.tb0 + b] += hist[b];
+          tsum[ft.tb0 / kScanTile] += hist[b];
+        }
     } else {
       ++f->global_tiles;
       gflag[item] = 1;
@@ -700,8 +731,8 @@ int emu_flock_step(emu_field* f, const kg_flock_params* p, uint64_t step) {
     const uint32_t tile = tile_of(f->rep.g, cx, cy);
     flock_agent_global(
         f->rep, f->rep.step_base, gi, R, f->offset.data(), W, f->count.data(), &f->err,
-        Leavers{ob_on ? Outbox{f->outbox_n.data(), f->outbox_slot.data()} : Outbox{}, 0u,
-                nullptr});
+        FlockSinks{ob_on ? Outbox{f->outbox_n.data(), f->outbox_slot.data()} : Outbox{}, 0u,
+                   nullptr, nullptr, tsum});
   }
   f->w_aligned = true;
   return f->err ? -3 : 0;
