@@ -217,18 +217,17 @@ def summarize(a):
                      f"and WRITE_SIZE x {kw:.3f} x 1024.\n")
         alg = BYTES_PER_AGENT * a.agents
         if three:
-            # one dispatch of each step kernel per step: the mean per dispatch
-            # of each, at the grid it ran most often (the profiled field's)
+            # each step kernel's mean per dispatch times its dispatches per
+            # step in the trace's steps (a kernel some steps leave out
+            # counts in proportion)
+            launches = collections.Counter((r["name"], r["grid"]) for g in sel for r in g)
             fetch = write = 0.0
-            for kn in STEP_KERNELS:
-                grids = [k for k in fn if k[0] == kn]
-                if not grids:
-                    continue
-                k = max(grids, key=lambda k: fn[k])
-                fetch += f[k] / fn[k] * 1024 * kf
-                if k in w:
-                    write += w[k] / wn[k] * 1024 * kw
-            what = "three-kernel step (one dispatch of each of its kernels)"
+            for k, cnt in launches.items():
+                if k in f and fn[k]:
+                    fetch += f[k] / fn[k] * 1024 * kf * cnt / len(sel)
+                if k in w and wn[k]:
+                    write += w[k] / wn[k] * 1024 * kw * cnt / len(sel)
+            what = "three-kernel step (its kernels' dispatches per step)"
         else:
             # per step: all k_rt_step dispatches of a counter pass over the
             # steps that pass ran (its own launches, grouped as the trace's)
