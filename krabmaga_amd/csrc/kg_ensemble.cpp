@@ -319,7 +319,7 @@ int reserve_binned(kg_ensemble* e, size_t agents, size_t tiles) {
       return set_error(KG_E_NOMEM, "ensemble outboxes: %s", hipGetErrorString(err));
     e->cap_outbox = n;
   }
-  if (!e->gate && (err = dmalloc(&e->gate, 2)) != hipSuccess)
+  if (!e->gate && (err = dmalloc(&e->gate, 3)) != hipSuccess)
     return set_error(KG_E_NOMEM, "ensemble gate: %s", hipGetErrorString(err));
   return KG_OK;
 }
@@ -485,7 +485,7 @@ int rebuild(kg_ensemble* e) {
     e->gather = rebin_gather_enabled();
     for (const RepDev& rp : e->h_reps)
       if ((uint32_t)(rp.g.tx * rp.g.tr) > kGatherMaxBins) e->gather = false;
-    KG_HIP(hipMemsetAsync(e->gate, 0, 2 * sizeof(uint32_t), e->stream));
+    KG_HIP(hipMemsetAsync(e->gate, 0, 3 * sizeof(uint32_t), e->stream));
     e->redone_seen = 0;
   } else {
     if (int rc = e->rt.setup(plan, cls, e->nagents, e->stream)) return rc;
@@ -514,14 +514,15 @@ hipError_t binned_step(kg_ensemble* e, const WorkList& w) {
   if ((err = launch_scan(e->count, e->offset2, e->cursor, e->nbins, e->ss, e->gstep, e->stream,
                          gate)) != hipSuccess)
     return err;
+  const Gate gt{gate, gate ? e->gate + 2 : nullptr, nullptr};
   if (gate && (err = launch_rebin_gather(w, e->R.b, e->W.b, e->offset, e->offset2, e->outbox,
-                                         e->R2.b, gate, e->stream)) != hipSuccess)
+                                         e->R2.b, gt, GatherScan{}, e->stream)) != hipSuccess)
     return err;
   if ((err = launch_scatter_aligned(w, e->R.b, e->W.b, e->offset, e->cursor, e->R2.b, e->stream,
                                     gate)) != hipSuccess)
     return err;
   // the write buffer has been consumed: it is the sort's scratch
-  if ((err = launch_sort_bins(w, e->R2.b, e->offset2, e->W.b, e->stream, gate)) != hipSuccess)
+  if ((err = launch_sort_bins(w, e->R2.b, e->offset2, e->W.b, e->stream, gt)) != hipSuccess)
     return err;
   std::swap(e->R, e->R2);
   std::swap(e->cap_r, e->cap_r2);
@@ -621,7 +622,7 @@ int check_flags(kg_ensemble* e) {
       // steps the gather could not rebin since the last check: a layout
       // that keeps failing it pays for it and for the scatter and the sort
       uint32_t redone = 0;
-      KG_HIP(hipMemcpy(&redone, e->gate + 1, sizeof(redone), hipMemcpyDeviceToHost));
+      KG_HIP(hipMemcpy(&redone, e->gate + 2, sizeof(redone), hipMemcpyDeviceToHost));
       if (redone != e->redone_seen) e->gather = false;
       e->redone_seen = redone;
     }

@@ -60,8 +60,8 @@ struct kg_field {
   // stamp and the count of global-path tiles so far
   uint32_t* gflag = nullptr;
   // gather rebinning (tiles of at most kGatherMaxBins bins): the flock
-  // kernel's lists of leavers, the gate words (flag, steps redone) and
-  // whether the last flock step filled the lists
+  // kernel's lists of leavers, the gate words (a flag per step parity, the
+  // steps redone) and whether the last flock step filled the lists
   bool gather_ok = false;
   Outbox outbox{};
   uint32_t* gate = nullptr;
@@ -76,9 +76,12 @@ struct kg_field {
   // tile itself and k_flock_global is not launched
   bool global_inline = false;
   // the steps of the current run add their counts to the scan's tile sums
-  // (by step parity, the other cleared by the scan): one scan launch
+  // and the gather does the scan: bin counts and sums by step parity, the
+  // gather clearing the other step's (count2 is the second count array)
   bool summed = false;
   uint32_t* tsum[2] = {nullptr, nullptr};
+  uint32_t* count2 = nullptr;
+  uint32_t* counts(int p) const { return p ? count2 : count; }
   RepDev h_rep{};
 
   uint32_t nR = 0, nW = 0, nA = 0;
@@ -212,7 +215,8 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
   // the gather rebins it, the scatter and the bin sort run only if it
   // sets the gate
   const bool gather = f->w_aligned && f->outbox_filled && f->nA == 0;
-  uint32_t* gate = gather ? f->gate : nullptr;
+  const int p = f->parity;
+  uint32_t* gate = gather ? f->gate + p : nullptr;
   f->outbox_filled = false;
   const bool summed = gather && f->summed;
   // one replica: the append work list needs no device prefix
@@ -220,20 +224,29 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
     ProfScope ps(f, KG_KERNEL_COUNT_APPEND);
     KG_HIP(launch_count_append(wl, f->A.b, f->count, f->err, f->stream));
   }
-  {
+  if (!summed) {
     ProfScope ps(f, KG_KERNEL_SCAN);
-    if (summed)
-      KG_HIP(launch_scan_summed(f->count, f->offset2, f->cursor, f->g.nbins, f->tsum[f->parity],
-                                f->tsum[f->parity ^ 1], step_gstep, gate, f->stream));
-    else
-      KG_HIP(launch_scan(f->count, f->offset2, f->cursor, f->g.nbins, f->ss, step_gstep,
-                         f->stream, gate));
+    KG_HIP(launch_scan(f->count, f->offset2, f->cursor, f->g.nbins, f->ss, step_gstep, f->stream,
+                       gate));
   }
+  const Gate gt{gate, gather ? f->gate + 2 : nullptr, summed ? f->gate + (p ^ 1) : nullptr};
   if (f->w_aligned) {
     ProfScope ps(f, KG_KERNEL_SCATTER);
-    if (gather)
+    if (gather) {
+      GatherScan sc;
+      if (summed) {
+        sc.count = f->counts(p);
+        sc.tile_sum = f->tsum[p];
+        sc.count_next = f->counts(p ^ 1);
+        sc.tile_sum_next = f->tsum[p ^ 1];
+        sc.cursor = f->cursor;
+        sc.gstep = step_gstep;
+        sc.nbins = f->g.nbins;
+        sc.last_item = f->g.ntiles - 1;
+      }
       KG_HIP(launch_rebin_gather(wl, f->R.b, f->W.b, f->offset, f->offset2, f->outbox,
-                                 f->R2.b, gate, f->stream));
+                                 f->R2.b, gt, sc, f->stream));
+    }
     KG_HIP(launch_scatter_aligned(wl, f->R.b, f->W.b, f->offset, f->cursor, f->R2.b, f->stream,
                                   gate));
   }
@@ -244,7 +257,7 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
   {
     // the write buffer has been consumed: it is the sort's scratch
     ProfScope ps(f, KG_KERNEL_SORT_BINS);
-    KG_HIP(launch_sort_bins(wl, f->R2.b, f->offset2, f->W.b, f->stream, gate));
+    KG_HIP(launch_sort_bins(wl, f->R2.b, f->offset2, f->W.b, f->stream, gt));
   }
   std::swap(f->R, f->R2);
   std::swap(f->offset, f->offset2);
@@ -262,10 +275,12 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
 
 int flock_impl(kg_field* f, const uint64_t* gstep) {
   ProfScope ps(f, KG_KERNEL_FLOCK);
-  KG_HIP(launch_flock(f->worklist(), f->R.b, f->offset, f->W.b, f->count, gstep, f->err,
+  const bool summed = f->gather_ok && f->summed;
+  KG_HIP(launch_flock(f->worklist(), f->R.b, f->offset, f->W.b,
+                      summed ? f->counts(f->parity) : f->count, gstep, f->err,
                       f->lds_cap, f->gflag, (uint32_t)f->nR, f->stream,
                       f->gather_ok ? f->outbox : Outbox{}, f->global_inline,
-                      f->gather_ok && f->summed ? f->tsum[f->parity] : nullptr));
+                      summed ? f->tsum[f->parity] : nullptr));
   f->outbox_filled = f->gather_ok;
   f->nW = f->nR;
   f->w_aligned = true;
@@ -500,8 +515,10 @@ int kg_field_create(const kg_field_desc* desc, kg_field** out) {
     if ((e = dmalloc(&f->outbox.n, f->g.ntiles)) != hipSuccess) return fail(e);
     if ((e = dmalloc(&f->outbox.slot, (size_t)f->g.ntiles * kOutboxCap)) != hipSuccess)
       return fail(e);
-    if ((e = dmalloc(&f->gate, 2)) != hipSuccess) return fail(e);
-    if ((e = hipMemset(f->gate, 0, 2 * sizeof(uint32_t))) != hipSuccess) return fail(e);
+    if ((e = dmalloc(&f->gate, 3)) != hipSuccess) return fail(e);
+    if ((e = hipMemset(f->gate, 0, 3 * sizeof(uint32_t))) != hipSuccess) return fail(e);
+    if ((e = dmalloc(&f->count2, nb)) != hipSuccess) return fail(e);
+    if ((e = hipMemset(f->count2, 0, nb * sizeof(uint32_t))) != hipSuccess) return fail(e);
     if ((e = hipHostMalloc((void**)&f->seen, 2 * sizeof(uint32_t))) != hipSuccess)
       return fail(e);
     f->seen[0] = f->seen[1] = 0;
@@ -559,6 +576,7 @@ void kg_field_destroy(kg_field* f) {
   (void)hipFree(f->gate);
   (void)hipFree(f->tsum[0]);
   (void)hipFree(f->tsum[1]);
+  (void)hipFree(f->count2);
   if (f->seen) (void)hipHostFree(f->seen);
   if (f->stream) (void)hipStreamDestroy(f->stream);
   delete f;
@@ -1020,7 +1038,7 @@ int kg_schedule_run(kg_field* f, const kg_flock_params* p, uint64_t first_step, 
     if (int rc = run_direct_step(f)) return rc;
     --left;
     uint32_t counts[2] = {0, 0};
-    KG_HIP(hipMemcpyAsync(&counts[0], f->gate + 1, sizeof(uint32_t), hipMemcpyDeviceToHost,
+    KG_HIP(hipMemcpyAsync(&counts[0], f->gate + 2, sizeof(uint32_t), hipMemcpyDeviceToHost,
                           f->stream));
     KG_HIP(hipMemcpyAsync(&counts[1], f->gflag + f->g.ntiles + 1, sizeof(uint32_t),
                           hipMemcpyDeviceToHost, f->stream));
@@ -1036,9 +1054,11 @@ int kg_schedule_run(kg_field* f, const kg_flock_params* p, uint64_t first_step, 
   const uint32_t nscan = (f->g.nbins + kScanTile - 1) / kScanTile;
   f->summed = f->gather_ok && f->global_inline && nscan <= kScanInlineTiles &&
               nbt <= kGatherMaxBins && (uint32_t)kScanTile % nbt == 0;
-  if (f->summed)
+  if (f->summed) {
     for (int k = 0; k < 2; ++k)
       KG_HIP(hipMemsetAsync(f->tsum[k], 0, nscan * sizeof(uint32_t), f->stream));
+    KG_HIP(hipMemsetAsync(f->gate, 0, 2 * sizeof(uint32_t), f->stream));
+  }
   if ((use_graph & KG_RUN_GRAPH) && left >= 2 && !f->prof_on) {
     GraphKey key;
     std::memcpy(key.params, p, sizeof(kg_flock_params));
@@ -1076,14 +1096,19 @@ int kg_schedule_run(kg_field* f, const kg_flock_params* p, uint64_t first_step, 
   }
   if (f->gather_ok) {
     f->gather_steps += nsteps;
-    KG_HIP(hipMemcpyAsync(f->seen, f->gate + 1, sizeof(uint32_t), hipMemcpyDeviceToHost,
+    KG_HIP(hipMemcpyAsync(f->seen, f->gate + 2, sizeof(uint32_t), hipMemcpyDeviceToHost,
                           f->stream));
     f->gather_steps_copied = f->gather_steps;
   }
   if (f->global_inline)
     KG_HIP(hipMemcpyAsync(f->seen + 1, f->gflag + f->g.ntiles + 1, sizeof(uint32_t),
                           hipMemcpyDeviceToHost, f->stream));
-  f->summed = false;  // flock steps outside a run count the usual way
+  if (f->summed) {
+    // the last step's counts were read, not cleared: bins are counted from
+    // zero in the main array outside a run
+    KG_HIP(launch_fill_u32(f->counts(f->parity ^ 1), 0u, f->g.nbins, f->stream));
+    f->summed = false;
+  }
   return KG_OK;
 }
 

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_scatter_aligned(const RepDev* __restric
                                                          uint32_t* __restrict__ cursor, Buf R2,
                                                          const uint32_t* __restrict__ gate,
                                                          uint32_t nitems) {
-  if (gate && gate[0] == 0u) return;  // the gather rebinned this step
+  if (gate && *gate == 0u) return;  // the gather rebinned this step
   for (uint32_t bi = blockIdx.x; bi < nitems; bi += gridDim.x) {
     const uint32_t item = xcd_remap(bi, nitems);
     const TileRange tr = tile_range(reps, nreps, tile_prefix, item, offset_cur);
@@ -162,10 +162,15 @@ hipError_t launch_scatter_aligned(const WorkList& wl, Buf R, Buf W, const uint32
 // Each thread keeps its candidates in registers from load to store, so a
 // bird is read once and written once; a tile with more candidates (or new
 // birds) than kGatherCap sends the step to the scatter and the bin sort.
+// With sc.count the work-group also does the scan's part for its tile: its
+// bins' new offsets follow from the earlier scan tiles' sums and the counts
+// of the earlier bins of its own scan tile (read from this step's counts,
+// which nothing clears in this step), and it writes them to offset_new.
 __global__ __launch_bounds__(kGatherThreads) void k_rebin_gather(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
-    Buf W, const uint32_t* __restrict__ offset_cur, const uint32_t* __restrict__ offset_new,
-    Outbox ob, Buf R2, uint32_t* __restrict__ gate) {
+    Buf W, const uint32_t* __restrict__ offset_cur, uint32_t* __restrict__ offset_new, Outbox ob,
+    Buf R2, Gate gate, GatherScan sc) {
+  __shared__ uint32_t s_warp[kGatherThreads / kWave];
   __shared__ uint32_t s_bstart[kGatherMaxBins + 1];
   __shared__ uint32_t s_fill[kGatherMaxBins];
   __shared__ uint64_t s_key[kGatherCap];
@@ -180,8 +185,38 @@ __global__ __launch_bounds__(kGatherThreads) void k_rebin_gather(
   const uint32_t nbt = 1u << (g.tx_log2 + g.tr_log2);
   const uint32_t tb0 = rp.bin_base + tile * nbt;
   const uint32_t lo = offset_cur[tb0], nown = offset_cur[tb0 + nbt] - lo;
-  const uint32_t nlo = offset_new[tb0];
-  for (uint32_t b = t; b <= nbt; b += kGatherThreads) s_bstart[b] = offset_new[tb0 + b] - nlo;
+  uint32_t nlo;
+  if (sc.count) {
+    // before the tile: the earlier scan tiles' sums and the earlier bins of
+    // its own scan tile; one bin per thread (nbt <= kGatherThreads)
+    const uint32_t q = tb0 / (uint32_t)kScanTile;
+    uint32_t part = 0;
+    for (uint32_t i = t; i < q; i += kGatherThreads) part += sc.tile_sum[i];
+    for (uint32_t b = q * (uint32_t)kScanTile + t; b < tb0; b += kGatherThreads)
+      part += sc.count[b];
+    block_exclusive_scan<kGatherThreads>(part, s_warp, &nlo);
+    const uint32_t c = t < nbt ? sc.count[tb0 + t] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan<kGatherThreads>(c, s_warp, &tot);
+    if (t < nbt) {
+      s_bstart[t] = ex;
+      offset_new[tb0 + t] = nlo + ex;
+      sc.cursor[tb0 + t] = nlo + ex;
+      sc.count_next[tb0 + t] = 0u;
+    }
+    if (t == 0) {
+      s_bstart[nbt] = tot;
+      if (item == sc.last_item) offset_new[sc.nbins] = nlo + tot;
+      if (tb0 % (uint32_t)kScanTile == 0) sc.tile_sum_next[q] = 0u;
+      if (item == 0) {
+        *sc.gstep += 1;
+        if (gate.clear) *gate.clear = 0u;
+      }
+    }
+  } else {
+    nlo = offset_new[tb0];
+    for (uint32_t b = t; b <= nbt; b += kGatherThreads) s_bstart[b] = offset_new[tb0 + b] - nlo;
+  }
   if (t < kGatherSources) {
     const int32_t sl = gather_source(g, tile, t);
     const uint32_t it = sl >= 0 ? tile_prefix[r] + (uint32_t)sl : 0u;
@@ -248,15 +283,15 @@ __global__ __launch_bounds__(kGatherThreads) void k_rebin_gather(
       }
     }
   }
-  if (t == 0 && s_fail) gate[0] = 1u;
+  if (t == 0 && s_fail) *gate.flag = 1u;
 }
 
 hipError_t launch_rebin_gather(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
-                               const uint32_t* offset_new, const Outbox& ob, Buf R2,
-                               uint32_t* gate, hipStream_t s) {
+                               uint32_t* offset_new, const Outbox& ob, Buf R2, const Gate& gate,
+                               const GatherScan& sc, hipStream_t s) {
   if (wl.ntiles == 0) return hipSuccess;
   hipLaunchKernelGGL(k_rebin_gather, dim3(wl.ntiles), dim3(kGatherThreads), 0, s, wl.reps,
-                     wl.nreps, wl.tile_prefix, R, W, offset_cur, offset_new, ob, R2, gate);
+                     wl.nreps, wl.tile_prefix, R, W, offset_cur, offset_new, ob, R2, gate, sc);
   return hipGetLastError();
 }
 
@@ -327,8 +362,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
                                                             uint32_t* __restrict__ cursor, uint32_t n,
                                                             const uint32_t* __restrict__ partial,
                                                             int prefixed, uint64_t* gstep,
-                                                            uint32_t* gate,
-                                                            uint32_t* __restrict__ zero_next) {
+                                                            uint32_t* gate) {
   __shared__ uint32_t s_warp[kScanThreads / kWave];
   const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
   // the tiles before this one (at most kScanInlineTiles sums, cached)
@@ -364,7 +398,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
     if (gstep) *gstep += 1;
     if (gate) *gate = 0u;
   }
-  if (zero_next && threadIdx.x == 0) zero_next[blockIdx.x] = 0u;
 }
 
 hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint32_t nbins,
@@ -376,16 +409,7 @@ hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint
   if (prefixed)
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, ss.partial, ntiles);
   hipLaunchKernelGGL(k_scan_down, dim3(ntiles), dim3(kScanThreads), 0, s, count, offset, cursor,
-                     nbins, ss.partial, prefixed, gstep, gate, nullptr);
-  return hipGetLastError();
-}
-
-hipError_t launch_scan_summed(uint32_t* count, uint32_t* offset, uint32_t* cursor,
-                              uint32_t nbins, const uint32_t* sums, uint32_t* sums_next,
-                              uint64_t* gstep, uint32_t* gate, hipStream_t s) {
-  const uint32_t ntiles = (nbins + kScanTile - 1) / kScanTile;
-  hipLaunchKernelGGL(k_scan_down, dim3(ntiles), dim3(kScanThreads), 0, s, count, offset, cursor,
-                     nbins, sums, 0, gstep, gate, sums_next);
+                     nbins, ss.partial, prefixed, gstep, gate);
   return hipGetLastError();
 }
 
@@ -752,9 +776,9 @@ __device__ __forceinline__ void sort_wave_bin(const Geom& g, Buf R, uint32_t s, 
 
 __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
-    const uint32_t* __restrict__ offset, Buf T, uint32_t* __restrict__ gate, uint32_t nitems) {
-  if (gate && gate[0] == 0u) return;  // the gather rebinned this step
-  if (gate && blockIdx.x == 0 && threadIdx.x == 0) gate[1] += 1u;  // steps redone
+    const uint32_t* __restrict__ offset, Buf T, Gate gate, uint32_t nitems) {
+  if (gate.flag && *gate.flag == 0u) return;  // the gather rebinned this step
+  if (gate.redone && blockIdx.x == 0 && threadIdx.x == 0) *gate.redone += 1u;  // steps redone
   __shared__ uint64_t s_key[kSortChunk];
   __shared__ uint32_t s_seg[kSortThreads];
   __shared__ uint32_t s_mid[kSortThreads];
@@ -857,9 +881,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
 }
 
 hipError_t launch_sort_bins(const WorkList& wl, Buf R, const uint32_t* offset, Buf T,
-                            hipStream_t s, uint32_t* gate) {
+                            hipStream_t s, const Gate& gate) {
   if (wl.ntiles == 0) return hipSuccess;
-  const uint32_t grid = gate && wl.ntiles > kGatedGrid ? kGatedGrid : wl.ntiles;
+  const uint32_t grid = gate.flag && wl.ntiles > kGatedGrid ? kGatedGrid : wl.ntiles;
   hipLaunchKernelGGL(k_sort_bins, dim3(grid), dim3(kSortThreads), 0, s, wl.reps, wl.nreps,
                      wl.tile_prefix, R, offset, T, gate, wl.ntiles);
   return hipGetLastError();

@@ -19,12 +19,6 @@ hipError_t launch_count_append(const WorkList& wl, Buf A, uint32_t* count, uint3
 hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint32_t nbins,
                        const ScanState& ss, uint64_t* gstep, hipStream_t s,
                        uint32_t* gate = nullptr);
-// The scan in one launch, with the sums of its tiles (kScanTile bins each,
-// at most kScanInlineTiles of them) already in `sums`, as a flock step with
-// tile sums leaves them; sums_next, the other step's sums, is cleared.
-hipError_t launch_scan_summed(uint32_t* count, uint32_t* offset, uint32_t* cursor,
-                              uint32_t nbins, const uint32_t* sums, uint32_t* sums_next,
-                              uint64_t* gstep, uint32_t* gate, hipStream_t s);
 
 // gate (or null): with a gate the kernel runs only if the gather set gate[0]
 hipError_t launch_scatter_aligned(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
@@ -32,19 +26,20 @@ hipError_t launch_scatter_aligned(const WorkList& wl, Buf R, Buf W, const uint32
                                   const uint32_t* gate = nullptr);
 // The gather rebinning (one work-group per tile; tiles of at most
 // kGatherMaxBins bins) of the flock step that filled the outboxes: R2 in
-// bin order with every bin sorted, or gate[0] set for the scatter and the
-// bin sort to redo it.
+// bin order with every bin sorted, or gate.flag set for the scatter and the
+// bin sort to redo it.  sc.count: the scan is done here too (offset_new is
+// written); else offset_new holds the scan's offsets.
 hipError_t launch_rebin_gather(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
-                               const uint32_t* offset_new, const Outbox& ob, Buf R2,
-                               uint32_t* gate, hipStream_t s);
+                               uint32_t* offset_new, const Outbox& ob, Buf R2, const Gate& gate,
+                               const GatherScan& sc, hipStream_t s);
 hipError_t launch_scatter_append(const WorkList& wl, Buf A, uint32_t* cursor, Buf R2,
                                  hipStream_t s);
 // Sorts every bin of R (offsets `offset`) by (cell y, id); T is scratch of
 // R's capacity whose contents are lost.
-// gate (or null): as for the scatter; a sort that runs also counts the step
-// in gate[1]
+// gate.flag (or null): as for the scatter; a sort that runs also counts the
+// step in gate.redone
 hipError_t launch_sort_bins(const WorkList& wl, Buf R, const uint32_t* offset, Buf T,
-                            hipStream_t s, uint32_t* gate = nullptr);
+                            hipStream_t s, const Gate& gate = Gate{});
 // gflag: one word per tile plus two (the last counts global-path tiles and
 // starts at zero), nagents: R's count; ob: the outboxes for the gather
 // rebinning that follows, or none; inline_global: each global-path tile is

@@ -103,6 +103,32 @@ struct Outbox {
   uint32_t* slot = nullptr;  // per tile: kOutboxCap read-buffer indices
 };
 
+// The gate of a gathered step: the flag a tile that cannot rebin sets (the
+// scatter and the bin sort run only if it is set), the count of steps they
+// redid, and the flag the gather clears for the next step (or none: the
+// scan clears this step's).
+struct Gate {
+  uint32_t* flag = nullptr;
+  uint32_t* redone = nullptr;
+  uint32_t* clear = nullptr;
+};
+
+// The scan done inside the gather, for a field whose flock step left the
+// sums of the scan tiles (whole tiles each): this step's bin counts and
+// sums, the next step's (cleared here), the scatter cursors and new offsets
+// to fill, the schedule-step counter to advance, the bin count and the last
+// tile.  count == null: the offsets come from the scan kernels.
+struct GatherScan {
+  uint32_t* count = nullptr;
+  const uint32_t* tile_sum = nullptr;
+  uint32_t* count_next = nullptr;
+  uint32_t* tile_sum_next = nullptr;
+  uint32_t* cursor = nullptr;
+  uint64_t* gstep = nullptr;
+  uint32_t nbins = 0;
+  uint32_t last_item = 0;
+};
+
 // Entry i (0..3) of the tile columns (or rows) of n whose birds can enter
 // column c in one step, or -1: c - 1, c, c + 1 cyclically and, across the
 // seam of a torus whose last tile column may be narrower than a step, the

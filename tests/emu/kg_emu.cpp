@@ -565,14 +565,27 @@ int emu_lazy_update(emu_field* f) {
   const bool gather = f->w_aligned && f->outbox_filled && f->nA == 0;
   f->outbox_filled = false;
   if (f->summed_step) {
-    // the tile sums the flock step left equal the scan's own
+    // the tile sums the flock step left equal the scan's own, and each
+    // tile's first new offset as the gather finds it (the earlier scan
+    // tiles' sums plus the earlier bins of its own scan tile) equals the
+    // exclusive scan's
+    const uint32_t nbt = 1u << (f->g.tx_log2 + f->g.tr_log2);
+    std::vector<uint32_t> ex(f->g.nbins + 1, 0u);
+    for (uint32_t b = 0; b < f->g.nbins; ++b) ex[b + 1] = ex[b] + f->count[b];
     for (uint32_t q = 0; q < f->tsum.size(); ++q) {
       uint32_t sum = 0;
       for (uint32_t b = q * kScanTile; b < std::min<uint32_t>(f->g.nbins, (q + 1) * kScanTile); ++b)
         sum += f->count[b];
       if (sum != f->tsum[q]) return -8;
-      f->tsum[q] = 0;
     }
+    for (uint32_t tile = 0; tile < f->g.ntiles; ++tile) {
+      const uint32_t tb0 = tile * nbt, q = tb0 / kScanTile;
+      uint32_t nlo = 0;
+      for (uint32_t i = 0; i < q; ++i) nlo += f->tsum[i];
+      for (uint32_t b = q * kScanTile; b < tb0; ++b) nlo += f->count[b];
+      if (nlo != ex[tb0]) return -9;
+    }
+    std::fill(f->tsum.begin(), f->tsum.end(), 0u);
     f->summed_step = false;
   }
   Buf A = f->A.view();
