@@ -318,9 +318,8 @@ hipError_t launch_materialize(Buf R, Buf W, uint32_t n, Buf A, hipStream_t s) {
 // of the tile count), three: a one-block scan of the tile sums between them.
 // No work-group waits on another, so nothing depends on inter-work-group
 // ordering.  The last launch also zeroes the counts (for the next
-// histogram), fills the scatter cursors and advances the schedule-step
-// counter.
-
+// histogram), fills the scatter cursors, clears the step's gate flag and
+// advances the schedule-step counter.
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ count,
                                                               uint32_t n,

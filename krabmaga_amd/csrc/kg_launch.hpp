@@ -15,12 +15,12 @@ hipError_t launch_init_uniform(Buf A, uint32_t base, uint32_t n, float w, float 
                                int heading_mode, hipStream_t s);
 hipError_t launch_count_append(const WorkList& wl, Buf A, uint32_t* count, uint32_t* err,
                                hipStream_t s);
-// gate (or null): the rebinning gate word, cleared for the step that follows
+// gate (or null): this step's gate flag, cleared for the gather that follows
 hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint32_t nbins,
                        const ScanState& ss, uint64_t* gstep, hipStream_t s,
                        uint32_t* gate = nullptr);
 
-// gate (or null): with a gate the kernel runs only if the gather set gate[0]
+// gate (or null): the step's gate flag; the kernel runs only if the gather set it
 hipError_t launch_scatter_aligned(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
                                   uint32_t* cursor, Buf R2, hipStream_t s,
                                   const uint32_t* gate = nullptr);
