@@ -335,23 +335,25 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __
   if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
-// One block: partial[0..m) becomes its exclusive prefix.
+// One block: partial[0..m) becomes its exclusive prefix.  Each thread sums
+// a run of consecutive entries (loads unrolled so several are in flight),
+// one block scan turns the run sums into run prefixes, and each thread
+// writes its run's prefixes.
 __global__ __launch_bounds__(1024) void k_scan_partials(uint32_t* __restrict__ partial,
                                                         uint32_t m) {
   __shared__ uint32_t s_warp[1024 / kWave];
-  __shared__ uint32_t s_carry;
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  for (uint32_t base = 0; base < m; base += 1024) {
-    const uint32_t i = base + threadIdx.x;
-    const uint32_t v = i < m ? partial[i] : 0u;
-    uint32_t total;
-    const uint32_t ex = block_exclusive_scan<1024>(v, s_warp, &total);
-    const uint32_t carry = s_carry;
-    if (i < m) partial[i] = carry + ex;
-    __syncthreads();
-    if (threadIdx.x == 0) s_carry = carry + total;
-    __syncthreads();
+  const uint32_t per = (m + 1023u) / 1024u;
+  const uint32_t lo = min(m, threadIdx.x * per), hi = min(m, lo + per);
+  uint32_t sum = 0;
+#pragma unroll 8
+  for (uint32_t i = lo; i < hi; ++i) sum += partial[i];
+  uint32_t total;
+  uint32_t run = block_exclusive_scan<1024>(sum, s_warp, &total);
+#pragma unroll 8
+  for (uint32_t i = lo; i < hi; ++i) {
+    const uint32_t v = partial[i];
+    partial[i] = run;
+    run += v;
   }
 }
 
