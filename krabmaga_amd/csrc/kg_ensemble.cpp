@@ -127,6 +127,15 @@ struct kg_ensemble {
   size_t cap_outbox = 0;
   uint32_t* gate = nullptr;
   uint32_t redone_seen = 0;
+  // gathered steps whose counts go straight into the scan tiles' sums (every
+  // run's bins start on a scan tile): the flock step counts into
+  // counts(parity) and tsum[parity], one block turns the sums into
+  // prefixes and the gather does the rest of the scan, clearing the other
+  // parity's counts, sums and gate flag for the next step
+  bool summed = false;
+  uint32_t* count2 = nullptr;      // with the bin arrays (cap_bins)
+  uint32_t* tsum[2] = {nullptr, nullptr};  // with the bin arrays (one per scan tile)
+  uint32_t* counts(int p) const { return p ? count2 : count; }
 
   // current layout
   std::vector<RepDev> h_reps;
@@ -149,7 +158,8 @@ struct kg_ensemble {
   }
 
   size_t bytes_held() const {
-    return (cap_a + cap_a2 + cap_r + cap_r2 + cap_w) * 20 + cap_bins * 16 + cap_gflag * 4 +
+    return (cap_a + cap_a2 + cap_r + cap_r2 + cap_w) * 20 + cap_bins * 20 +
+           (cap_bins / kScanTile + 2) * 12 + cap_gflag * 4 +
            cap_outbox * (kOutboxCap + 1) * 4 +
            rt.stay_alloc * 40 + rt.inbox_alloc * 40 + rt.spill_alloc * 44 +
            rt.boff_alloc * 8 + cap_tiles * 48;
@@ -162,6 +172,12 @@ struct kg_ensemble {
 };
 
 namespace {
+
+// Every run's bins start on a scan tile, so the scan tiles' sums that the
+// flock step leaves are each one run's.
+uint64_t scan_tile_align(uint64_t bins) {
+  return (bins + kScanTile - 1) / kScanTile * kScanTile;
+}
 
 template <class T>
 hipError_t regrow(T** p, size_t* cap, size_t want) {
@@ -196,22 +212,26 @@ int reserve(kg_ensemble* e, size_t bins, size_t reps, size_t tiles) {
   if (bins > e->cap_bins) {
     KG_HIP(hipStreamSynchronize(e->stream));
     const size_t n = std::max(bins, e->cap_bins + e->cap_bins / 2);
-    (void)hipFree(e->count);
-    (void)hipFree(e->offset);
-    (void)hipFree(e->offset2);
-    (void)hipFree(e->cursor);
-    (void)hipFree(e->ss.partial);
-    e->count = e->offset = e->offset2 = e->cursor = e->ss.partial = nullptr;
+    for (uint32_t* p : {e->count, e->count2, e->offset, e->offset2, e->cursor, e->ss.partial,
+                        e->tsum[0], e->tsum[1]})
+      (void)hipFree(p);
+    e->count = e->count2 = e->offset = e->offset2 = e->cursor = e->ss.partial = nullptr;
+    e->tsum[0] = e->tsum[1] = nullptr;
     e->cap_bins = 0;
     const size_t nscan = (n + kScanTile - 1) / kScanTile + 1;
     // offset and offset2 have the same size: the bin-grouped steps swap them
     if ((err = dmalloc(&e->count, n)) != hipSuccess || (err = dmalloc(&e->offset, n + 1)) != hipSuccess ||
         (err = dmalloc(&e->offset2, n + 1)) != hipSuccess ||
         (err = dmalloc(&e->cursor, n)) != hipSuccess ||
-        (err = dmalloc(&e->ss.partial, nscan)) != hipSuccess)
+        (err = dmalloc(&e->ss.partial, nscan)) != hipSuccess ||
+        (err = dmalloc(&e->count2, n)) != hipSuccess ||
+        (err = dmalloc(&e->tsum[0], nscan)) != hipSuccess ||
+        (err = dmalloc(&e->tsum[1], nscan)) != hipSuccess)
       return set_error(KG_E_NOMEM, "ensemble bin arrays: %s", hipGetErrorString(err));
-    // the scan leaves the counts at zero; a fresh array starts there
+    // the scan leaves the counts at zero, and a summed run leaves both
+    // arrays there; a fresh array starts there
     KG_HIP(hipMemsetAsync(e->count, 0, n * sizeof(uint32_t), e->stream));
+    KG_HIP(hipMemsetAsync(e->count2, 0, n * sizeof(uint32_t), e->stream));
     e->cap_bins = n;
   }
   if (reps > e->cap_reps) {
@@ -383,7 +403,7 @@ int rebuild(kg_ensemble* e) {
   for (int attempt = 0;; ++attempt) {
     uint64_t lbins = 0, ltiles = 0;
     for (const EnsRun& r : e->runs) {
-      lbins += r.g.nbins;
+      lbins = scan_tile_align(lbins) + r.g.nbins;
       ltiles += r.g.ntiles;
     }
     if (lbins >= (1ull << 31) || ltiles >= (1ull << 31))
@@ -417,7 +437,7 @@ int rebuild(kg_ensemble* e) {
       infos[i] = RtRepInfo{r.g, rp.disc_dist, tile0};
       e->h_prefix[i] = tile0;
       e->h_prefix[nreps + 1 + i] = r.app_base;
-      bin_base += r.g.nbins;
+      bin_base = (uint32_t)scan_tile_align(bin_base + r.g.nbins);
       boff_base += r.g.ntiles * (uint32_t)(r.g.tx * r.g.tr + 1);
       tile0 += r.g.ntiles;
     }
@@ -425,7 +445,7 @@ int rebuild(kg_ensemble* e) {
     e->h_prefix[2 * nreps + 1] = base;
     e->nreps = nreps;
     e->ntiles = tile0;
-    e->nbins = bin_base;
+    e->nbins = nreps ? e->h_reps[nreps - 1].bin_base + e->runs[nreps - 1].g.nbins : 0;
     e->nagents = base;
     e->dead_tiles = 0;
     e->built = true;
@@ -506,18 +526,37 @@ int rebuild(kg_ensemble* e) {
 // scatter and the bin sort only if it cannot), then the buffers swap.
 hipError_t binned_step(kg_ensemble* e, const WorkList& w) {
   hipError_t err;
-  uint32_t* gate = e->gather ? e->gate : nullptr;
-  if ((err = launch_flock(w, e->R.b, e->offset, e->W.b, e->count, e->gstep, e->err, e->lds_cap,
+  const int p = e->parity;
+  const bool summed = e->summed;
+  uint32_t* gate = e->gather ? e->gate + (summed ? p : 0) : nullptr;
+  uint32_t* count = summed ? e->counts(p) : e->count;
+  if ((err = launch_flock(w, e->R.b, e->offset, e->W.b, count, e->gstep, e->err, e->lds_cap,
                           e->gflag, e->nagents, e->stream,
-                          e->gather ? e->outbox : Outbox{})) != hipSuccess)
+                          e->gather ? e->outbox : Outbox{}, false,
+                          summed ? e->tsum[p] : nullptr)) != hipSuccess)
     return err;
-  if ((err = launch_scan(e->count, e->offset2, e->cursor, e->nbins, e->ss, e->gstep, e->stream,
-                         gate)) != hipSuccess)
-    return err;
-  const Gate gt{gate, gate ? e->gate + 2 : nullptr, nullptr};
-  if (gate && (err = launch_rebin_gather(w, e->R.b, e->W.b, e->offset, e->offset2, e->outbox,
-                                         e->R2.b, gt, GatherScan{}, e->stream)) != hipSuccess)
-    return err;
+  const uint32_t nscan = (e->nbins + kScanTile - 1) / kScanTile;
+  if (summed) err = launch_scan_prefix(e->tsum[p], nscan, e->stream);
+  else err = launch_scan(e->count, e->offset2, e->cursor, e->nbins, e->ss, e->gstep, e->stream, gate);
+  if (err != hipSuccess) return err;
+  const Gate gt{gate, gate ? e->gate + 2 : nullptr, summed ? e->gate + (p ^ 1) : nullptr};
+  if (gate) {
+    GatherScan sc;
+    if (summed) {
+      sc.count = count;
+      sc.tile_sum = e->tsum[p];
+      sc.count_next = e->counts(p ^ 1);
+      sc.tile_sum_next = e->tsum[p ^ 1];
+      sc.cursor = e->cursor;
+      sc.gstep = e->gstep;
+      sc.nbins = e->nbins;
+      sc.last_item = e->ntiles - 1;
+      sc.prefixed = 1;
+    }
+    if ((err = launch_rebin_gather(w, e->R.b, e->W.b, e->offset, e->offset2, e->outbox, e->R2.b,
+                                   gt, sc, e->stream)) != hipSuccess)
+      return err;
+  }
   if ((err = launch_scatter_aligned(w, e->R.b, e->W.b, e->offset, e->cursor, e->R2.b, e->stream,
                                     gate)) != hipSuccess)
     return err;
@@ -536,6 +575,15 @@ hipError_t binned_step(kg_ensemble* e, const WorkList& w) {
 int binned_steps(kg_ensemble* e, uint32_t n, bool use_graph) {
   const WorkList w = e->wl();
   uint32_t left = n;
+  // every gathered step is summed: the sums and both gate flags start at
+  // zero, and so do both count arrays outside a run
+  e->summed = e->gather;
+  if (e->summed) {
+    const uint32_t nscan = (e->nbins + kScanTile - 1) / kScanTile;
+    for (int k = 0; k < 2; ++k)
+      KG_HIP(hipMemsetAsync(e->tsum[k], 0, nscan * sizeof(uint32_t), e->stream));
+    KG_HIP(hipMemsetAsync(e->gate, 0, 2 * sizeof(uint32_t), e->stream));
+  }
   if (use_graph && left >= 2) {
     hipGraphExec_t exec = nullptr;
     const int key = e->parity + (e->gather ? 2 : 0);
@@ -562,6 +610,11 @@ int binned_steps(kg_ensemble* e, uint32_t n, bool use_graph) {
     }
   }
   for (; left > 0; --left) KG_HIP(binned_step(e, w));
+  if (e->summed) {
+    // the last step's counts were read, not cleared
+    KG_HIP(launch_fill_u32(e->counts(e->parity ^ 1), 0u, e->nbins, e->stream));
+    e->summed = false;
+  }
   e->steps_run += n;
   for (EnsRun& r : e->runs) r.steps_done += n;
   return KG_OK;
@@ -663,7 +716,7 @@ int kg_run_footprint(const kg_run_desc* run, const kg_tuning* tuning, uint64_t* 
   // about 2.5x the count) and inboxes, all with the growth headroom of half
   // again that reserve_buf and the layout give them; spill, bins (count,
   // offset, cursor) and the per-tile tables
-  *bytes = n * 20 * 3 * 3 / 2 + n * 40 * 3 * 3 / 2 + n * 44 + (uint64_t)g.nbins * 12 +
+  *bytes = n * 20 * 3 * 3 / 2 + n * 40 * 3 * 3 / 2 + n * 44 + ((uint64_t)g.nbins + kScanTile) * 20 +
            (uint64_t)g.ntiles * ((uint64_t)(g.tx * g.tr + 1) * 8 + 64 + (kOutboxCap + 1) * 4) *
                3 / 2;
   return KG_OK;
@@ -704,6 +757,7 @@ void kg_ensemble_destroy(kg_ensemble* e) {
                   (void*)e->rep_cursor, (void*)e->tcount, (void*)e->err, (void*)e->gflag,
                   (void*)e->gstep, (void*)e->outbox.n, (void*)e->outbox.slot, (void*)e->gate})
     (void)hipFree(p);
+  for (void* p : {(void*)e->count2, (void*)e->tsum[0], (void*)e->tsum[1]}) (void)hipFree(p);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }

@@ -191,7 +191,10 @@ __global__ __launch_bounds__(kGatherThreads) void k_rebin_gather(
     // its own scan tile; one bin per thread (nbt <= kGatherThreads)
     const uint32_t q = tb0 / (uint32_t)kScanTile;
     uint32_t part = 0;
-    for (uint32_t i = t; i < q; i += kGatherThreads) part += sc.tile_sum[i];
+    if (!sc.prefixed)
+      for (uint32_t i = t; i < q; i += kGatherThreads) part += sc.tile_sum[i];
+    else if (t == 0)
+      part = sc.tile_sum[q];
     for (uint32_t b = q * (uint32_t)kScanTile + t; b < tb0; b += kGatherThreads)
       part += sc.count[b];
     block_exclusive_scan<kGatherThreads>(part, s_warp, &nlo);
@@ -203,6 +206,13 @@ __global__ __launch_bounds__(kGatherThreads) void k_rebin_gather(
       offset_new[tb0 + t] = nlo + ex;
       sc.cursor[tb0 + t] = nlo + ex;
       sc.count_next[tb0 + t] = 0u;
+    }
+    if (tile == (uint32_t)g.ntiles - 1u) {
+      // bins between this replica's last and the next replica's first (on
+      // the next scan tile) hold nobody: they start where this tile ends
+      const uint32_t pad_end = min(sc.nbins, (tb0 + nbt + (uint32_t)kScanTile - 1u) /
+                                                 (uint32_t)kScanTile * (uint32_t)kScanTile);
+      for (uint32_t b = tb0 + nbt + t; b < pad_end; b += kGatherThreads) offset_new[b] = nlo + tot;
     }
     if (t == 0) {
       s_bstart[nbt] = tot;
@@ -335,25 +345,62 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __
   if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
-// One block: partial[0..m) becomes its exclusive prefix.  Each thread sums
-// a run of consecutive entries (loads unrolled so several are in flight),
-// one block scan turns the run sums into run prefixes, and each thread
-// writes its run's prefixes.
+// One block: partial[0..m) becomes its exclusive prefix, in segments of
+// kPrefixSeg entries.  A segment is loaded coalesced (thread t takes entries
+// t, t + 1024, ...) into LDS, each thread scans a run of kPrefixRun
+// consecutive entries from there (the padding word after every 32 keeps the
+// runs' reads on distinct banks), one block scan gives the runs' bases, and
+// the prefixes go back out coalesced.  The next segment's loads are issued
+// before the current one is scanned, so one round trip to memory overlaps
+// the LDS work.
+constexpr uint32_t kPrefixRun = 8;
+constexpr uint32_t kPrefixSeg = 1024 * kPrefixRun;
+
+__device__ __forceinline__ uint32_t prefix_pad(uint32_t i) { return i + (i >> 5); }
+
 __global__ __launch_bounds__(1024) void k_scan_partials(uint32_t* __restrict__ partial,
                                                         uint32_t m) {
   __shared__ uint32_t s_warp[1024 / kWave];
-  const uint32_t per = (m + 1023u) / 1024u;
-  const uint32_t lo = min(m, threadIdx.x * per), hi = min(m, lo + per);
-  uint32_t sum = 0;
-#pragma unroll 8
-  for (uint32_t i = lo; i < hi; ++i) sum += partial[i];
-  uint32_t total;
-  uint32_t run = block_exclusive_scan<1024>(sum, s_warp, &total);
-#pragma unroll 8
-  for (uint32_t i = lo; i < hi; ++i) {
-    const uint32_t v = partial[i];
-    partial[i] = run;
-    run += v;
+  __shared__ uint32_t s_seg[kPrefixSeg + kPrefixSeg / 32];
+  const uint32_t t = threadIdx.x;
+  uint32_t nxt[kPrefixRun];
+#pragma unroll
+  for (uint32_t u = 0; u < kPrefixRun; ++u) {
+    const uint32_t i = u * 1024u + t;
+    nxt[u] = i < m ? partial[i] : 0u;
+  }
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < m; base += kPrefixSeg) {
+#pragma unroll
+    for (uint32_t u = 0; u < kPrefixRun; ++u) s_seg[prefix_pad(u * 1024u + t)] = nxt[u];
+    const uint32_t nb = base + kPrefixSeg;
+#pragma unroll
+    for (uint32_t u = 0; u < kPrefixRun; ++u) {
+      const uint32_t i = nb + u * 1024u + t;
+      nxt[u] = i < m ? partial[i] : 0u;
+    }
+    __syncthreads();
+    uint32_t v[kPrefixRun], sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPrefixRun; ++k) {
+      v[k] = s_seg[prefix_pad(t * kPrefixRun + k)];
+      sum += v[k];
+    }
+    uint32_t total;
+    uint32_t run = carry + block_exclusive_scan<1024>(sum, s_warp, &total);
+#pragma unroll
+    for (uint32_t k = 0; k < kPrefixRun; ++k) {
+      s_seg[prefix_pad(t * kPrefixRun + k)] = run;
+      run += v[k];
+    }
+    carry += total;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t u = 0; u < kPrefixRun; ++u) {
+      const uint32_t i = u * 1024u + t;
+      if (base + i < m) partial[base + i] = s_seg[prefix_pad(i)];
+    }
+    __syncthreads();
   }
 }
 
@@ -399,6 +446,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(uint32_t* __restrict
     if (gstep) *gstep += 1;
     if (gate) *gate = 0u;
   }
+}
+
+hipError_t launch_scan_prefix(uint32_t* sums, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, sums, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint32_t nbins,

@@ -15,6 +15,8 @@ hipError_t launch_init_uniform(Buf A, uint32_t base, uint32_t n, float w, float 
                                int heading_mode, hipStream_t s);
 hipError_t launch_count_append(const WorkList& wl, Buf A, uint32_t* count, uint32_t* err,
                                hipStream_t s);
+// One work-group: sums[0..n) becomes its exclusive prefix.
+hipError_t launch_scan_prefix(uint32_t* sums, uint32_t n, hipStream_t s);
 // gate (or null): this step's gate flag, cleared for the gather that follows
 hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint32_t nbins,
                        const ScanState& ss, uint64_t* gstep, hipStream_t s,

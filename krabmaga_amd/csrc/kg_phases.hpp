@@ -113,11 +113,13 @@ struct Gate {
   uint32_t* clear = nullptr;
 };
 
-// The scan done inside the gather, for a field whose flock step left the
-// sums of the scan tiles (whole tiles each): this step's bin counts and
-// sums, the next step's (cleared here), the scatter cursors and new offsets
-// to fill, the schedule-step counter to advance, the bin count and the last
-// tile.  count == null: the offsets come from the scan kernels.
+// The scan done inside the gather, for a layout whose flock step left the
+// sums of the scan tiles (whole tiles each; every replica's bins start on a
+// scan tile): this step's bin counts and sums (or, with prefixed, the sums'
+// exclusive prefixes), the next step's (cleared here), the scatter cursors
+// and new offsets to fill, the schedule-step counter to advance, the bin
+// count and the last tile.  count == null: the offsets come from the scan
+// kernels.
 struct GatherScan {
   uint32_t* count = nullptr;
   const uint32_t* tile_sum = nullptr;
@@ -127,6 +129,7 @@ struct GatherScan {
   uint64_t* gstep = nullptr;
   uint32_t nbins = 0;
   uint32_t last_item = 0;
+  int prefixed = 0;
 };
 
 // Entry i (0..3) of the tile columns (or rows) of n whose birds can enter
