@@ -524,6 +524,11 @@ int rebuild(kg_ensemble* e) {
 // One bin-grouped schedule step of every replica: k_flock over all tiles,
 // the scan over all bins (it advances the step counter), the gather (the
 // scatter and the bin sort only if it cannot), then the buffers swap.
+// k_flock_global's work-groups in an ensemble step: a layout of many runs
+// seldom has a global-path tile, and a grid of one work-group per run of
+// birds costs about 20 us at 23M birds just to launch and leave.
+constexpr uint32_t kEnsGlobalGrid = 2048;
+
 hipError_t binned_step(kg_ensemble* e, const WorkList& w) {
   hipError_t err;
   const int p = e->parity;
@@ -533,7 +538,7 @@ hipError_t binned_step(kg_ensemble* e, const WorkList& w) {
   if ((err = launch_flock(w, e->R.b, e->offset, e->W.b, count, e->gstep, e->err, e->lds_cap,
                           e->gflag, e->nagents, e->stream,
                           e->gather ? e->outbox : Outbox{}, false,
-                          summed ? e->tsum[p] : nullptr)) != hipSuccess)
+                          summed ? e->tsum[p] : nullptr, kEnsGlobalGrid)) != hipSuccess)
     return err;
   const uint32_t nscan = (e->nbins + kScanTile - 1) / kScanTile;
   if (summed) err = launch_scan_prefix(e->tsum[p], nscan, e->stream);

@@ -593,8 +593,11 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock(
 // buffer holds the replicas one after another (replica r at
 // app_prefix[r], its tiles one after another), each tile's bins in order.
 // kLeavers: outboxes are filled (compiled apart: the listing costs the walk
-// registers).
-template <bool kLeavers>
+// registers).  kStride: each work-group takes every gridDim.x-th run of
+// kFlockThreads birds (a grid much smaller than the runs is cheap to launch
+// in the steps no tile takes this path; the loop costs registers), else
+// work-group i takes run i.
+template <bool kLeavers, bool kStride>
 __global__ __launch_bounds__(kFlockThreads) void k_flock_global(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix,
     const uint32_t* __restrict__ app_prefix, Buf R, const uint32_t* __restrict__ offset, Buf W,
@@ -604,8 +607,6 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock_global(
   __shared__ uint32_t s_any;
   // no tile took the global path in this step
   if (gflag[ntiles] != (uint32_t)*gstep + 1u) return;
-  const uint32_t g0 = blockIdx.x * kFlockThreads;
-  const uint32_t gi = g0 + threadIdx.x;
   // global tile of read-buffer slot q, or false for a bird outside its grid
   auto tile_at = [&](uint32_t q, uint32_t* t) {
     const int r = nreps == 1 ? 0 : find_rep(app_prefix, nreps, q);
@@ -616,42 +617,57 @@ __global__ __launch_bounds__(kFlockThreads) void k_flock_global(
     *t = tile_prefix[r] + tile_of(g, cx, cy);
     return true;
   };
-  // the block's birds belong to the tiles from its first bird's to its last
-  // bird's: skip the block unless one of those took the global path
-  if (threadIdx.x == 0) {
-    const uint32_t g1 = min(n, g0 + kFlockThreads) - 1u;
-    uint32_t t0 = 0, t1 = 0, any = 1u;  // an out-of-grid bird: let every thread look
-    if (tile_at(g0, &t0) && tile_at(g1, &t1) && t1 - t0 < kFlockThreads) {
-      any = 0u;
-      for (uint32_t t = t0; t <= t1; ++t) any |= gflag[t];
+  auto one_run = [&](uint32_t run) {
+    const uint32_t g0 = run * kFlockThreads;
+    const uint32_t gi = g0 + threadIdx.x;
+    // the run's birds belong to the tiles from its first bird's to its last
+    // bird's: skip the run unless one of those took the global path
+    if (threadIdx.x == 0) {
+      const uint32_t g1 = min(n, g0 + kFlockThreads) - 1u;
+      uint32_t t0 = 0, t1 = 0, any = 1u;  // an out-of-grid bird: let every thread look
+      if (tile_at(g0, &t0) && tile_at(g1, &t1) && t1 - t0 < kFlockThreads) {
+        any = 0u;
+        for (uint32_t t = t0; t <= t1; ++t) any |= gflag[t];
+      }
+      s_any = any;
     }
-    s_any = any;
+    __syncthreads();
+    const uint32_t any = s_any;
+    if (kStride) __syncthreads();  // s_any is rewritten for the next run
+    uint32_t t;
+    if (!any || gi >= n || !tile_at(gi, &t) || !gflag[t]) return;
+    const int r = nreps == 1 ? 0 : find_rep(app_prefix, nreps, gi);
+    const RepDev& rp = reps[r];
+    flock_agent_global(rp, rp.step_base + *gstep, gi, R, offset, W, count_next, err,
+                       FlockSinks{kLeavers ? ob : Outbox{}, tile_prefix[r], nullptr, nullptr,
+                                  tile_sum});
+  };
+  if (kStride) {
+    const uint32_t nruns = (n + kFlockThreads - 1) / kFlockThreads;
+    for (uint32_t run = blockIdx.x; run < nruns; run += gridDim.x) one_run(run);
+  } else {
+    one_run(blockIdx.x);
   }
-  __syncthreads();
-  if (!s_any || gi >= n) return;
-  uint32_t t;
-  if (!tile_at(gi, &t) || !gflag[t]) return;
-  const int r = nreps == 1 ? 0 : find_rep(app_prefix, nreps, gi);
-  const RepDev& rp = reps[r];
-  flock_agent_global(rp, rp.step_base + *gstep, gi, R, offset, W, count_next, err,
-                     FlockSinks{kLeavers ? ob : Outbox{}, tile_prefix[r], nullptr, nullptr, tile_sum});
 }
 
 hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W,
                         uint32_t* count_next, const uint64_t* gstep, uint32_t* err, int lds_cap,
                         uint32_t* gflag, uint32_t nagents, hipStream_t s, const Outbox& ob,
-                        bool inline_global, uint32_t* tile_sum) {
+                        bool inline_global, uint32_t* tile_sum, uint32_t global_grid) {
   if (wl.ntiles == 0) return hipSuccess;
   hipLaunchKernelGGL(inline_global ? k_flock<true> : k_flock<false>, dim3(wl.ntiles),
                      dim3(kFlockThreads), flock_lds_bytes(lds_cap), s, wl.reps, wl.nreps,
                      wl.tile_prefix, R, offset, W, count_next, gstep, err, lds_cap, gflag, ob,
                      tile_sum);
-  if (nagents && !inline_global)
-    hipLaunchKernelGGL(ob.n ? k_flock_global<true> : k_flock_global<false>,
-                       dim3((nagents + kFlockThreads - 1) / kFlockThreads),
-                       dim3(kFlockThreads), 0, s, wl.reps, wl.nreps, wl.tile_prefix,
-                       wl.app_prefix, R, offset, W, count_next, gstep, err, gflag, wl.ntiles,
-                       nagents, ob, tile_sum);
+  const uint32_t nruns = (nagents + kFlockThreads - 1) / kFlockThreads;
+  if (nagents && !inline_global) {
+    const bool stride = global_grid && global_grid < nruns;
+    auto kern = stride ? (ob.n ? k_flock_global<true, true> : k_flock_global<false, true>)
+                       : (ob.n ? k_flock_global<true, false> : k_flock_global<false, false>);
+    hipLaunchKernelGGL(kern, dim3(stride ? global_grid : nruns), dim3(kFlockThreads), 0, s,
+                       wl.reps, wl.nreps, wl.tile_prefix, wl.app_prefix, R, offset, W, count_next,
+                       gstep, err, gflag, wl.ntiles, nagents, ob, tile_sum);
+  }
   return hipGetLastError();
 }
 

@@ -47,12 +47,14 @@ hipError_t launch_sort_bins(const WorkList& wl, Buf R, const uint32_t* offset, B
 // rebinning that follows, or none; inline_global: each global-path tile is
 // stepped by its own work-group, and k_flock_global is not launched;
 // tile_sum: the scan's tile sums to add the counts to (zero on entry; tiles
-// of at most kGatherMaxBins bins, whole tiles per scan tile), or none.
+// of at most kGatherMaxBins bins, whole tiles per scan tile), or none;
+// global_grid: k_flock_global's work-groups, each looping over runs of
+// birds, or 0 for one per run.
 hipError_t launch_flock(const WorkList& wl, Buf R, const uint32_t* offset, Buf W,
                         uint32_t* count_next, const uint64_t* gstep, uint32_t* err, int lds_cap,
                         uint32_t* gflag, uint32_t nagents, hipStream_t s,
                         const Outbox& ob = Outbox{}, bool inline_global = false,
-                        uint32_t* tile_sum = nullptr);
+                        uint32_t* tile_sum = nullptr, uint32_t global_grid = 0);
 hipError_t launch_materialize(Buf R, Buf W, uint32_t n, Buf A, hipStream_t s);
 
 // Tile-resident step (one launch per schedule step); with load != 0 the

@@ -160,6 +160,20 @@ def test_ensemble_shortens_tiles_that_fit_no_lds_class():
         assert_same(ens, rid, s, 12, "dense run with shortened tiles")
 
 
+def test_ensemble_binned_global_tiles_among_many_birds():
+    # 560,000 uniform birds make more runs of 256 than the bin-grouped step
+    # gives k_flock_global work-groups, so they loop over the runs; the dense
+    # run's tiles (about 22 birds a cell) exceed every LDS capacity and take
+    # the global path, and overflow the gather, in every step
+    specs = [spec(560000, 171), spec(20000, 172, density=0.5)]
+    with Ensemble() as ens:
+        ids = ens.admit(specs)
+        ens.run(8, binned=True)
+        for rid, s in zip(ids, specs):
+            assert ens.steps(rid) == 8
+            assert_same(ens, rid, s, 8, f"run {rid} (n={s.agents})")
+
+
 def test_ensemble_rejects_non_toroidal_runs():
     s = spec(500, 1)
     s_bad = RunSpec(width=s.width, height=s.height, agents=500, params=s.params)
