@@ -8,9 +8,11 @@ from the seeded Philox stream.  One step = one Schedule::step of the
 flockers model: neighbour query + Bird::step for every agent and the
 rebinning of the field (lazy_update).  State is resident in HBM before the
 timed region starts.  The headline runs the field's default path (at this
-size the three-kernel path: k_flock, the bin-count scan, k_scatter_aligned,
-k_sort_bins); the tile-resident step (one k_rt_step launch per tile class)
-is timed beside it from the same initial state.
+size the three-kernel path: k_flock, then k_rebin_gather, which also does
+the bin-count scan, then k_rebin_fallback, the scatter and bin sort that
+run only in a step the gather could not rebin); the tile-resident step
+(one k_rt_step launch per tile class) is timed beside it from the same
+initial state.
 
 Beside it, in the same JSON line: the clustered-density workload (8,388,608
 boids, 64 Gaussian blobs), the batched ensemble (512 replicas of uneven
@@ -525,9 +527,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ("schedule step: k_flock + k_scan_reduce + k_scan_down + "
-                       "k_rebin_gather + k_scatter_aligned + k_sort_bins (the last two "
-                       "return at once unless the gather could not rebin the step)"
+            "kernel": ("schedule step: k_flock + k_rebin_gather (which does the bin-count "
+                       "scan in this run) + k_rebin_fallback (scatter and bin sort; returns "
+                       "at once unless the gather could not rebin the step)"
                        if three else "schedule step: all k_rt_step launches of the step"),
             "launch_us": step_us,
             "unit_of_launch": "one schedule step",

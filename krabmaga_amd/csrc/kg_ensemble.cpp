@@ -120,7 +120,8 @@ struct kg_ensemble {
   // two-step graphs by parity + 2 * gather
   std::vector<std::pair<int, hipGraphExec_t>> bgraphs;
   // gather rebinning (every run's tiles of at most kGatherMaxBins bins): the
-  // lists of leavers, the gate words (flag, steps redone) and the redone
+  // lists of leavers, the gate words (a flag per step parity, steps redone,
+  // the fallback's grid-wait counter) and the redone
   // count at the last check; a layout whose steps fail it drops it
   bool gather = false;
   Outbox outbox{};
@@ -339,7 +340,7 @@ int reserve_binned(kg_ensemble* e, size_t agents, size_t tiles) {
       return set_error(KG_E_NOMEM, "ensemble outboxes: %s", hipGetErrorString(err));
     e->cap_outbox = n;
   }
-  if (!e->gate && (err = dmalloc(&e->gate, 3)) != hipSuccess)
+  if (!e->gate && (err = dmalloc(&e->gate, 4)) != hipSuccess)
     return set_error(KG_E_NOMEM, "ensemble gate: %s", hipGetErrorString(err));
   return KG_OK;
 }
@@ -505,7 +506,7 @@ int rebuild(kg_ensemble* e) {
     e->gather = rebin_gather_enabled();
     for (const RepDev& rp : e->h_reps)
       if ((uint32_t)(rp.g.tx * rp.g.tr) > kGatherMaxBins) e->gather = false;
-    KG_HIP(hipMemsetAsync(e->gate, 0, 3 * sizeof(uint32_t), e->stream));
+    KG_HIP(hipMemsetAsync(e->gate, 0, 4 * sizeof(uint32_t), e->stream));
     e->redone_seen = 0;
   } else {
     if (int rc = e->rt.setup(plan, cls, e->nagents, e->stream)) return rc;
@@ -544,7 +545,8 @@ hipError_t binned_step(kg_ensemble* e, const WorkList& w) {
   if (summed) err = launch_scan_prefix(e->tsum[p], nscan, e->stream);
   else err = launch_scan(e->count, e->offset2, e->cursor, e->nbins, e->ss, e->gstep, e->stream, gate);
   if (err != hipSuccess) return err;
-  const Gate gt{gate, gate ? e->gate + 2 : nullptr, summed ? e->gate + (p ^ 1) : nullptr};
+  const Gate gt{gate, gate ? e->gate + 2 : nullptr, summed ? e->gate + (p ^ 1) : nullptr,
+                gate ? e->gate + 3 : nullptr};
   if (gate) {
     GatherScan sc;
     if (summed) {
@@ -561,13 +563,19 @@ hipError_t binned_step(kg_ensemble* e, const WorkList& w) {
     if ((err = launch_rebin_gather(w, e->R.b, e->W.b, e->offset, e->offset2, e->outbox, e->R2.b,
                                    gt, sc, e->stream)) != hipSuccess)
       return err;
+    // what the gather could not rebin, in one launch (the write buffer,
+    // consumed by the scatter, is the sort's scratch)
+    if ((err = launch_rebin_fallback(w, e->R.b, e->W.b, e->offset, e->cursor, e->R2.b,
+                                     e->offset2, e->W.b, gt, e->err, e->stream)) != hipSuccess)
+      return err;
+  } else {
+    if ((err = launch_scatter_aligned(w, e->R.b, e->W.b, e->offset, e->cursor, e->R2.b,
+                                      e->stream)) != hipSuccess)
+      return err;
+    // the write buffer has been consumed: it is the sort's scratch
+    if ((err = launch_sort_bins(w, e->R2.b, e->offset2, e->W.b, e->stream)) != hipSuccess)
+      return err;
   }
-  if ((err = launch_scatter_aligned(w, e->R.b, e->W.b, e->offset, e->cursor, e->R2.b, e->stream,
-                                    gate)) != hipSuccess)
-    return err;
-  // the write buffer has been consumed: it is the sort's scratch
-  if ((err = launch_sort_bins(w, e->R2.b, e->offset2, e->W.b, e->stream, gt)) != hipSuccess)
-    return err;
   std::swap(e->R, e->R2);
   std::swap(e->cap_r, e->cap_r2);
   std::swap(e->offset, e->offset2);
@@ -675,6 +683,8 @@ int check_flags(kg_ensemble* e) {
   if (e->binned) {
     uint32_t errflag = 0;
     KG_HIP(hipMemcpy(&errflag, e->err, sizeof(errflag), hipMemcpyDeviceToHost));
+    if (errflag & 2u)
+      return set_error(KG_E_HIP, "the rebinning fallback's grid-wide wait timed out");
     if (errflag) return set_error(KG_E_RANGE, "an agent left its run's bag grid");
     if (e->gather) {
       // steps the gather could not rebin since the last check: a layout

@@ -61,7 +61,8 @@ struct kg_field {
   uint32_t* gflag = nullptr;
   // gather rebinning (tiles of at most kGatherMaxBins bins): the flock
   // kernel's lists of leavers, the gate words (a flag per step parity, the
-  // steps redone) and whether the last flock step filled the lists
+  // steps redone, the fallback's grid-wait counter) and whether the last
+  // flock step filled the lists
   bool gather_ok = false;
   Outbox outbox{};
   uint32_t* gate = nullptr;
@@ -87,6 +88,7 @@ struct kg_field {
   uint32_t nR = 0, nW = 0, nA = 0;
   bool w_aligned = false;
   bool poisoned = false;
+  uint32_t poison_bits = 0;  // the device error flag's bits that poisoned it
 
   // Field2D bookkeeping (field_2d.rs:277, :290-293)
   size_t nagents = 0;
@@ -171,6 +173,10 @@ int prof_collect(kg_field* f) {
 }
 
 int check_poison(kg_field* f) {
+  if (f->poisoned && (f->poison_bits & 2u))
+    return set_error(KG_E_HIP,
+                     "the rebinning fallback's grid-wide wait timed out in an earlier step; "
+                     "the field is unusable");
   if (f->poisoned)
     return set_error(KG_E_RANGE,
                      "a location fell outside the bag grid in an earlier step; the field is "
@@ -185,7 +191,10 @@ int sync_only(kg_field* f) {
   KG_HIP(hipStreamSynchronize(f->stream));
   uint32_t e = 0;
   KG_HIP(hipMemcpy(&e, f->err, sizeof(e), hipMemcpyDeviceToHost));
-  if (e) f->poisoned = true;
+  if (e) {
+    f->poisoned = true;
+    f->poison_bits |= e;
+  }
   return check_poison(f);
 }
 
@@ -196,7 +205,10 @@ int sync_and_check(kg_field* f) {
   KG_HIP(hipStreamSynchronize(f->stream));
   uint32_t e = 0;
   KG_HIP(hipMemcpy(&e, f->err, sizeof(e), hipMemcpyDeviceToHost));
-  if (e) f->poisoned = true;
+  if (e) {
+    f->poisoned = true;
+    f->poison_bits |= e;
+  }
   return check_poison(f);
 }
 
@@ -229,7 +241,8 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
     KG_HIP(launch_scan(f->count, f->offset2, f->cursor, f->g.nbins, f->ss, step_gstep, f->stream,
                        gate));
   }
-  const Gate gt{gate, gather ? f->gate + 2 : nullptr, summed ? f->gate + (p ^ 1) : nullptr};
+  const Gate gt{gate, gather ? f->gate + 2 : nullptr, summed ? f->gate + (p ^ 1) : nullptr,
+                gather ? f->gate + 3 : nullptr};
   if (f->w_aligned) {
     ProfScope ps(f, KG_KERNEL_SCATTER);
     if (gather) {
@@ -246,18 +259,24 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
       }
       KG_HIP(launch_rebin_gather(wl, f->R.b, f->W.b, f->offset, f->offset2, f->outbox,
                                  f->R2.b, gt, sc, f->stream));
+      // what the gather could not rebin: scatter and bin sort in one launch
+      // (no appended agents in a gathered step); the write buffer has been
+      // consumed by then and is the sort's scratch
+      KG_HIP(launch_rebin_fallback(wl, f->R.b, f->W.b, f->offset, f->cursor, f->R2.b,
+                                   f->offset2, f->W.b, gt, f->err, f->stream));
+    } else {
+      KG_HIP(launch_scatter_aligned(wl, f->R.b, f->W.b, f->offset, f->cursor, f->R2.b,
+                                    f->stream));
     }
-    KG_HIP(launch_scatter_aligned(wl, f->R.b, f->W.b, f->offset, f->cursor, f->R2.b, f->stream,
-                                  gate));
   }
-  if (f->nA) {
-    ProfScope ps(f, KG_KERNEL_SCATTER_APPEND);
-    KG_HIP(launch_scatter_append(wl, f->A.b, f->cursor, f->R2.b, f->stream));
-  }
-  {
+  if (!gather) {
+    if (f->nA) {
+      ProfScope ps(f, KG_KERNEL_SCATTER_APPEND);
+      KG_HIP(launch_scatter_append(wl, f->A.b, f->cursor, f->R2.b, f->stream));
+    }
     // the write buffer has been consumed: it is the sort's scratch
     ProfScope ps(f, KG_KERNEL_SORT_BINS);
-    KG_HIP(launch_sort_bins(wl, f->R2.b, f->offset2, f->W.b, f->stream, gt));
+    KG_HIP(launch_sort_bins(wl, f->R2.b, f->offset2, f->W.b, f->stream));
   }
   std::swap(f->R, f->R2);
   std::swap(f->offset, f->offset2);
@@ -515,8 +534,8 @@ int kg_field_create(const kg_field_desc* desc, kg_field** out) {
     if ((e = dmalloc(&f->outbox.n, f->g.ntiles)) != hipSuccess) return fail(e);
     if ((e = dmalloc(&f->outbox.slot, (size_t)f->g.ntiles * kOutboxCap)) != hipSuccess)
       return fail(e);
-    if ((e = dmalloc(&f->gate, 3)) != hipSuccess) return fail(e);
-    if ((e = hipMemset(f->gate, 0, 3 * sizeof(uint32_t))) != hipSuccess) return fail(e);
+    if ((e = dmalloc(&f->gate, 4)) != hipSuccess) return fail(e);
+    if ((e = hipMemset(f->gate, 0, 4 * sizeof(uint32_t))) != hipSuccess) return fail(e);
     if ((e = dmalloc(&f->count2, nb)) != hipSuccess) return fail(e);
     if ((e = hipMemset(f->count2, 0, nb * sizeof(uint32_t))) != hipSuccess) return fail(e);
     if ((e = hipHostMalloc((void**)&f->seen, 2 * sizeof(uint32_t))) != hipSuccess)

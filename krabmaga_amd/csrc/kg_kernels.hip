@@ -85,14 +85,12 @@ hipError_t launch_scatter_append(const WorkList& wl, Buf A, uint32_t* cursor, Bu
 // Rebinning of the write buffer the flock kernel produced: it is aligned with
 // the current read buffer, so work item (replica, tile) moves exactly the
 // agents of that tile's old range.
-__global__ __launch_bounds__(256) void k_scatter_aligned(const RepDev* __restrict__ reps, int nreps,
-                                                         const uint32_t* __restrict__ tile_prefix,
-                                                         Buf R, Buf W,
-                                                         const uint32_t* __restrict__ offset_cur,
-                                                         uint32_t* __restrict__ cursor, Buf R2,
-                                                         const uint32_t* __restrict__ gate,
-                                                         uint32_t nitems) {
-  if (gate && *gate == 0u) return;  // the gather rebinned this step
+__device__ __forceinline__ void scatter_aligned_tiles(const RepDev* __restrict__ reps, int nreps,
+                                                      const uint32_t* __restrict__ tile_prefix,
+                                                      Buf R, Buf W,
+                                                      const uint32_t* __restrict__ offset_cur,
+                                                      uint32_t* __restrict__ cursor, Buf R2,
+                                                      uint32_t nitems) {
   for (uint32_t bi = blockIdx.x; bi < nitems; bi += gridDim.x) {
     const uint32_t item = xcd_remap(bi, nitems);
     const TileRange tr = tile_range(reps, nreps, tile_prefix, item, offset_cur);
@@ -144,6 +142,17 @@ __global__ __launch_bounds__(256) void k_scatter_aligned(const RepDev* __restric
   }
 }
 
+__global__ __launch_bounds__(256) void k_scatter_aligned(const RepDev* __restrict__ reps, int nreps,
+                                                         const uint32_t* __restrict__ tile_prefix,
+                                                         Buf R, Buf W,
+                                                         const uint32_t* __restrict__ offset_cur,
+                                                         uint32_t* __restrict__ cursor, Buf R2,
+                                                         const uint32_t* __restrict__ gate,
+                                                         uint32_t nitems) {
+  if (gate && *gate == 0u) return;  // the gather rebinned this step
+  scatter_aligned_tiles(reps, nreps, tile_prefix, R, W, offset_cur, cursor, R2, nitems);
+}
+
 // Work-groups of a gated scatter or bin sort: in most steps they only read
 // the gate (a launch costs in proportion to its work-groups), and in a step
 // they redo they loop over the tiles.
@@ -185,6 +194,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_rebin_gather(
   const uint32_t nbt = 1u << (g.tx_log2 + g.tr_log2);
   const uint32_t tb0 = rp.bin_base + tile * nbt;
   const uint32_t lo = offset_cur[tb0], nown = offset_cur[tb0 + nbt] - lo;
+  if (gate.bar && item == 0 && t == 0) *gate.bar = 0u;  // for the fallback's grid wait
   uint32_t nlo;
   if (sc.count) {
     // before the tile: the earlier scan tiles' sums and the earlier bins of
@@ -843,11 +853,10 @@ __device__ __forceinline__ void sort_wave_bin(const Geom& g, Buf R, uint32_t s, 
   }
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_sort_bins(
-    const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
-    const uint32_t* __restrict__ offset, Buf T, Gate gate, uint32_t nitems) {
-  if (gate.flag && *gate.flag == 0u) return;  // the gather rebinned this step
-  if (gate.redone && blockIdx.x == 0 && threadIdx.x == 0) *gate.redone += 1u;  // steps redone
+__device__ __forceinline__ void sort_bins_tiles(const RepDev* __restrict__ reps, int nreps,
+                                                const uint32_t* __restrict__ tile_prefix, Buf R,
+                                                const uint32_t* __restrict__ offset, Buf T,
+                                                uint32_t nitems) {
   __shared__ uint64_t s_key[kSortChunk];
   __shared__ uint32_t s_seg[kSortThreads];
   __shared__ uint32_t s_mid[kSortThreads];
@@ -947,6 +956,74 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
       __syncthreads();
     }
   }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_bins(
+    const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
+    const uint32_t* __restrict__ offset, Buf T, Gate gate, uint32_t nitems) {
+  if (gate.flag && *gate.flag == 0u) return;  // the gather rebinned this step
+  if (gate.redone && blockIdx.x == 0 && threadIdx.x == 0) *gate.redone += 1u;  // steps redone
+  sort_bins_tiles(reps, nreps, tile_prefix, R, offset, T, nitems);
+}
+
+// Waits until all `nblocks` work-groups of the grid have arrived at `bar`
+// (zeroed before the launch), with every write the grid made before it
+// visible to every work-group after it.  The fallback's 256 work-groups are
+// all resident at once on this GPU; the wait still ends after kGridWaitTicks
+// of the 100 MHz wall clock, returning false, so no launch can hang on it.
+constexpr uint64_t kGridWaitTicks = 10000000;  // 100 ms
+
+__device__ bool grid_wait(uint32_t* bar, uint32_t nblocks) {
+  __shared__ uint32_t s_ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t t0 = wall_clock64();
+    uint32_t ok = 1u;
+    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < nblocks) {
+      if (wall_clock64() - t0 > kGridWaitTicks) {
+        ok = 0u;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(4);
+    }
+    __threadfence();
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0u;
+}
+
+// The rebinning a gathered step could not do, in one launch: the scatter of
+// every tile, a grid-wide wait, then the bin sort of every tile (T: the
+// sort's scratch).  Returns at once unless the gate is set.  A wait that
+// times out sets bit 2 of err and leaves the bins unsorted.
+__global__ __launch_bounds__(256) void k_rebin_fallback(
+    const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
+    Buf W, const uint32_t* __restrict__ offset_cur, uint32_t* __restrict__ cursor, Buf R2,
+    const uint32_t* __restrict__ offset_new, Buf T, Gate gate, uint32_t* __restrict__ err,
+    uint32_t nitems) {
+  if (*gate.flag == 0u) return;  // the gather rebinned this step
+  if (blockIdx.x == 0 && threadIdx.x == 0) *gate.redone += 1u;  // steps redone
+  scatter_aligned_tiles(reps, nreps, tile_prefix, R, W, offset_cur, cursor, R2, nitems);
+  if (!grid_wait(gate.bar, gridDim.x)) {
+    if (threadIdx.x == 0) atomicOr(err, 2u);
+    return;
+  }
+  sort_bins_tiles(reps, nreps, tile_prefix, R2, offset_new, T, nitems);
+}
+
+hipError_t launch_rebin_fallback(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
+                                 uint32_t* cursor, Buf R2, const uint32_t* offset_new, Buf T,
+                                 const Gate& gate, uint32_t* err, hipStream_t s) {
+  if (wl.ntiles == 0) return hipSuccess;
+  if (!gate.flag || !gate.redone || !gate.bar || !err) return hipErrorInvalidValue;
+  const uint32_t grid = wl.ntiles > kGatedGrid ? kGatedGrid : wl.ntiles;
+  hipLaunchKernelGGL(k_rebin_fallback, dim3(grid), dim3(256), 0, s, wl.reps, wl.nreps,
+                     wl.tile_prefix, R, W, offset_cur, cursor, R2, offset_new, T, gate, err,
+                     wl.ntiles);
+  return hipGetLastError();
 }
 
 hipError_t launch_sort_bins(const WorkList& wl, Buf R, const uint32_t* offset, Buf T,

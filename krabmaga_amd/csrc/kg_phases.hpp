@@ -105,12 +105,14 @@ struct Outbox {
 
 // The gate of a gathered step: the flag a tile that cannot rebin sets (the
 // scatter and the bin sort run only if it is set), the count of steps they
-// redid, and the flag the gather clears for the next step (or none: the
-// scan clears this step's).
+// redid, the flag the gather clears for the next step (or none: the scan
+// clears this step's), and the arrival counter of the fallback's grid wait
+// (the gather zeroes it; or none).
 struct Gate {
   uint32_t* flag = nullptr;
   uint32_t* redone = nullptr;
   uint32_t* clear = nullptr;
+  uint32_t* bar = nullptr;
 };
 
 // The scan done inside the gather, for a layout whose flock step left the

@@ -1,9 +1,11 @@
 """Summaries of rocprofv3 output for the flockers schedule step.
 
 Two device paths run a step.  The three-kernel path (the default for large
-fields) launches k_flock (plus k_flock_global for crowded tiles),
-k_scan_reduce, k_scan_down, k_scatter_aligned and k_sort_bins once each per
-step, one after another on one stream: with --step-kernel kg::k_flock a
+fields) launches k_flock (plus k_flock_global for crowded tiles), the scan
+(k_scan_reduce and k_scan_down, or k_scan_partials, or none when the gather
+does it), then k_rebin_gather and k_rebin_fallback, or k_scatter_aligned and
+k_sort_bins, once each per step, one after another on one stream: with
+--step-kernel kg::k_flock a
 step runs from one k_flock launch to the next, and --fetch / --write sum
 those kernels' bytes per step.  The tile-resident path is described next.
 
@@ -37,7 +39,8 @@ KERNEL = "kg::k_rt_step"
 FLOCK = "kg::k_flock"
 # the kernels of one three-kernel schedule step
 STEP_KERNELS = ("kg::k_flock", "kg::k_flock_global", "kg::k_scan_reduce", "kg::k_scan_down",
-                "kg::k_rebin_gather", "kg::k_scatter_aligned", "kg::k_sort_bins")
+                "kg::k_rebin_gather", "kg::k_rebin_fallback", "kg::k_scatter_aligned",
+                "kg::k_sort_bins")
 # launched within a three-kernel step on some layouts: the tile-sum prefix
 # pass of large scans, the device step counter
 STEP_AUX_KERNELS = ("kg::k_scan_partials", "kg::k_add_u64")
