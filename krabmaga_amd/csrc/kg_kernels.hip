@@ -147,23 +147,20 @@ __global__ __launch_bounds__(256) void k_scatter_aligned(const RepDev* __restric
                                                          Buf R, Buf W,
                                                          const uint32_t* __restrict__ offset_cur,
                                                          uint32_t* __restrict__ cursor, Buf R2,
-                                                         const uint32_t* __restrict__ gate,
                                                          uint32_t nitems) {
-  if (gate && *gate == 0u) return;  // the gather rebinned this step
   scatter_aligned_tiles(reps, nreps, tile_prefix, R, W, offset_cur, cursor, R2, nitems);
 }
 
-// Work-groups of a gated scatter or bin sort: in most steps they only read
-// the gate (a launch costs in proportion to its work-groups), and in a step
-// they redo they loop over the tiles.
+// Work-groups of the gated fallback: in most steps they only read the gate
+// (a launch costs in proportion to its work-groups), and in a step they redo
+// they loop over the tiles.
 constexpr uint32_t kGatedGrid = 256;
 
 hipError_t launch_scatter_aligned(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
-                                  uint32_t* cursor, Buf R2, hipStream_t s, const uint32_t* gate) {
+                                  uint32_t* cursor, Buf R2, hipStream_t s) {
   if (wl.ntiles == 0) return hipSuccess;
-  const uint32_t grid = gate && wl.ntiles > kGatedGrid ? kGatedGrid : wl.ntiles;
-  hipLaunchKernelGGL(k_scatter_aligned, dim3(grid), dim3(256), 0, s, wl.reps, wl.nreps,
-                     wl.tile_prefix, R, W, offset_cur, cursor, R2, gate, wl.ntiles);
+  hipLaunchKernelGGL(k_scatter_aligned, dim3(wl.ntiles), dim3(256), 0, s, wl.reps, wl.nreps,
+                     wl.tile_prefix, R, W, offset_cur, cursor, R2, wl.ntiles);
   return hipGetLastError();
 }
 
@@ -960,9 +957,7 @@ __device__ __forceinline__ void sort_bins_tiles(const RepDev* __restrict__ reps,
 
 __global__ __launch_bounds__(kSortThreads) void k_sort_bins(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
-    const uint32_t* __restrict__ offset, Buf T, Gate gate, uint32_t nitems) {
-  if (gate.flag && *gate.flag == 0u) return;  // the gather rebinned this step
-  if (gate.redone && blockIdx.x == 0 && threadIdx.x == 0) *gate.redone += 1u;  // steps redone
+    const uint32_t* __restrict__ offset, Buf T, uint32_t nitems) {
   sort_bins_tiles(reps, nreps, tile_prefix, R, offset, T, nitems);
 }
 
@@ -1027,11 +1022,10 @@ hipError_t launch_rebin_fallback(const WorkList& wl, Buf R, Buf W, const uint32_
 }
 
 hipError_t launch_sort_bins(const WorkList& wl, Buf R, const uint32_t* offset, Buf T,
-                            hipStream_t s, const Gate& gate) {
+                            hipStream_t s) {
   if (wl.ntiles == 0) return hipSuccess;
-  const uint32_t grid = gate.flag && wl.ntiles > kGatedGrid ? kGatedGrid : wl.ntiles;
-  hipLaunchKernelGGL(k_sort_bins, dim3(grid), dim3(kSortThreads), 0, s, wl.reps, wl.nreps,
-                     wl.tile_prefix, R, offset, T, gate, wl.ntiles);
+  hipLaunchKernelGGL(k_sort_bins, dim3(wl.ntiles), dim3(kSortThreads), 0, s, wl.reps, wl.nreps,
+                     wl.tile_prefix, R, offset, T, wl.ntiles);
   return hipGetLastError();
 }
 

@@ -22,14 +22,13 @@ hipError_t launch_scan(uint32_t* count, uint32_t* offset, uint32_t* cursor, uint
                        const ScanState& ss, uint64_t* gstep, hipStream_t s,
                        uint32_t* gate = nullptr);
 
-// gate (or null): the step's gate flag; the kernel runs only if the gather set it
+// Moves every agent of W (aligned with R) to its new bin's cursor in R2.
 hipError_t launch_scatter_aligned(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
-                                  uint32_t* cursor, Buf R2, hipStream_t s,
-                                  const uint32_t* gate = nullptr);
+                                  uint32_t* cursor, Buf R2, hipStream_t s);
 // The gather rebinning (one work-group per tile; tiles of at most
 // kGatherMaxBins bins) of the flock step that filled the outboxes: R2 in
-// bin order with every bin sorted, or gate.flag set for the scatter and the
-// bin sort to redo it.  sc.count: the scan is done here too (offset_new is
+// bin order with every bin sorted, or gate.flag set for
+// launch_rebin_fallback to redo it.  sc.count: the scan is done here too (offset_new is
 // written); else offset_new holds the scan's offsets.
 hipError_t launch_rebin_gather(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
                                uint32_t* offset_new, const Outbox& ob, Buf R2, const Gate& gate,
@@ -38,10 +37,8 @@ hipError_t launch_scatter_append(const WorkList& wl, Buf A, uint32_t* cursor, Bu
                                  hipStream_t s);
 // Sorts every bin of R (offsets `offset`) by (cell y, id); T is scratch of
 // R's capacity whose contents are lost.
-// gate.flag (or null): as for the scatter; a sort that runs also counts the
-// step in gate.redone
 hipError_t launch_sort_bins(const WorkList& wl, Buf R, const uint32_t* offset, Buf T,
-                            hipStream_t s, const Gate& gate = Gate{});
+                            hipStream_t s);
 // The scatter and the bin sort of a gathered step in one launch, run only if
 // gate.flag is set (gate.redone counts such steps, gate.bar is the grid
 // wait's counter, zeroed by the gather): the scatter as
