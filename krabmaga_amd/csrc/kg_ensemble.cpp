@@ -554,7 +554,6 @@ hipError_t binned_step(kg_ensemble* e, const WorkList& w) {
       sc.tile_sum = e->tsum[p];
       sc.count_next = e->counts(p ^ 1);
       sc.tile_sum_next = e->tsum[p ^ 1];
-      sc.cursor = e->cursor;
       sc.gstep = e->gstep;
       sc.nbins = e->nbins;
       sc.last_item = e->ntiles - 1;
@@ -565,8 +564,9 @@ hipError_t binned_step(kg_ensemble* e, const WorkList& w) {
       return err;
     // what the gather could not rebin, in one launch (the write buffer,
     // consumed by the scatter, is the sort's scratch)
-    if ((err = launch_rebin_fallback(w, e->R.b, e->W.b, e->offset, e->cursor, e->R2.b,
-                                     e->offset2, e->W.b, gt, e->err, e->stream)) != hipSuccess)
+    if ((err = launch_rebin_fallback(w, e->R.b, e->W.b, e->offset,
+                                     summed ? e->counts(p) : e->cursor, e->R2.b, e->offset2,
+                                     e->W.b, gt, e->err, summed, e->stream)) != hipSuccess)
       return err;
   } else {
     if ((err = launch_scatter_aligned(w, e->R.b, e->W.b, e->offset, e->cursor, e->R2.b,

@@ -252,7 +252,6 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
         sc.tile_sum = f->tsum[p];
         sc.count_next = f->counts(p ^ 1);
         sc.tile_sum_next = f->tsum[p ^ 1];
-        sc.cursor = f->cursor;
         sc.gstep = step_gstep;
         sc.nbins = f->g.nbins;
         sc.last_item = f->g.ntiles - 1;
@@ -262,8 +261,9 @@ int lazy_update_impl(kg_field* f, uint64_t* step_gstep) {
       // what the gather could not rebin: scatter and bin sort in one launch
       // (no appended agents in a gathered step); the write buffer has been
       // consumed by then and is the sort's scratch
-      KG_HIP(launch_rebin_fallback(wl, f->R.b, f->W.b, f->offset, f->cursor, f->R2.b,
-                                   f->offset2, f->W.b, gt, f->err, f->stream));
+      KG_HIP(launch_rebin_fallback(wl, f->R.b, f->W.b, f->offset,
+                                   summed ? f->counts(p) : f->cursor, f->R2.b, f->offset2,
+                                   f->W.b, gt, f->err, summed, f->stream));
     } else {
       KG_HIP(launch_scatter_aligned(wl, f->R.b, f->W.b, f->offset, f->cursor, f->R2.b,
                                     f->stream));

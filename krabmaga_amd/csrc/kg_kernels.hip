@@ -84,12 +84,15 @@ hipError_t launch_scatter_append(const WorkList& wl, Buf A, uint32_t* cursor, Bu
 
 // Rebinning of the write buffer the flock kernel produced: it is aligned with
 // the current read buffer, so work item (replica, tile) moves exactly the
-// agents of that tile's old range.
+// agents of that tile's old range.  fill_base null: cursor[b] is bin b's next
+// free slot; else cursor[b] is bin b's count of agents still to place, and
+// they go to fill_base[b] + count - 1 downwards (the count ends at zero).
 __device__ __forceinline__ void scatter_aligned_tiles(const RepDev* __restrict__ reps, int nreps,
                                                       const uint32_t* __restrict__ tile_prefix,
                                                       Buf R, Buf W,
                                                       const uint32_t* __restrict__ offset_cur,
                                                       uint32_t* __restrict__ cursor, Buf R2,
+                                                      const uint32_t* __restrict__ fill_base,
                                                       uint32_t nitems) {
   for (uint32_t bi = blockIdx.x; bi < nitems; bi += gridDim.x) {
     const uint32_t item = xcd_remap(bi, nitems);
@@ -127,7 +130,11 @@ __device__ __forceinline__ void scatter_aligned_tiles(const RepDev* __restrict__
       const uint64_t from_h = ~((1ull << h) - 1ull);
       const uint64_t to_nh = nh == kWave ? ~0ull : ((1ull << nh) - 1ull);
       uint32_t base = 0;
-      if (head) base = atomicAdd(&cursor[bin], (uint32_t)__popcll(vmask & from_h & to_nh));
+      if (head) {
+        const uint32_t run = (uint32_t)__popcll(vmask & from_h & to_nh);
+        base = fill_base ? fill_base[bin] + atomicSub(&cursor[bin], run) - run
+                         : atomicAdd(&cursor[bin], run);
+      }
       base = __shfl(base, h, kWave);
       if (valid) {
         // base plus the valid lanes of the run before this one
@@ -148,7 +155,8 @@ __global__ __launch_bounds__(256) void k_scatter_aligned(const RepDev* __restric
                                                          const uint32_t* __restrict__ offset_cur,
                                                          uint32_t* __restrict__ cursor, Buf R2,
                                                          uint32_t nitems) {
-  scatter_aligned_tiles(reps, nreps, tile_prefix, R, W, offset_cur, cursor, R2, nitems);
+  scatter_aligned_tiles(reps, nreps, tile_prefix, R, W, offset_cur, cursor, R2, nullptr,
+                        nitems);
 }
 
 // Work-groups of the gated fallback: in most steps they only read the gate
@@ -211,7 +219,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_rebin_gather(
     if (t < nbt) {
       s_bstart[t] = ex;
       offset_new[tb0 + t] = nlo + ex;
-      sc.cursor[tb0 + t] = nlo + ex;
+      if (sc.cursor) sc.cursor[tb0 + t] = nlo + ex;
       sc.count_next[tb0 + t] = 0u;
     }
     if (tile == (uint32_t)g.ntiles - 1u) {
@@ -992,16 +1000,19 @@ __device__ bool grid_wait(uint32_t* bar, uint32_t nblocks) {
 
 // The rebinning a gathered step could not do, in one launch: the scatter of
 // every tile, a grid-wide wait, then the bin sort of every tile (T: the
-// sort's scratch).  Returns at once unless the gate is set.  A wait that
-// times out sets bit 2 of err and leaves the bins unsorted.
+// sort's scratch).  Returns at once unless the gate is set.  count_down:
+// cursor holds this step's bin counts, counted down from offset_new (the
+// summed steps' gather writes no cursors).  A wait that times out sets bit
+// 2 of err and leaves the bins unsorted.
 __global__ __launch_bounds__(256) void k_rebin_fallback(
     const RepDev* __restrict__ reps, int nreps, const uint32_t* __restrict__ tile_prefix, Buf R,
     Buf W, const uint32_t* __restrict__ offset_cur, uint32_t* __restrict__ cursor, Buf R2,
     const uint32_t* __restrict__ offset_new, Buf T, Gate gate, uint32_t* __restrict__ err,
-    uint32_t nitems) {
+    int count_down, uint32_t nitems) {
   if (*gate.flag == 0u) return;  // the gather rebinned this step
   if (blockIdx.x == 0 && threadIdx.x == 0) *gate.redone += 1u;  // steps redone
-  scatter_aligned_tiles(reps, nreps, tile_prefix, R, W, offset_cur, cursor, R2, nitems);
+  scatter_aligned_tiles(reps, nreps, tile_prefix, R, W, offset_cur, cursor, R2,
+                        count_down ? offset_new : nullptr, nitems);
   if (!grid_wait(gate.bar, gridDim.x)) {
     if (threadIdx.x == 0) atomicOr(err, 2u);
     return;
@@ -1011,13 +1022,14 @@ __global__ __launch_bounds__(256) void k_rebin_fallback(
 
 hipError_t launch_rebin_fallback(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
                                  uint32_t* cursor, Buf R2, const uint32_t* offset_new, Buf T,
-                                 const Gate& gate, uint32_t* err, hipStream_t s) {
+                                 const Gate& gate, uint32_t* err, bool count_down,
+                                 hipStream_t s) {
   if (wl.ntiles == 0) return hipSuccess;
   if (!gate.flag || !gate.redone || !gate.bar || !err) return hipErrorInvalidValue;
   const uint32_t grid = wl.ntiles > kGatedGrid ? kGatedGrid : wl.ntiles;
   hipLaunchKernelGGL(k_rebin_fallback, dim3(grid), dim3(256), 0, s, wl.reps, wl.nreps,
                      wl.tile_prefix, R, W, offset_cur, cursor, R2, offset_new, T, gate, err,
-                     wl.ntiles);
+                     count_down ? 1 : 0, wl.ntiles);
   return hipGetLastError();
 }
 

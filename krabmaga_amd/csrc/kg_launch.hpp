@@ -42,11 +42,14 @@ hipError_t launch_sort_bins(const WorkList& wl, Buf R, const uint32_t* offset, B
 // The scatter and the bin sort of a gathered step in one launch, run only if
 // gate.flag is set (gate.redone counts such steps, gate.bar is the grid
 // wait's counter, zeroed by the gather): the scatter as
-// launch_scatter_aligned, then the sort of R2's bins by offset_new with T as
-// scratch.  A grid wait that times out sets bit 2 of err.
+// launch_scatter_aligned (count_down: cursor holds the step's bin counts,
+// each counted down to zero from offset_new), then the sort of R2's bins by
+// offset_new with T as scratch.  A grid wait that times out sets bit 2 of
+// err.
 hipError_t launch_rebin_fallback(const WorkList& wl, Buf R, Buf W, const uint32_t* offset_cur,
                                  uint32_t* cursor, Buf R2, const uint32_t* offset_new, Buf T,
-                                 const Gate& gate, uint32_t* err, hipStream_t s);
+                                 const Gate& gate, uint32_t* err, bool count_down,
+                                 hipStream_t s);
 // gflag: one word per tile plus two (the last counts global-path tiles and
 // starts at zero), nagents: R's count; ob: the outboxes for the gather
 // rebinning that follows, or none; inline_global: each global-path tile is

@@ -119,8 +119,9 @@ struct Gate {
 // sums of the scan tiles (whole tiles each; every replica's bins start on a
 // scan tile): this step's bin counts and sums (or, with prefixed, the sums'
 // exclusive prefixes), the next step's (cleared here), the scatter cursors
-// and new offsets to fill, the schedule-step counter to advance, the bin
-// count and the last tile.  count == null: the offsets come from the scan
+// to fill (or none: the fallback counts this step's counts down instead),
+// the new offsets, the schedule-step counter to advance, the bin count and
+// the last tile.  count == null: the offsets come from the scan
 // kernels.
 struct GatherScan {
   uint32_t* count = nullptr;
